@@ -1,0 +1,136 @@
+/*
+ * libjlc_hip: MI355X (gfx950) DGEMM engine behind a plain C ABI.
+ *
+ * This is the native side of the "HIP" kernel choice that plugs into the
+ * reference's GEMM dispatch. Every entry point returns an int status
+ * (JLC_OK == 0), never throws and never aborts; jlc_last_error() returns a
+ * thread-local message for the last failing call on the calling thread.
+ * All entry points are thread-safe: each host thread gets its own HIP stream
+ * and its own staging buffers, so concurrent callers (the reference calls
+ * its GEMM from many ForkJoin threads) never share mutable state.
+ *
+ * Index conventions (shared by every GEMM entry point). C is always row-major:
+ * element (i,j) of C is C[i*ldc + j]. For op(A) (m x k) and op(B) (k x n):
+ *   layout_a == JLC_ROW : op(A)(i,p) = A[i*lda + p]
+ *   layout_a == JLC_COL : op(A)(i,p) = A[p*lda + i]
+ *   layout_b == JLC_ROW : op(B)(p,j) = B[p*ldb + j]
+ *   layout_b == JLC_COL : op(B)(p,j) = B[j*ldb + p]
+ * Semantics are BLAS semantics: C = alpha*op(A)*op(B) + beta*C where beta == 0
+ * never reads C (NaN in C does not survive) and alpha == 0 or k == 0 never
+ * reads A or B (C = beta*C). This is the behaviour of the reference's
+ * MICROKERNEL / SMALL / MATVEC regimes (PackingUtils.scaleCPanel,
+ * src/main/java/net/faulj/compute/PackingUtils.java:159-183;
+ * OptimizedBLAS3.java:190-192; SpecializedKernels.java:32-49).
+ */
+#ifndef JLC_HIP_H
+#define JLC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define JLC_OK 0
+#define JLC_ERR_INVALID_ARG 1   /* null pointer, negative size, bad ld, bad layout/mode */
+#define JLC_ERR_NO_DEVICE 2     /* no usable gfx950 device */
+#define JLC_ERR_DISABLED 3      /* JLC_HIP_ENABLED=0 in the environment */
+#define JLC_ERR_HIP 4           /* a HIP runtime call failed */
+#define JLC_ERR_ALLOC 5         /* device allocation failed */
+#define JLC_ERR_UNSUPPORTED 6   /* valid request this build does not serve */
+#define JLC_ERR_ACCURACY 7      /* split precision cannot meet the requested error bound */
+
+/* Operand layouts. */
+#define JLC_ROW 0
+#define JLC_COL 1
+
+/* Arithmetic modes. The low byte selects the form; for the split forms bits
+ * 8..15 carry the slice count (0 = choose from the inputs' exponent spread). */
+#define JLC_MODE_FP64 0
+#define JLC_MODE_SPLIT_INT8 1
+#define JLC_MODE_SPLIT_BF16 2
+#define JLC_MODE_SLICES(mode_, s_) ((mode_) | ((s_) << 8))
+
+/* Kernel identifiers reported in jlc_stats.kernel. */
+#define JLC_KERNEL_NONE 0
+#define JLC_KERNEL_SCALE 1      /* alpha == 0 or k == 0: C = beta*C */
+#define JLC_KERNEL_MFMA_F64 2   /* FP64 MFMA macro-tile kernel */
+#define JLC_KERNEL_GEMV 3       /* wavefront-reduction matvec */
+#define JLC_KERNEL_SPLIT_INT8 4 /* INT8 slice products, FP64 recombination */
+#define JLC_KERNEL_SPLIT_BF16 5 /* BF16 slice products, FP64 recombination */
+#define JLC_KERNEL_GROUPED 6    /* persistent grouped kernel */
+
+typedef struct jlc_stats {
+    double kernel_ms;   /* device time of the call's kernels (HIP events) */
+    double total_ms;    /* host wall time of the call, transfers included */
+    int32_t kernel;     /* JLC_KERNEL_* that produced C */
+    int32_t tile_m;     /* macro tile chosen */
+    int32_t tile_n;
+    int32_t tile_k;
+    int32_t slices;     /* split-precision slice count (0 for FP64) */
+    int32_t launches;   /* kernel launches issued */
+} jlc_stats;
+
+/* ---- device detection and lifetime (analogue of CudaSupport / CudaContext,
+ * src/main/java/net/faulj/compute/CudaSupport.java:34-123, CudaContext.java:11-55).
+ * Honours JLC_HIP_ENABLED (0 disables), JLC_HIP_FORCE (1: skip the gfx950
+ * check) and HIP_VISIBLE_DEVICES. */
+int jlc_hip_probe(int* device_count, char* name, size_t name_len);
+int jlc_hip_init(int device);
+int jlc_hip_shutdown(void);
+const char* jlc_last_error(void);
+const char* jlc_version(void);
+
+/* ---- GEMM on host memory. Replaces the body of CudaGemm.gemm
+ * (src/main/java/net/faulj/compute/CudaGemm.java:53-135) and serves every
+ * layout of BLAS3Kernels.gemmStrided / gemmStridedTransA / gemmStridedColMajorA /
+ * gemmStridedColMajorB (BLAS3Kernels.java:244, :272, :284, :296) and
+ * OptimizedBLAS3.gemm / gemmStrided (OptimizedBLAS3.java:61, :135).
+ * Operands are staged into per-thread device buffers that persist across
+ * calls; C is updated in place. stats may be NULL. */
+int jlc_dgemm(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+              int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
+              jlc_stats* stats);
+
+/* ---- GEMM on device memory (device residency twin of jlc_dgemm).
+ * stream is a hipStream_t (NULL = the calling thread's library stream). The
+ * call is asynchronous unless stats is non-NULL (then it waits for the
+ * kernels to finish to report their time). */
+int jlc_dgemm_dev(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                  int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
+                  void* stream, jlc_stats* stats);
+
+/* ---- MATVEC: y = alpha*A*x + beta*y, A row-major m x k (lda). Replaces
+ * SpecializedKernels.matvec (src/main/java/net/faulj/compute/SpecializedKernels.java:27-71),
+ * the n == 1 regime of GemmDispatch.selectKernel (GemmDispatch.java:80-82).
+ * incx / incy are element strides of x and y (the B and C columns of an
+ * n == 1 GEMM have strides ldb and ldc). */
+int jlc_dgemv(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x, int64_t incx,
+              double beta, double* y, int64_t incy, jlc_stats* stats);
+int jlc_dgemv_dev(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x,
+                  int64_t incx, double beta, double* y, int64_t incy, void* stream, jlc_stats* stats);
+
+/* ---- device memory helpers for callers that keep operands resident. */
+int jlc_dev_alloc(void** ptr, size_t bytes);
+int jlc_dev_free(void* ptr);
+int jlc_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int jlc_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int jlc_dev_synchronize(void);
+/* The calling thread's library stream (what a NULL stream argument means). */
+int jlc_stream_get(void** stream);
+int jlc_stream_synchronize(void* stream);
+
+/* HIP-event timers for callers that time work on a stream (benchmarks). */
+typedef struct jlc_timer jlc_timer;
+int jlc_timer_create(jlc_timer** timer);
+int jlc_timer_start(jlc_timer* timer, void* stream);
+/* Records the end event on stream, waits for it and returns start->end in ms. */
+int jlc_timer_stop(jlc_timer* timer, void* stream, double* elapsed_ms);
+int jlc_timer_destroy(jlc_timer* timer);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

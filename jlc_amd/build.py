@@ -1,0 +1,93 @@
+"""Builds libjlc_hip.so (gfx950) in-tree with hipcc, and the CPU oracle with make.
+
+The library is a plain C-ABI shared object: no torch or Python symbols, so
+any host (Java FFM, ctypes, C++) can load it. Objects compile in parallel;
+every call recompiles (the build check must exercise the compiler).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIBDIR = PKG / "lib"
+OBJDIR = PKG / "_build"
+LIBNAME = "libjlc_hip.so"
+
+ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -packed-fp32-ops: no packed FP32 instructions are emitted (they have given
+# silent wrong results on these parts); harmless for the host pass.
+HIPFLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", str(ROOT / "include"),
+]
+
+
+def hip_sources() -> list[Path]:
+    return sorted(CSRC.glob("*.hip"))
+
+
+def _compile(src: Path) -> Path:
+    obj = OBJDIR / (src.stem + ".o")
+    cmd = [HIPCC, *HIPFLAGS, "-c", str(src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build_hip(verbose: bool = True) -> Path:
+    OBJDIR.mkdir(exist_ok=True)
+    LIBDIR.mkdir(exist_ok=True)
+    srcs = hip_sources()
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(_compile, srcs))
+    out = LIBDIR / LIBNAME
+    tmp = LIBDIR / (LIBNAME + ".tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"[build] {out} ({len(objs)} objects)", flush=True)
+    return out
+
+
+def build_oracle(verbose: bool = True) -> Path:
+    make = shutil.which("make") or "make"
+    r = subprocess.run([make, "-B", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    out = ROOT / "oracle" / "build" / "liboracle.so"
+    if verbose:
+        print(f"[build] {out}", flush=True)
+    return out
+
+
+def build_all(verbose: bool = True) -> None:
+    build_hip(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    if "hip" in what:
+        build_hip()
+    elif "oracle" in what:
+        build_oracle()
+    else:
+        build_all()

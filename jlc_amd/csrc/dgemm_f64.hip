@@ -1,0 +1,270 @@
+// FP64 MFMA DGEMM for gfx950 (CDNA4).
+//
+// C = alpha * op(A) * op(B) + beta * C, C row-major, op(A) / op(B) row- or
+// column-major (see include/jlc_hip.h). This is the device body of the HIP
+// kernel choice behind the reference's GEMM API; the layout variants fuse the
+// reference's pack routines (PackingUtils.packA alpha fold, java:37-56;
+// BLAS3Kernels.packAStridedTransposed / packAStridedColMajor /
+// packBStridedColMajor, src/main/java/net/faulj/compute/BLAS3Kernels.java:702-787)
+// into the global->LDS staging of each tile, so no separate pack pass runs.
+//
+// Macro tile 128 x 128 x 16 per 256-thread workgroup, four wavefronts in a
+// 2 x 2 grid, each wavefront owning a 64 x 64 block of C as 4 x 4
+// v_mfma_f64_16x16x4 accumulators. Operand tiles live in LDS as [k][m] and
+// [k][n] (128 doubles per k row) in two buffers: while the MFMAs consume one
+// buffer, the next K tile is fetched into registers and written to the other,
+// one barrier per K tile. Column indices inside each 16-wide group are XOR
+// swizzled with (k & 14) so both the transposing stores and the fragment
+// reads are free of LDS bank conflicts without padding (64 KiB per group,
+// two groups per CU).
+//
+// Edges are masked: loads outside [0,M) x [0,K) or [0,K) x [0,N) read as
+// zero and stores outside C are skipped, so any M, N, K >= 1 is valid.
+#include "internal.h"
+
+namespace jlc {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+constexpr int BM = 128;   // rows of C per workgroup
+constexpr int BN = 128;   // cols of C per workgroup
+constexpr int BK = 16;    // K per stage
+constexpr int NT = 256;   // threads per workgroup
+constexpr int SROW = 128; // doubles per k row of a staged tile (BM == BN)
+
+__device__ __forceinline__ int swz(int k, int r) { return r ^ (k & 14); }
+
+// Loads one 128 x 16 operand tile into 4 double pairs per thread.
+// KCONTIG: element (r, kk) at base[r*ld + kk] (k is the contiguous index).
+// !KCONTIG: element (r, kk) at base[kk*ld + r] (r is the contiguous index).
+// rdim / kdim bound r and kk; out-of-range elements read as zero.
+template <bool KCONTIG, bool VEC>
+__device__ __forceinline__ void load_tile(const double* __restrict__ base, int64_t ld, int rdim, int kdim, int r0,
+                                          int k0, int t, d2 (&regs)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int r, kk;
+        if (KCONTIG) {
+            r = r0 + (t >> 3) + 32 * i;
+            kk = k0 + (t & 7) * 2;
+        } else {
+            kk = k0 + (t >> 6) + 4 * i;
+            r = r0 + (t & 63) * 2;
+        }
+        d2 v = {0.0, 0.0};
+        if (KCONTIG) {
+            if (r < rdim) {
+                const double* p = base + (int64_t)r * ld + kk;
+                if (VEC && kk + 1 < kdim) {
+                    v = *reinterpret_cast<const d2*>(p);
+                } else {
+                    if (kk < kdim) v.x = p[0];
+                    if (kk + 1 < kdim) v.y = p[1];
+                }
+            }
+        } else {
+            if (kk < kdim) {
+                const double* p = base + (int64_t)kk * ld + r;
+                if (VEC && r + 1 < rdim) {
+                    v = *reinterpret_cast<const d2*>(p);
+                } else {
+                    if (r < rdim) v.x = p[0];
+                    if (r + 1 < rdim) v.y = p[1];
+                }
+            }
+        }
+        regs[i] = v;
+    }
+}
+
+// Writes the registers of load_tile into a staged [k][r] tile (swizzled).
+template <bool KCONTIG>
+__device__ __forceinline__ void store_tile(double* __restrict__ s, int t, const d2 (&regs)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (KCONTIG) {
+            int r = (t >> 3) + 32 * i;
+            int kk = (t & 7) * 2;
+            s[kk * SROW + swz(kk, r)] = regs[i].x;
+            s[(kk + 1) * SROW + swz(kk + 1, r)] = regs[i].y;
+        } else {
+            int kk = (t >> 6) + 4 * i;
+            int r = (t & 63) * 2;
+            *reinterpret_cast<d2*>(&s[kk * SROW + swz(kk, r)]) = regs[i];
+        }
+    }
+}
+
+// Maps a linear workgroup id to a C tile. Workgroups are dealt to the eight
+// XCDs round-robin; the first step regroups ids so each XCD owns a contiguous
+// run of logical tiles, the second walks tiles in column strips of GROUP tile
+// rows so the workgroups resident on one XCD share A and B panels in its L2.
+__device__ __forceinline__ void tile_coords(int bid, int nblocks, int tiles_m, int tiles_n, int& tm, int& tn) {
+    constexpr int XCDS = 8;
+    constexpr int GROUP = 8;
+    int full = nblocks - nblocks % XCDS;
+    int lid = bid;
+    if (bid < full) lid = (bid % XCDS) * (full / XCDS) + bid / XCDS;
+    int per_group = GROUP * tiles_n;
+    int g = lid / per_group;
+    int first_m = g * GROUP;
+    int gsize = min(tiles_m - first_m, GROUP);
+    int in = lid % per_group;
+    tm = first_m + in % gsize;
+    tn = in / gsize;
+}
+
+template <bool A_COL, bool B_COL, bool VEC>
+__global__ __launch_bounds__(NT) void dgemm_f64_mfma(int M, int N, int K, double alpha,
+                                                     const double* __restrict__ A, int64_t lda,
+                                                     const double* __restrict__ B, int64_t ldb, double beta,
+                                                     double* __restrict__ C, int64_t ldc, int tiles_m,
+                                                     int tiles_n) {
+    __shared__ __attribute__((aligned(16))) double As[2][BK * SROW];
+    __shared__ __attribute__((aligned(16))) double Bs[2][BK * SROW];
+
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wid = t >> 6;
+    const int wm = wid >> 1;
+    const int wn = wid & 1;
+
+    int tm, tn;
+    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * BM;
+    const int n0 = tn * BN;
+
+    // op(A) row-major => k contiguous; op(B) column-major => k contiguous.
+    constexpr bool A_KC = !A_COL;
+    constexpr bool B_KC = B_COL;
+
+    d4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    d2 ra[4], rb[4];
+    load_tile<A_KC, VEC>(A, lda, M, K, m0, 0, t, ra);
+    load_tile<B_KC, VEC>(B, ldb, N, K, n0, 0, t, rb);
+    store_tile<A_KC>(As[0], t, ra);
+    store_tile<B_KC>(Bs[0], t, rb);
+    __syncthreads();
+
+    const int nk = (K + BK - 1) / BK;
+    const int g = lane >> 4;   // k row within a 4-deep MFMA step
+    const int c16 = lane & 15; // column within a 16-wide fragment
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            load_tile<A_KC, VEC>(A, lda, M, K, m0, (kt + 1) * BK, t, ra);
+            load_tile<B_KC, VEC>(B, ldb, N, K, n0, (kt + 1) * BK, t, rb);
+        }
+        const double* as = As[cur];
+        const double* bs = Bs[cur];
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ks++) {
+            const int kr = ks * 4 + g;
+            double af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                af[i] = as[kr * SROW + swz(kr, wm * 64 + i * 16 + c16)];
+                bf[i] = bs[kr * SROW + swz(kr, wn * 64 + i * 16 + c16)];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            store_tile<A_KC>(As[cur ^ 1], t, ra);
+            store_tile<B_KC>(Bs[cur ^ 1], t, rb);
+        }
+        __syncthreads();
+    }
+
+    // Accumulator element r of lane l holds C(4*(l/16) + r, l%16) of its 16 x 16 block.
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int col = n0 + wn * 64 + j * 16 + c16;
+            if (col >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wm * 64 + i * 16 + g * 4 + r;
+                if (row < M) {
+                    double* p = C + (int64_t)row * ldc + col;
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v += beta * *p;
+                    *p = v;
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void scale_kernel(int M, int N, double beta, double* __restrict__ C, int64_t ldc) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)M * N;
+    for (; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = idx / N, j = idx % N;
+        double* p = C + i * ldc + j;
+        *p = (beta == 0.0) ? 0.0 : beta * *p;
+    }
+}
+
+template <bool A_COL, bool B_COL, bool VEC>
+hipError_t launch_variant(int M, int N, int K, double alpha, const double* A, int64_t lda, const double* B,
+                          int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream) {
+    int tiles_m = (M + BM - 1) / BM;
+    int tiles_n = (N + BN - 1) / BN;
+    int64_t nblocks = (int64_t)tiles_m * tiles_n;
+    hipLaunchKernelGGL((dgemm_f64_mfma<A_COL, B_COL, VEC>), dim3((unsigned)nblocks), dim3(NT), 0, stream, M, N, K,
+                       alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_scale(int64_t m, int64_t n, double beta, double* c, int64_t ldc, hipStream_t stream,
+                        TileInfo* info) {
+    if (m <= 0 || n <= 0 || beta == 1.0) return hipSuccess;
+    int64_t total = m * n;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (int)m, (int)n, beta, c, ldc);
+    if (info) info->launches++;
+    return hipGetLastError();
+}
+
+hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha,
+                            const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
+                            int64_t ldc, hipStream_t stream, TileInfo* info) {
+    const bool vec = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) % 16 == 0) &&
+                     (lda % 2 == 0) && (ldb % 2 == 0);
+    const int M = (int)m, N = (int)n, K = (int)k;
+    if (info) {
+        info->tile_m = BM;
+        info->tile_n = BN;
+        info->tile_k = BK;
+        info->launches++;
+    }
+    const int code = (layout_a ? 4 : 0) | (layout_b ? 2 : 0) | (vec ? 1 : 0);
+    switch (code) {
+        case 0: return launch_variant<false, false, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+        case 1: return launch_variant<false, false, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+        case 2: return launch_variant<false, true, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+        case 3: return launch_variant<false, true, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+        case 4: return launch_variant<true, false, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+        case 5: return launch_variant<true, false, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+        case 6: return launch_variant<true, true, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+        default: return launch_variant<true, true, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+    }
+}
+
+}  // namespace jlc

@@ -1,0 +1,513 @@
+// libjlc_hip C ABI: device detection, per-thread streams and staging
+// buffers, argument validation and the host-pointer / device-pointer entry
+// points declared in include/jlc_hip.h.
+//
+// Threading model. The reference calls its GEMM from many threads at once
+// (ThreadLocal workspaces, GemmWorkspace.java:8; ForkJoin tasks,
+// OptimizedBLAS3.java:314-345). Each calling host thread here owns a HIP
+// stream, a pair of timing events and grow-only device staging buffers, so
+// no lock is held while work runs; a process-wide mutex guards only device
+// selection at initialisation.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/jlc_hip.h"
+#include "internal.h"
+
+namespace {
+
+std::mutex g_mutex;
+std::atomic<int> g_device{-1};
+std::atomic<bool> g_alive{true};
+
+thread_local std::string t_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(JLC_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+#define JLC_HIP_CHECK(expr)                                    \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);      \
+    } while (0)
+
+bool env_flag(const char* name, bool dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    if (!std::strcmp(v, "0") || !strcasecmp(v, "false") || !strcasecmp(v, "off") || !strcasecmp(v, "no"))
+        return false;
+    return true;
+}
+
+struct ThreadCtx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void* buf[3] = {nullptr, nullptr, nullptr};
+    size_t cap[3] = {0, 0, 0};
+
+    ~ThreadCtx() {
+        if (!g_alive.load() || device < 0) return;
+        for (int i = 0; i < 3; i++)
+            if (buf[i]) (void)hipFree(buf[i]);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+thread_local ThreadCtx t_ctx;
+
+void mark_dead() { g_alive.store(false); }
+
+int check_arch(int device, char* name, size_t name_len) {
+    hipDeviceProp_t prop;
+    JLC_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0 && !env_flag("JLC_HIP_FORCE", false))
+        return fail(JLC_ERR_NO_DEVICE, "device %d is %s, this library is built for gfx950", device,
+                    prop.gcnArchName);
+    return JLC_OK;
+}
+
+int ensure_init() {
+    if (!env_flag("JLC_HIP_ENABLED", true)) return fail(JLC_ERR_DISABLED, "JLC_HIP_ENABLED disables the HIP path");
+    if (g_device.load() < 0) {
+        int rc = jlc_hip_init(0);
+        if (rc != JLC_OK) return rc;
+    }
+    ThreadCtx& c = t_ctx;
+    int dev = g_device.load();
+    if (c.device != dev) {
+        JLC_HIP_CHECK(hipSetDevice(dev));
+        JLC_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        JLC_HIP_CHECK(hipEventCreate(&c.ev0));
+        JLC_HIP_CHECK(hipEventCreate(&c.ev1));
+        c.device = dev;
+    } else {
+        JLC_HIP_CHECK(hipSetDevice(dev));
+    }
+    return JLC_OK;
+}
+
+int ensure_buf(int slot, size_t bytes) {
+    ThreadCtx& c = t_ctx;
+    if (bytes <= c.cap[slot]) return JLC_OK;
+    if (c.buf[slot]) {
+        JLC_HIP_CHECK(hipFree(c.buf[slot]));
+        c.buf[slot] = nullptr;
+        c.cap[slot] = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&c.buf[slot], want);
+    if (e != hipSuccess) {
+        c.buf[slot] = nullptr;
+        return fail(JLC_ERR_ALLOC, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    c.cap[slot] = want;
+    return JLC_OK;
+}
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Validates a GEMM request (shapes, leading dimensions, layouts, pointers).
+int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const void* a, int64_t lda,
+                  const void* b, int64_t ldb, const void* c, int64_t ldc, int mode) {
+    if (la != JLC_ROW && la != JLC_COL) return fail(JLC_ERR_INVALID_ARG, "layout_a must be JLC_ROW or JLC_COL");
+    if (lb != JLC_ROW && lb != JLC_COL) return fail(JLC_ERR_INVALID_ARG, "layout_b must be JLC_ROW or JLC_COL");
+    if (m < 0 || n < 0 || k < 0) return fail(JLC_ERR_INVALID_ARG, "negative dimension m=%lld n=%lld k=%lld",
+                                             (long long)m, (long long)n, (long long)k);
+    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX)
+        return fail(JLC_ERR_INVALID_ARG, "dimension above 2^31-1");
+    int form = mode & 0xff;
+    if (form != JLC_MODE_FP64 && form != JLC_MODE_SPLIT_INT8 && form != JLC_MODE_SPLIT_BF16)
+        return fail(JLC_ERR_INVALID_ARG, "unknown mode %d", mode);
+    if (m == 0 || n == 0) return JLC_OK;
+    if (!c) return fail(JLC_ERR_INVALID_ARG, "C must not be null");
+    if (ldc < n) return fail(JLC_ERR_INVALID_ARG, "ldc=%lld < n=%lld", (long long)ldc, (long long)n);
+    if (k > 0 && alpha != 0.0) {
+        if (!a || !b) return fail(JLC_ERR_INVALID_ARG, "A and B must not be null");
+        int64_t need_a = la == JLC_ROW ? k : m;
+        int64_t need_b = lb == JLC_ROW ? n : k;
+        if (lda < need_a) return fail(JLC_ERR_INVALID_ARG, "lda=%lld < %lld", (long long)lda, (long long)need_a);
+        if (ldb < need_b) return fail(JLC_ERR_INVALID_ARG, "ldb=%lld < %lld", (long long)ldb, (long long)need_b);
+    }
+    return JLC_OK;
+}
+
+// Device-side dispatch shared by the host and device entry points.
+int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+                    const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
+                    hipStream_t stream, jlc::TileInfo* info, int* kernel) {
+    if (alpha == 0.0 || k == 0) {
+        *kernel = JLC_KERNEL_SCALE;
+        JLC_HIP_CHECK(jlc::launch_scale(m, n, beta, c, ldc, stream, info));
+        return JLC_OK;
+    }
+    int form = mode & 0xff;
+    if (form != JLC_MODE_FP64) return fail(JLC_ERR_UNSUPPORTED, "split-precision modes are not built in this library");
+    if (n == 1 && la == JLC_ROW) {
+        // The n == 1 regime is the matvec (GemmDispatch.selectKernel, GemmDispatch.java:80-82):
+        // x is the B column, y the C column.
+        *kernel = JLC_KERNEL_GEMV;
+        int64_t incx = lb == JLC_ROW ? ldb : 1;
+        JLC_HIP_CHECK(jlc::launch_dgemv(m, k, alpha, a, lda, b, incx, beta, c, ldc, stream, info));
+        return JLC_OK;
+    }
+    *kernel = JLC_KERNEL_MFMA_F64;
+    JLC_HIP_CHECK(jlc::launch_dgemm_f64(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, info));
+    return JLC_OK;
+}
+
+void fill_stats(jlc_stats* s, const jlc::TileInfo& info, int kernel, double kms, double tms) {
+    if (!s) return;
+    s->kernel_ms = kms;
+    s->total_ms = tms;
+    s->kernel = kernel;
+    s->tile_m = info.tile_m;
+    s->tile_n = info.tile_n;
+    s->tile_k = info.tile_k;
+    s->slices = 0;
+    s->launches = info.launches;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* jlc_version(void) { return "libjlc_hip 0.1 gfx950"; }
+
+const char* jlc_last_error(void) { return t_error.c_str(); }
+
+int jlc_hip_probe(int* device_count, char* name, size_t name_len) {
+    if (device_count) *device_count = 0;
+    if (name && name_len) name[0] = '\0';
+    if (!env_flag("JLC_HIP_ENABLED", true)) return fail(JLC_ERR_DISABLED, "JLC_HIP_ENABLED disables the HIP path");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(JLC_ERR_NO_DEVICE, "no HIP device visible");
+    if (device_count) *device_count = count;
+    return check_arch(0, name, name_len);
+}
+
+int jlc_hip_init(int device) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!env_flag("JLC_HIP_ENABLED", true)) return fail(JLC_ERR_DISABLED, "JLC_HIP_ENABLED disables the HIP path");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(JLC_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) return fail(JLC_ERR_INVALID_ARG, "device %d out of range [0,%d)", device, count);
+    int rc = check_arch(device, nullptr, 0);
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipSetDevice(device));
+    static bool registered = false;
+    if (!registered) {
+        std::atexit(mark_dead);
+        registered = true;
+    }
+    g_device.store(device);
+    return JLC_OK;
+}
+
+int jlc_hip_shutdown(void) {
+    ThreadCtx& c = t_ctx;
+    if (c.device >= 0) {
+        for (int i = 0; i < 3; i++) {
+            if (c.buf[i]) (void)hipFree(c.buf[i]);
+            c.buf[i] = nullptr;
+            c.cap[i] = 0;
+        }
+        if (c.stream) (void)hipStreamSynchronize(c.stream);
+    }
+    return JLC_OK;
+}
+
+int jlc_dgemm_dev(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                  int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
+                  void* stream, jlc_stats* stats) {
+    double t0 = now_ms();
+    int rc = validate_gemm(layout_a, layout_b, m, n, k, alpha, a, lda, b, ldb, c, ldc, mode);
+    if (rc != JLC_OK) return rc;
+    jlc::TileInfo info;
+    if (m == 0 || n == 0) {
+        fill_stats(stats, info, JLC_KERNEL_NONE, 0.0, now_ms() - t0);
+        return JLC_OK;
+    }
+    rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    int kernel = JLC_KERNEL_NONE;
+    if (stats) JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    rc = run_gemm_device(layout_a, layout_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode, s, &info, &kernel);
+    if (rc != JLC_OK) return rc;
+    double kms = 0.0;
+    if (stats) {
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev1, s));
+        JLC_HIP_CHECK(hipEventSynchronize(t_ctx.ev1));
+        float f = 0.0f;
+        JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
+        kms = f;
+    }
+    fill_stats(stats, info, kernel, kms, now_ms() - t0);
+    return JLC_OK;
+}
+
+int jlc_dgemm(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+              int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
+              jlc_stats* stats) {
+    double t0 = now_ms();
+    int rc = validate_gemm(layout_a, layout_b, m, n, k, alpha, a, lda, b, ldb, c, ldc, mode);
+    if (rc != JLC_OK) return rc;
+    jlc::TileInfo info;
+    if (m == 0 || n == 0) {
+        fill_stats(stats, info, JLC_KERNEL_NONE, 0.0, now_ms() - t0);
+        return JLC_OK;
+    }
+    rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    ThreadCtx& ctx = t_ctx;
+    hipStream_t s = ctx.stream;
+    const bool use_ab = k > 0 && alpha != 0.0;
+    // Stage operands compactly: op(A) rows (ROW) or columns (COL) become
+    // contiguous records, which is the reference's pack step done by the copy engine.
+    int64_t a_rows = layout_a == JLC_ROW ? m : k, a_cols = layout_a == JLC_ROW ? k : m;
+    int64_t b_rows = layout_b == JLC_ROW ? k : n, b_cols = layout_b == JLC_ROW ? n : k;
+    // Round each device leading dimension up to even so 16-byte loads stay aligned.
+    int64_t dlda = (a_cols + 1) & ~int64_t(1), dldb = (b_cols + 1) & ~int64_t(1), dldc = n;
+    if (use_ab) {
+        if ((rc = ensure_buf(0, (size_t)(a_rows * dlda) * sizeof(double))) != JLC_OK) return rc;
+        if ((rc = ensure_buf(1, (size_t)(b_rows * dldb) * sizeof(double))) != JLC_OK) return rc;
+    }
+    if ((rc = ensure_buf(2, (size_t)(m * dldc) * sizeof(double))) != JLC_OK) return rc;
+    double* da = static_cast<double*>(ctx.buf[0]);
+    double* db = static_cast<double*>(ctx.buf[1]);
+    double* dc = static_cast<double*>(ctx.buf[2]);
+    if (use_ab) {
+        JLC_HIP_CHECK(hipMemcpy2DAsync(da, dlda * sizeof(double), a, lda * sizeof(double), a_cols * sizeof(double),
+                                       a_rows, hipMemcpyHostToDevice, s));
+        JLC_HIP_CHECK(hipMemcpy2DAsync(db, dldb * sizeof(double), b, ldb * sizeof(double), b_cols * sizeof(double),
+                                       b_rows, hipMemcpyHostToDevice, s));
+    }
+    if (beta != 0.0)
+        JLC_HIP_CHECK(hipMemcpy2DAsync(dc, dldc * sizeof(double), c, ldc * sizeof(double), n * sizeof(double), m,
+                                       hipMemcpyHostToDevice, s));
+    int kernel = JLC_KERNEL_NONE;
+    JLC_HIP_CHECK(hipEventRecord(ctx.ev0, s));
+    rc = run_gemm_device(layout_a, layout_b, m, n, k, alpha, da, dlda, db, dldb, beta, dc, dldc, mode, s, &info,
+                         &kernel);
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipEventRecord(ctx.ev1, s));
+    JLC_HIP_CHECK(hipMemcpy2DAsync(c, ldc * sizeof(double), dc, dldc * sizeof(double), n * sizeof(double), m,
+                                   hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    float f = 0.0f;
+    JLC_HIP_CHECK(hipEventElapsedTime(&f, ctx.ev0, ctx.ev1));
+    fill_stats(stats, info, kernel, f, now_ms() - t0);
+    return JLC_OK;
+}
+
+int jlc_dgemv_dev(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x,
+                  int64_t incx, double beta, double* y, int64_t incy, void* stream, jlc_stats* stats) {
+    double t0 = now_ms();
+    if (m < 0 || k < 0) return fail(JLC_ERR_INVALID_ARG, "negative dimension");
+    if (m > INT32_MAX || k > INT32_MAX) return fail(JLC_ERR_INVALID_ARG, "dimension above 2^31-1");
+    jlc::TileInfo info;
+    if (m == 0) {
+        fill_stats(stats, info, JLC_KERNEL_NONE, 0.0, now_ms() - t0);
+        return JLC_OK;
+    }
+    if (!y || incy < 1) return fail(JLC_ERR_INVALID_ARG, "y must not be null and incy >= 1");
+    if (k > 0 && alpha != 0.0 && (!a || !x || incx < 1 || lda < k))
+        return fail(JLC_ERR_INVALID_ARG, "A/x must not be null, incx >= 1, lda >= k");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    if (stats) JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    JLC_HIP_CHECK(jlc::launch_dgemv(m, k, alpha, a, lda, x, incx, beta, y, incy, s, &info));
+    double kms = 0.0;
+    if (stats) {
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev1, s));
+        JLC_HIP_CHECK(hipEventSynchronize(t_ctx.ev1));
+        float f = 0.0f;
+        JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
+        kms = f;
+    }
+    fill_stats(stats, info, JLC_KERNEL_GEMV, kms, now_ms() - t0);
+    return JLC_OK;
+}
+
+int jlc_dgemv(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x, int64_t incx,
+              double beta, double* y, int64_t incy, jlc_stats* stats) {
+    double t0 = now_ms();
+    if (m < 0 || k < 0) return fail(JLC_ERR_INVALID_ARG, "negative dimension");
+    jlc::TileInfo info;
+    if (m == 0) {
+        fill_stats(stats, info, JLC_KERNEL_NONE, 0.0, now_ms() - t0);
+        return JLC_OK;
+    }
+    if (!y || incy < 1) return fail(JLC_ERR_INVALID_ARG, "y must not be null and incy >= 1");
+    const bool use_ax = k > 0 && alpha != 0.0;
+    if (use_ax && (!a || !x || incx < 1 || lda < k))
+        return fail(JLC_ERR_INVALID_ARG, "A/x must not be null, incx >= 1, lda >= k");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    ThreadCtx& ctx = t_ctx;
+    hipStream_t s = ctx.stream;
+    int64_t dlda = (k + 1) & ~int64_t(1);
+    if (use_ax) {
+        if ((rc = ensure_buf(0, (size_t)(m * dlda) * sizeof(double))) != JLC_OK) return rc;
+        if ((rc = ensure_buf(1, (size_t)k * sizeof(double))) != JLC_OK) return rc;
+    }
+    if ((rc = ensure_buf(2, (size_t)m * sizeof(double))) != JLC_OK) return rc;
+    double* da = static_cast<double*>(ctx.buf[0]);
+    double* dx = static_cast<double*>(ctx.buf[1]);
+    double* dy = static_cast<double*>(ctx.buf[2]);
+    if (use_ax) {
+        JLC_HIP_CHECK(hipMemcpy2DAsync(da, dlda * sizeof(double), a, lda * sizeof(double), k * sizeof(double), m,
+                                       hipMemcpyHostToDevice, s));
+        JLC_HIP_CHECK(hipMemcpy2DAsync(dx, sizeof(double), x, incx * sizeof(double), sizeof(double), k,
+                                       hipMemcpyHostToDevice, s));
+    }
+    if (beta != 0.0)
+        JLC_HIP_CHECK(hipMemcpy2DAsync(dy, sizeof(double), y, incy * sizeof(double), sizeof(double), m,
+                                       hipMemcpyHostToDevice, s));
+    JLC_HIP_CHECK(hipEventRecord(ctx.ev0, s));
+    JLC_HIP_CHECK(jlc::launch_dgemv(m, k, alpha, da, dlda, dx, 1, beta, dy, 1, s, &info));
+    JLC_HIP_CHECK(hipEventRecord(ctx.ev1, s));
+    JLC_HIP_CHECK(hipMemcpy2DAsync(y, incy * sizeof(double), dy, sizeof(double), sizeof(double), m,
+                                   hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    float f = 0.0f;
+    JLC_HIP_CHECK(hipEventElapsedTime(&f, ctx.ev0, ctx.ev1));
+    fill_stats(stats, info, JLC_KERNEL_GEMV, f, now_ms() - t0);
+    return JLC_OK;
+}
+
+int jlc_dev_alloc(void** ptr, size_t bytes) {
+    if (!ptr) return fail(JLC_ERR_INVALID_ARG, "ptr must not be null");
+    *ptr = nullptr;
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return JLC_OK;
+}
+
+int jlc_dev_free(void* ptr) {
+    if (!ptr) return JLC_OK;
+    JLC_HIP_CHECK(hipFree(ptr));
+    return JLC_OK;
+}
+
+int jlc_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return JLC_OK;
+}
+
+int jlc_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return JLC_OK;
+}
+
+int jlc_dev_synchronize(void) {
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipStreamSynchronize(t_ctx.stream));
+    JLC_HIP_CHECK(hipDeviceSynchronize());
+    return JLC_OK;
+}
+
+int jlc_stream_get(void** stream) {
+    if (!stream) return fail(JLC_ERR_INVALID_ARG, "stream must not be null");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    *stream = t_ctx.stream;
+    return JLC_OK;
+}
+
+int jlc_stream_synchronize(void* stream) {
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipStreamSynchronize(stream ? static_cast<hipStream_t>(stream) : t_ctx.stream));
+    return JLC_OK;
+}
+
+struct jlc_timer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+int jlc_timer_create(jlc_timer** timer) {
+    if (!timer) return fail(JLC_ERR_INVALID_ARG, "timer must not be null");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    jlc_timer* t = new jlc_timer;
+    hipError_t e = hipEventCreate(&t->start);
+    if (e == hipSuccess) e = hipEventCreate(&t->stop);
+    if (e != hipSuccess) {
+        if (t->start) (void)hipEventDestroy(t->start);
+        delete t;
+        return hip_fail(e, "hipEventCreate");
+    }
+    *timer = t;
+    return JLC_OK;
+}
+
+int jlc_timer_start(jlc_timer* timer, void* stream) {
+    if (!timer) return fail(JLC_ERR_INVALID_ARG, "timer must not be null");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipEventRecord(timer->start, stream ? static_cast<hipStream_t>(stream) : t_ctx.stream));
+    return JLC_OK;
+}
+
+int jlc_timer_stop(jlc_timer* timer, void* stream, double* elapsed_ms) {
+    if (!timer || !elapsed_ms) return fail(JLC_ERR_INVALID_ARG, "timer and elapsed_ms must not be null");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipEventRecord(timer->stop, stream ? static_cast<hipStream_t>(stream) : t_ctx.stream));
+    JLC_HIP_CHECK(hipEventSynchronize(timer->stop));
+    float f = 0.0f;
+    JLC_HIP_CHECK(hipEventElapsedTime(&f, timer->start, timer->stop));
+    *elapsed_ms = f;
+    return JLC_OK;
+}
+
+int jlc_timer_destroy(jlc_timer* timer) {
+    if (!timer) return JLC_OK;
+    if (g_alive.load()) {
+        (void)hipEventDestroy(timer->start);
+        (void)hipEventDestroy(timer->stop);
+    }
+    delete timer;
+    return JLC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+"""ctypes binding of libjlc_hip.so (the C ABI declared in include/jlc_hip.h).
+
+This is the Python analogue of the Java FFM downcall layer a JLC maintainer
+adds: plain pointers and sizes, an int status per call,
+and the thread-local error message turned into an exception. There is no
+fallback: if the shared library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libjlc_hip.so"
+
+JLC_OK = 0
+JLC_ERR_INVALID_ARG = 1
+JLC_ERR_NO_DEVICE = 2
+JLC_ERR_DISABLED = 3
+JLC_ERR_HIP = 4
+JLC_ERR_ALLOC = 5
+JLC_ERR_UNSUPPORTED = 6
+JLC_ERR_ACCURACY = 7
+
+ROW = 0
+COL = 1
+
+MODE_FP64 = 0
+MODE_SPLIT_INT8 = 1
+MODE_SPLIT_BF16 = 2
+
+KERNEL_NAMES = {0: "none", 1: "scale", 2: "mfma_f64", 3: "gemv", 4: "split_int8", 5: "split_bf16", 6: "grouped"}
+
+
+def mode_slices(mode: int, slices: int) -> int:
+    return mode | (slices << 8)
+
+
+class JlcError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"jlc status {status}: {message}")
+        self.status = status
+        self.message = message
+
+
+class NativeLibraryMissing(ImportError):
+    pass
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("kernel", C.c_int32),
+        ("tile_m", C.c_int32),
+        ("tile_n", C.c_int32),
+        ("tile_k", C.c_int32),
+        ("slices", C.c_int32),
+        ("launches", C.c_int32),
+    ]
+
+    @property
+    def kernel_name(self) -> str:
+        return KERNEL_NAMES.get(self.kernel, str(self.kernel))
+
+
+_i64 = C.c_int64
+_d = C.c_double
+_p = C.c_void_p
+_i = C.c_int
+
+_SIGNATURES = {
+    "jlc_hip_probe": (_i, [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "jlc_hip_init": (_i, [_i]),
+    "jlc_hip_shutdown": (_i, []),
+    "jlc_last_error": (C.c_char_p, []),
+    "jlc_version": (C.c_char_p, []),
+    "jlc_dgemm": (_i, [_i, _i, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i, C.POINTER(Stats)]),
+    "jlc_dgemm_dev": (_i, [_i, _i, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i, _p,
+                           C.POINTER(Stats)]),
+    "jlc_dgemv": (_i, [_i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, C.POINTER(Stats)]),
+    "jlc_dgemv_dev": (_i, [_i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p, C.POINTER(Stats)]),
+    "jlc_dev_alloc": (_i, [C.POINTER(_p), C.c_size_t]),
+    "jlc_dev_free": (_i, [_p]),
+    "jlc_memcpy_h2d": (_i, [_p, _p, C.c_size_t]),
+    "jlc_memcpy_d2h": (_i, [_p, _p, C.c_size_t]),
+    "jlc_dev_synchronize": (_i, []),
+    "jlc_stream_get": (_i, [C.POINTER(_p)]),
+    "jlc_stream_synchronize": (_i, [_p]),
+    "jlc_timer_create": (_i, [C.POINTER(_p)]),
+    "jlc_timer_start": (_i, [_p, _p]),
+    "jlc_timer_stop": (_i, [_p, _p, C.POINTER(_d)]),
+    "jlc_timer_destroy": (_i, [_p]),
+}
+
+_lock = threading.Lock()
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Loads libjlc_hip.so once; raises NativeLibraryMissing if it was not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise NativeLibraryMissing(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error() -> str:
+    return (load().jlc_last_error() or b"").decode()
+
+
+def check(rc: int) -> None:
+    if rc != JLC_OK:
+        raise JlcError(rc, last_error())
+
+
+def probe() -> tuple[int, int, str]:
+    """(status, device_count, name); never raises for a missing device."""
+    lib = load()
+    cnt = C.c_int(0)
+    buf = C.create_string_buffer(256)
+    rc = lib.jlc_hip_probe(C.byref(cnt), buf, 256)
+    return rc, cnt.value, buf.value.decode()
+
+
+def _ptr(arr: np.ndarray | None, offset: int = 0) -> int | None:
+    if arr is None:
+        return None
+    if arr.dtype != np.float64:
+        raise TypeError("operands must be float64")
+    if not arr.flags.c_contiguous:
+        raise ValueError("operands must be C-contiguous buffers")
+    if offset < 0 or (arr.size and offset > arr.size):
+        raise ValueError("offset out of range")
+    return arr.ctypes.data + 8 * offset
+
+
+def dgemm(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, a: np.ndarray | None, lda: int,
+          b: np.ndarray | None, ldb: int, beta: float, c: np.ndarray, ldc: int, mode: int = MODE_FP64,
+          a_off: int = 0, b_off: int = 0, c_off: int = 0) -> Stats:
+    """Host-memory GEMM through jlc_dgemm (C updated in place)."""
+    st = Stats()
+    rc = load().jlc_dgemm(layout_a, layout_b, m, n, k, alpha, _ptr(a, a_off), lda, _ptr(b, b_off), ldb, beta,
+                          _ptr(c, c_off), ldc, mode, C.byref(st))
+    check(rc)
+    return st
+
+
+def dgemv(m: int, k: int, alpha: float, a: np.ndarray | None, lda: int, x: np.ndarray | None, incx: int,
+          beta: float, y: np.ndarray, incy: int, a_off: int = 0, x_off: int = 0, y_off: int = 0) -> Stats:
+    st = Stats()
+    rc = load().jlc_dgemv(m, k, alpha, _ptr(a, a_off), lda, _ptr(x, x_off), incx, beta, _ptr(y, y_off), incy,
+                          C.byref(st))
+    check(rc)
+    return st
+
+
+class DeviceArray:
+    """A float64 device allocation owned by the library's allocator."""
+
+    def __init__(self, count: int):
+        self.count = int(count)
+        p = C.c_void_p()
+        check(load().jlc_dev_alloc(C.byref(p), max(1, self.count) * 8))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr: np.ndarray) -> "DeviceArray":
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        d = cls(arr.size)
+        if arr.size:
+            check(load().jlc_memcpy_h2d(d.ptr, arr.ctypes.data, arr.size * 8))
+        return d
+
+    def to_numpy(self, shape=None) -> np.ndarray:
+        out = np.empty(self.count, dtype=np.float64)
+        if self.count:
+            check(load().jlc_memcpy_d2h(out.ctypes.data, self.ptr, self.count * 8))
+        return out.reshape(shape) if shape is not None else out
+
+    def at(self, offset: int) -> int:
+        return self.ptr + 8 * int(offset)
+
+    def free(self) -> None:
+        if self.ptr:
+            check(load().jlc_dev_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) and _lib is not None:
+                _lib.jlc_dev_free(self.ptr)
+        except Exception:
+            pass
+
+
+def dgemm_dev(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, a: int | None, lda: int,
+              b: int | None, ldb: int, beta: float, c: int, ldc: int, mode: int = MODE_FP64, stream: int | None = None,
+              stats: bool = True) -> Stats | None:
+    """Device-pointer GEMM through jlc_dgemm_dev (pointers are ints)."""
+    st = Stats() if stats else None
+    rc = load().jlc_dgemm_dev(layout_a, layout_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode, stream,
+                              C.byref(st) if st is not None else None)
+    check(rc)
+    return st
+
+
+def dgemv_dev(m: int, k: int, alpha: float, a: int | None, lda: int, x: int | None, incx: int, beta: float, y: int,
+              incy: int, stream: int | None = None, stats: bool = True) -> Stats | None:
+    st = Stats() if stats else None
+    rc = load().jlc_dgemv_dev(m, k, alpha, a, lda, x, incx, beta, y, incy, stream,
+                              C.byref(st) if st is not None else None)
+    check(rc)
+    return st
+
+
+def synchronize() -> None:
+    check(load().jlc_dev_synchronize())
+
+
+def stream() -> int:
+    p = C.c_void_p()
+    check(load().jlc_stream_get(C.byref(p)))
+    return p.value
+
+
+class Timer:
+    """HIP-event timer on a library stream."""
+
+    def __init__(self, stream_handle: int | None = None):
+        p = C.c_void_p()
+        check(load().jlc_timer_create(C.byref(p)))
+        self.handle = p.value
+        self.stream = stream_handle
+
+    def start(self) -> None:
+        check(load().jlc_timer_start(self.handle, self.stream))
+
+    def stop(self) -> float:
+        ms = C.c_double(0.0)
+        check(load().jlc_timer_stop(self.handle, self.stream, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and _lib is not None:
+                _lib.jlc_timer_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/*.h (used by the ABI export test)."""
+    import re
+    inc = Path(__file__).resolve().parent.parent / "include"
+    names: list[str] = []
+    for h in sorted(inc.glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for mt in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(jlc_\w+)\s*\(", text, flags=re.M):
+            names.append(mt.group(1))
+    return names
+
+
+def env_disabled() -> bool:
+    v = os.environ.get("JLC_HIP_ENABLED", "")
+    return v.lower() in ("0", "false", "off", "no")

@@ -1,0 +1,465 @@
+/*
+ * CPU oracle for the DGEMM path: TEST INFRASTRUCTURE ONLY (see oracle.h).
+ * Built with -ffp-contract=off so that "a*b + c" is two roundings, as in Java.
+ * Every function cites the reference code it restates; paths are relative to
+ * the reference repository root.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <immintrin.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Seeded inputs: java.util.SplittableRandom (SplitMix64, Steele-Lea-Flood 2014)
+ * as used by GemmReference.seededArray (src/test/java/net/faulj/compute/GemmReference.java:14-25).
+ * nextLong = mix64(seed += GOLDEN_GAMMA); nextDouble = (nextLong >>> 11) * 2^-53;
+ * nextDouble(origin, bound) = r*(bound-origin)+origin, clamped below bound. */
+static const uint64_t GOLDEN_GAMMA = 0x9e3779b97f4a7c15ULL;
+
+static uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+uint64_t oracle_splitmix_next(uint64_t* state) {
+    *state += GOLDEN_GAMMA;
+    return mix64(*state);
+}
+
+static double next_double(uint64_t* state, double origin, double bound) {
+    double r = (double)(oracle_splitmix_next(state) >> 11) * 0x1.0p-53;
+    r = r * (bound - origin) + origin;
+    if (r >= bound) r = nextafter(bound, -INFINITY);
+    return r;
+}
+
+void oracle_uniform_array(double* out, int64_t len, int64_t seed, double origin, double bound) {
+    uint64_t s = (uint64_t)seed;
+    for (int64_t i = 0; i < len; i++) out[i] = next_double(&s, origin, bound);
+}
+
+void oracle_seeded_array(double* out, int64_t len, int64_t seed) {
+    uint64_t s = (uint64_t)seed;
+    for (int64_t i = 0; i < len; i++) {
+        double v = next_double(&s, -1.0, 1.0);
+        out[i] = (i % 11 == 0) ? 0.0 : v;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* GemmReference triple loops (GemmReference.java:31-112): for each (i,j),
+ * sum += a*b over p in increasing order (separate rounding), then
+ * out = alpha*sum + beta*out. Rows are independent, so splitting rows over
+ * threads leaves every result bit-identical to the serial loop. */
+void oracle_gemm_general(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha,
+                         const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                         double* c, int64_t ldc, int threads) {
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int64_t i = 0; i < m; i++) {
+        for (int64_t j = 0; j < n; j++) {
+            double sum = 0.0;
+            for (int64_t p = 0; p < k; p++) {
+                double av = (a_layout == ORACLE_ROW) ? a[i * lda + p] : a[p * lda + i];
+                double bv = (b_layout == ORACLE_ROW) ? b[p * ldb + j] : b[j * ldb + p];
+                sum += av * bv;
+            }
+            double* out = &c[i * ldc + j];
+            *out = alpha * sum + beta * *out;
+        }
+    }
+}
+
+/* GemmReference.gemm (GemmReference.java:31-44): contiguous row-major, C updated in place here. */
+void oracle_gemm(const double* a, const double* b, double* c, int m, int k, int n, double alpha, double beta) {
+    oracle_gemm_general(ORACLE_ROW, ORACLE_ROW, m, n, k, alpha, a, k, b, n, beta, c, n, 1);
+}
+
+/* GemmReference.gemmStrided (GemmReference.java:46-61): a[aOff+i*lda+p], b[bOff+p*ldb+j]. */
+void oracle_gemm_strided(const double* a, int aOff, int lda, const double* b, int bOff, int ldb,
+                         double* c, int cOff, int ldc, int m, int k, int n, double alpha, double beta) {
+    oracle_gemm_general(ORACLE_ROW, ORACLE_ROW, m, n, k, alpha, a + aOff, lda, b + bOff, ldb, beta, c + cOff, ldc, 1);
+}
+
+/* GemmReference.gemmStridedTransA (GemmReference.java:63-78): C is k x n, inner dimension m,
+ * a[aOff+p*lda+i] — the same index map as a column-major op(A) of k rows. */
+void oracle_gemm_strided_trans_a(const double* a, int aOff, int lda, const double* b, int bOff, int ldb,
+                                 double* c, int cOff, int ldc, int m, int k, int n, double alpha, double beta) {
+    oracle_gemm_general(ORACLE_COL, ORACLE_ROW, k, n, m, alpha, a + aOff, lda, b + bOff, ldb, beta, c + cOff, ldc, 1);
+}
+
+/* GemmReference.gemmStridedColMajorA (GemmReference.java:80-95): a[aOff+p*lda+i]. */
+void oracle_gemm_strided_col_major_a(const double* a, int aOff, int lda, const double* b, int bOff, int ldb,
+                                     double* c, int cOff, int ldc, int m, int k, int n, double alpha, double beta) {
+    oracle_gemm_general(ORACLE_COL, ORACLE_ROW, m, n, k, alpha, a + aOff, lda, b + bOff, ldb, beta, c + cOff, ldc, 1);
+}
+
+/* GemmReference.gemmStridedColMajorB (GemmReference.java:97-112): b[bOff+j*ldb+p]. */
+void oracle_gemm_strided_col_major_b(const double* a, int aOff, int lda, const double* b, int bOff, int ldb,
+                                     double* c, int cOff, int ldc, int m, int k, int n, double alpha, double beta) {
+    oracle_gemm_general(ORACLE_ROW, ORACLE_COL, m, n, k, alpha, a + aOff, lda, b + bOff, ldb, beta, c + cOff, ldc, 1);
+}
+
+/* GemmReference.assertParity metrics (GemmReference.java:148-173). */
+void oracle_parity(const double* expected, const double* actual, int64_t len, double* max_abs_err, double* rel_err) {
+    double maxe = 0.0, diff_sum = 0.0, ref_sum = 0.0;
+    for (int64_t i = 0; i < len; i++) {
+        double d = actual[i] - expected[i];
+        double ad = fabs(d);
+        if (ad > maxe || isnan(ad)) maxe = isnan(ad) ? NAN : (isnan(maxe) ? maxe : ad);
+        diff_sum += d * d;
+        ref_sum += expected[i] * expected[i];
+    }
+    double ref_norm = sqrt(ref_sum);
+    *max_abs_err = maxe;
+    *rel_err = ref_norm == 0.0 ? 0.0 : sqrt(diff_sum) / ref_norm;
+}
+
+static double max_abs(const double* v, int64_t len) {
+    double m = 0.0;
+    for (int64_t i = 0; i < len; i++) m = fmax(m, fabs(v[i]));
+    return m;
+}
+
+/* cpuAbsTolerance (GemmReference.java:175-177). */
+double oracle_cpu_abs_tol(const double* expected, int64_t len, int64_t k) {
+    return fmax(1e-12, 1e-9 * fmax(1.0, max_abs(expected, len)) * fmax(1.0, (double)k / 16.0));
+}
+
+/* cudaAbsTolerance (GemmReference.java:179-181): the reference's GPU grade. */
+double oracle_gpu_abs_tol(const double* expected, int64_t len, int64_t k) {
+    return fmax(1e-12, 1e-8 * fmax(1.0, max_abs(expected, len)) * fmax(1.0, (double)k / 16.0));
+}
+
+/* ------------------------------------------------------------------------- */
+/* PARALLEL_MICRO restatement, the timed CPU baseline.
+ * Block sizes: GemmDispatch.computeBlockSizes (GemmDispatch.java:118-149) with
+ * the 32 KiB / 256 KiB / 8 MiB cache constants (:25-27) gives KC=128, NC=128,
+ * MC=2048 for both the AVX2 (MR=4, NR=4) and AVX-512 (MR=6, NR=8) cases
+ * (MicroKernel.optimalMR, MicroKernel.java:29-37). */
+#define PM_KC 128
+#define PM_NC 128
+#define PM_MC 2048
+
+
+static int pm_use_avx512(void) {
+#if defined(__x86_64__)
+    return __builtin_cpu_supports("avx512f");
+#else
+    return 0;
+#endif
+}
+
+void oracle_block_sizes(int* out5) {
+    int wide = pm_use_avx512();
+    out5[0] = PM_MC; out5[1] = PM_KC; out5[2] = PM_NC;
+    out5[3] = wide ? 6 : 4; out5[4] = wide ? 8 : 4;
+}
+
+/* PackingUtils.scaleCPanel (PackingUtils.java:159-183): beta==0 fills zero, beta==1 no-op. */
+static void pm_scale_c(double* c, int64_t m, int64_t n, int64_t ldc, double beta) {
+    if (beta == 1.0) return;
+    for (int64_t i = 0; i < m; i++) {
+        double* row = c + i * ldc;
+        if (beta == 0.0) memset(row, 0, (size_t)n * sizeof(double));
+        else for (int64_t j = 0; j < n; j++) row[j] *= beta;
+    }
+}
+
+/* MicroKernel.computeBlock (MicroKernel.java:94-271): C held in registers for
+ * one NR-wide column strip while an FMA chain runs over the KC panel. */
+__attribute__((target("avx2,fma")))
+static void pm_micro_4x4(int mr, int kb, int packed_n, int actual_n, const double* ap, const double* bp,
+                         double* c, int64_t ldc) {
+    for (int j = 0; j < actual_n; j += 4) {
+        int w = actual_n - j < 4 ? actual_n - j : 4;
+        __m256d acc[4];
+        for (int r = 0; r < mr; r++) {
+            double t[4] = {0, 0, 0, 0};
+            for (int q = 0; q < w; q++) t[q] = c[r * ldc + j + q];
+            acc[r] = _mm256_loadu_pd(t);
+        }
+        for (int p = 0; p < kb; p++) {
+            __m256d bv = _mm256_loadu_pd(bp + (int64_t)p * packed_n + j);
+            for (int r = 0; r < mr; r++)
+                acc[r] = _mm256_fmadd_pd(_mm256_set1_pd(ap[r * kb + p]), bv, acc[r]);
+        }
+        for (int r = 0; r < mr; r++) {
+            double t[4];
+            _mm256_storeu_pd(t, acc[r]);
+            for (int q = 0; q < w; q++) c[r * ldc + j + q] = t[q];
+        }
+    }
+}
+
+__attribute__((target("avx512f")))
+static void pm_micro_6x8(int mr, int kb, int packed_n, int actual_n, const double* ap, const double* bp,
+                         double* c, int64_t ldc) {
+    for (int j = 0; j < actual_n; j += 8) {
+        int w = actual_n - j < 8 ? actual_n - j : 8;
+        __m512d acc[6];
+        for (int r = 0; r < mr; r++) {
+            double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int q = 0; q < w; q++) t[q] = c[r * ldc + j + q];
+            acc[r] = _mm512_loadu_pd(t);
+        }
+        for (int p = 0; p < kb; p++) {
+            __m512d bv = _mm512_loadu_pd(bp + (int64_t)p * packed_n + j);
+            for (int r = 0; r < mr; r++)
+                acc[r] = _mm512_fmadd_pd(_mm512_set1_pd(ap[r * kb + p]), bv, acc[r]);
+        }
+        for (int r = 0; r < mr; r++) {
+            double t[8];
+            _mm512_storeu_pd(t, acc[r]);
+            for (int q = 0; q < w; q++) c[r * ldc + j + q] = t[q];
+        }
+    }
+}
+
+/* OptimizedBLAS3.computeTile (OptimizedBLAS3.java:356-385): per KC block, packB
+ * (PackingUtils.java:121-133, zero padded to NR), then per MR strip packA with
+ * alpha folded (PackingUtils.java:37-56) and the microkernel. */
+static void pm_tile(const double* a, int64_t lda, const double* b, int64_t ldb, double* c, int64_t ldc, int64_t k,
+                    int64_t r0, int64_t r1, int64_t c0, int64_t c1, double alpha, int mr, int nr, int wide,
+                    double* bpack, double* apack) {
+    int nb = (int)(c1 - c0);
+    int packed_n = (nb + nr - 1) / nr * nr;
+    for (int64_t kk = 0; kk < k; kk += PM_KC) {
+        int kb = (int)((kk + PM_KC < k ? kk + PM_KC : k) - kk);
+        for (int p = 0; p < kb; p++) {
+            memcpy(bpack + (int64_t)p * packed_n, b + (kk + p) * ldb + c0, (size_t)nb * sizeof(double));
+            for (int q = nb; q < packed_n; q++) bpack[(int64_t)p * packed_n + q] = 0.0;
+        }
+        for (int64_t i = r0; i < r1; i += mr) {
+            int mb = (int)(r1 - i < mr ? r1 - i : mr);
+            for (int r = 0; r < mb; r++) {
+                const double* src = a + (i + r) * lda + kk;
+                if (alpha == 1.0) memcpy(apack + r * kb, src, (size_t)kb * sizeof(double));
+                else for (int p = 0; p < kb; p++) apack[r * kb + p] = src[p] * alpha;
+            }
+            if (wide) pm_micro_6x8(mb, kb, packed_n, nb, apack, bpack, c + i * ldc + c0, ldc);
+            else pm_micro_4x4(mb, kb, packed_n, nb, apack, bpack, c + i * ldc + c0, ldc);
+        }
+    }
+}
+
+/* GemmDispatch.optimalParallelism (GemmDispatch.java:183-208). */
+static int pm_threads(int64_t m, int64_t n, int64_t k, int max_threads) {
+    if (max_threads <= 1) return 1;
+    int64_t flops = 2 * m * n * k;
+    if (flops < 5000000) return 1;
+    int64_t tiles = ((m + PM_MC - 1) / PM_MC) * ((n + PM_NC - 1) / PM_NC);
+    int64_t useful = flops / 500000;
+    if (useful > tiles) useful = tiles;
+    int64_t t = max_threads < useful ? max_threads : useful;
+    if (t < 1) t = 1;
+    int64_t p = 1;
+    while (p * 2 <= t) p *= 2;
+    return (int)p;
+}
+
+/* OptimizedBLAS3.gemmParallelMicrokernel (OptimizedBLAS3.java:295-351): scale C
+ * by beta, return when alpha==0 or k==0, then one task per MC x NC tile of C. */
+void oracle_parallel_micro(int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+                           const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int max_threads,
+                           int* threads_used) {
+    pm_scale_c(c, m, n, ldc, beta);
+    if (threads_used) *threads_used = 1;
+    if (alpha == 0.0 || k == 0 || m == 0 || n == 0) return;
+    int wide = pm_use_avx512();
+    int mr = wide ? 6 : 4, nr = wide ? 8 : 4;
+    int threads = pm_threads(m, n, k, max_threads);
+    if (threads_used) *threads_used = threads;
+    int64_t brows = (m + PM_MC - 1) / PM_MC, bcols = (n + PM_NC - 1) / PM_NC;
+    int64_t tiles = brows * bcols;
+#pragma omp parallel num_threads(threads)
+    {
+        double* bpack = (double*)aligned_alloc(64, (size_t)PM_KC * (PM_NC + 8) * sizeof(double));
+        double* apack = (double*)aligned_alloc(64, (size_t)8 * PM_KC * sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t t = 0; t < tiles; t++) {
+            int64_t br = t / bcols, bc = t % bcols;
+            int64_t r0 = br * PM_MC, c0 = bc * PM_NC;
+            int64_t r1 = r0 + PM_MC < m ? r0 + PM_MC : m;
+            int64_t c1 = c0 + PM_NC < n ? c0 + PM_NC : n;
+            pm_tile(a, lda, b, ldb, c, ldc, k, r0, r1, c0, c1, alpha, mr, nr, wide, bpack, apack);
+        }
+        free(bpack);
+        free(apack);
+    }
+}
+
+/* SpecializedKernels.matvec (SpecializedKernels.java:27-71): beta-scale y
+ * (beta==0 fills zero), return on alpha==0 or k==0, then y += alpha*dot(row, x). */
+void oracle_matvec(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x,
+                   double beta, double* y) {
+    for (int64_t i = 0; i < m; i++) y[i] = (beta == 0.0) ? 0.0 : (beta == 1.0 ? y[i] : y[i] * beta);
+    if (alpha == 0.0 || k == 0) return;
+    for (int64_t i = 0; i < m; i++) {
+        double s = 0.0;
+        for (int64_t p = 0; p < k; p++) s = fma(a[i * lda + p], x[p], s);
+        y[i] += alpha * s;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Double-double arithmetic: TwoSum (Knuth), TwoProduct via FMA, and the
+ * accurate dd+dd addition (Li et al., "Design, implementation and testing of
+ * extended and mixed precision BLAS", 2002; QD library algorithm). */
+static inline void two_sum(double a, double b, double* s, double* e) {
+    double x = a + b;
+    double bb = x - a;
+    *e = (a - (x - bb)) + (b - bb);
+    *s = x;
+}
+
+static inline void fast_two_sum(double a, double b, double* s, double* e) {
+    double x = a + b;
+    *e = b - (x - a);
+    *s = x;
+}
+
+static inline void dd_add(double ah, double al, double bh, double bl, double* rh, double* rl) {
+    double s1, s2, t1, t2;
+    two_sum(ah, bh, &s1, &s2);
+    two_sum(al, bl, &t1, &t2);
+    s2 += t1;
+    fast_two_sum(s1, s2, &s1, &s2);
+    s2 += t2;
+    fast_two_sum(s1, s2, rh, rl);
+}
+
+static inline void dd_mul_d(double ah, double al, double b, double* rh, double* rl) {
+    double p = ah * b;
+    double e = fma(ah, b, -p);
+    e = fma(al, b, e);
+    fast_two_sum(p, e, rh, rl);
+}
+
+static void dd_dot(int a_layout, int b_layout, int64_t k, const double* a, int64_t lda, const double* b,
+                   int64_t ldb, int64_t i, int64_t j, double* hi, double* lo, double* absum) {
+    double sh = 0.0, sl = 0.0, ab = 0.0;
+    for (int64_t p = 0; p < k; p++) {
+        double av = (a_layout == ORACLE_ROW) ? a[i * lda + p] : a[p * lda + i];
+        double bv = (b_layout == ORACLE_ROW) ? b[p * ldb + j] : b[j * ldb + p];
+        double ph = av * bv;
+        double pl = fma(av, bv, -ph);
+        dd_add(sh, sl, ph, pl, &sh, &sl);
+        ab += fabs(ph);
+    }
+    *hi = sh;
+    *lo = sl;
+    if (absum) *absum = ab * (1.0 + 2.0 * (double)(k + 1) * 0x1.0p-53);
+}
+
+void oracle_dd_entries(int a_layout, int b_layout, int64_t k, const double* a, int64_t lda, const double* b,
+                       int64_t ldb, int64_t count, const int64_t* ii, const int64_t* jj, double* hi, double* lo,
+                       double* abs_out, int threads) {
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 16)
+    for (int64_t t = 0; t < count; t++)
+        dd_dot(a_layout, b_layout, k, a, lda, b, ldb, ii[t], jj[t], &hi[t], &lo[t], abs_out ? &abs_out[t] : NULL);
+}
+
+void oracle_dd_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                    int64_t lda, const double* b, int64_t ldb, double beta, const double* c, int64_t ldc,
+                    double* hi, double* lo, int threads) {
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int64_t i = 0; i < m; i++) {
+        for (int64_t j = 0; j < n; j++) {
+            double sh, sl;
+            dd_dot(a_layout, b_layout, k, a, lda, b, ldb, i, j, &sh, &sl, NULL);
+            double xh, xl, yh = 0.0, yl = 0.0;
+            dd_mul_d(sh, sl, alpha, &xh, &xl);
+            if (beta != 0.0) {
+                double cv = c[i * ldc + j];
+                yh = cv * beta;
+                yl = fma(cv, beta, -yh);
+            }
+            dd_add(xh, xl, yh, yl, &hi[i * n + j], &lo[i * n + j]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Householder QR restatement (src/main/java/net/faulj/decomposition/qr/HouseholderQR.java).
+ * Column-major storage, element (i,j) at a[j*lda+i] (the reference's AT layout, :98-127).
+ * Reflector convention of computeHouseholder (:219-291): max-scaled norm,
+ * beta = -sign(x0)*hypot(x0,||x1||), tau = (beta-x0)/beta, v = x/(x0-beta) with
+ * v0 = 1 implicit, R diagonal = beta. The blocked driver (:133-209) and the
+ * unblocked one give the same factors in exact arithmetic; this restatement
+ * applies each reflector to the trailing columns directly (the panel apply of
+ * applyHouseholderToColumn, :343-385), which is the numerically plain form. */
+static double qr_reflector(int64_t m, double* col, int64_t start, double* tau_out) {
+    int64_t len = m - start;
+    if (len <= 1) { *tau_out = 0.0; return col[start]; }
+    double maxv = fabs(col[start]);
+    for (int64_t i = start + 1; i < m; i++) maxv = fmax(maxv, fabs(col[i]));
+    if (maxv < 2.2250738585072014e-308) { *tau_out = 0.0; return col[start]; }
+    double inv = 1.0 / maxv;
+    double x0 = col[start] * inv;
+    double sigma = 0.0;
+    for (int64_t i = start + 1; i < m; i++) { double v = col[i] * inv; sigma += v * v; }
+    double xnorm = sqrt(sigma);
+    if (xnorm == 0.0) {
+        if (x0 >= 0.0) { *tau_out = 0.0; return col[start]; }
+        *tau_out = 2.0;
+        col[start] = -col[start];
+        return -col[start];
+    }
+    double nbeta = -copysign(hypot(x0, xnorm), x0);
+    double beta = nbeta * maxv;
+    double tau = (nbeta - x0) / nbeta;
+    double v0inv = 1.0 / ((x0 - nbeta) * maxv);
+    for (int64_t i = start + 1; i < m; i++) col[i] *= v0inv;
+    *tau_out = tau;
+    col[start] = beta;
+    return beta;
+}
+
+int oracle_qr_factor(int64_t m, int64_t n, double* a, int64_t lda, double* tau, int64_t nb) {
+    (void)nb;
+    int64_t kmax = m < n ? m : n;
+    for (int64_t k = 0; k < kmax; k++) {
+        double* ck = a + k * lda;
+        qr_reflector(m, ck, k, &tau[k]);
+        if (tau[k] == 0.0) continue;
+        /* H = I - tau v v^T with v = [1; ck[k+1:m]] applied to columns j > k. */
+        for (int64_t j = k + 1; j < n; j++) {
+            double* cj = a + j * lda;
+            double dot = cj[k];
+            for (int64_t i = k + 1; i < m; i++) dot += ck[i] * cj[i];
+            double s = tau[k] * dot;
+            cj[k] -= s;
+            for (int64_t i = k + 1; i < m; i++) cj[i] -= s * ck[i];
+        }
+    }
+    return 0;
+}
+
+/* Q (m x n, thin) by reverse reflector application, as buildQ (:390-444). */
+void oracle_qr_form_q(int64_t m, int64_t n, const double* qr, int64_t lda, const double* tau, double* q,
+                      int64_t ldq) {
+    int64_t kmax = m < n ? m : n;
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t i = 0; i < m; i++) q[j * ldq + i] = (i == j) ? 1.0 : 0.0;
+    for (int64_t k = kmax - 1; k >= 0; k--) {
+        const double* v = qr + k * lda;
+        if (tau[k] == 0.0) continue;
+        for (int64_t j = 0; j < n; j++) {
+            double* qj = q + j * ldq;
+            double dot = qj[k];
+            for (int64_t i = k + 1; i < m; i++) dot += v[i] * qj[i];
+            double s = tau[k] * dot;
+            qj[k] -= s;
+            for (int64_t i = k + 1; i < m; i++) qj[i] -= s * v[i];
+        }
+    }
+}

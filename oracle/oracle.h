@@ -1,0 +1,92 @@
+/*
+ * CPU oracle for the DGEMM path: TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker. Only tests, the smoke check and the benchmark's
+ * CPU-baseline leg may load it; the product path (libjlc_hip.so) never does.
+ *
+ * Contents, each restating the reference algorithm cited beside it:
+ *   - oracle_seeded_array      GemmReference.seededArray      (src/test/java/net/faulj/compute/GemmReference.java:14-25)
+ *   - oracle_gemm_*            GemmReference.gemm* layouts    (GemmReference.java:31-112)
+ *   - oracle_parity            GemmReference.assertParity     (GemmReference.java:148-173)
+ *   - oracle_cpu_abs_tol /     cpuAbsTolerance / cudaAbsTolerance (GemmReference.java:175-181)
+ *     oracle_gpu_abs_tol
+ *   - oracle_parallel_micro    OptimizedBLAS3 PARALLEL_MICRO  (src/main/java/net/faulj/compute/OptimizedBLAS3.java:295-385,
+ *                              GemmDispatch.java:118-149, :183-208, PackingUtils.java:37-56, :121-133, :159-183,
+ *                              MicroKernel.java:61-271)
+ *   - oracle_matvec            SpecializedKernels.matvec      (src/main/java/net/faulj/compute/SpecializedKernels.java:27-71)
+ *   - oracle_dd_*              double-double dot products (Knuth TwoSum, FMA TwoProduct; Ogita-Rump-Oishi 2005).
+ *                              No reference counterpart: it is the accuracy oracle for the split-precision form.
+ *   - oracle_qr_*              HouseholderQR blocked restatement (src/main/java/net/faulj/decomposition/qr/HouseholderQR.java)
+ *
+ * Parity pins (the checks that make a plausible mistake fail) live in tests/test_oracle.py.
+ * SplitMix64 bit-equality with java.util.SplittableRandom is "parity unpinned": no JDK exists
+ * here to produce a vector; parity never depends on it because the oracle and the kernel read
+ * the same arrays.
+ */
+#ifndef JLC_ORACLE_H
+#define JLC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Layout codes shared with the product ABI: element (i,p) of op(A) is
+ * a[i*lda+p] for ROW and a[p*lda+i] for COL; element (p,j) of op(B) is
+ * b[p*ldb+j] for ROW and b[j*ldb+p] for COL. */
+enum { ORACLE_ROW = 0, ORACLE_COL = 1 };
+
+void oracle_seeded_array(double* out, int64_t len, int64_t seed);
+void oracle_uniform_array(double* out, int64_t len, int64_t seed, double origin, double bound);
+uint64_t oracle_splitmix_next(uint64_t* state);
+
+/* General restatement: c (strided, row-major, ldc) <- alpha*op(A)op(B) + beta*c,
+ * sum accumulated left-to-right with separate multiply and add (no FMA). */
+void oracle_gemm_general(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha,
+                         const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                         double* c, int64_t ldc, int threads);
+
+/* Reference-named entry points (same argument order and meaning as GemmReference). */
+void oracle_gemm(const double* a, const double* b, double* c, int m, int k, int n, double alpha, double beta);
+void oracle_gemm_strided(const double* a, int aOff, int lda, const double* b, int bOff, int ldb,
+                         double* c, int cOff, int ldc, int m, int k, int n, double alpha, double beta);
+void oracle_gemm_strided_trans_a(const double* a, int aOff, int lda, const double* b, int bOff, int ldb,
+                                 double* c, int cOff, int ldc, int m, int k, int n, double alpha, double beta);
+void oracle_gemm_strided_col_major_a(const double* a, int aOff, int lda, const double* b, int bOff, int ldb,
+                                     double* c, int cOff, int ldc, int m, int k, int n, double alpha, double beta);
+void oracle_gemm_strided_col_major_b(const double* a, int aOff, int lda, const double* b, int bOff, int ldb,
+                                     double* c, int cOff, int ldc, int m, int k, int n, double alpha, double beta);
+
+void oracle_parity(const double* expected, const double* actual, int64_t len, double* max_abs_err, double* rel_err);
+double oracle_cpu_abs_tol(const double* expected, int64_t len, int64_t k);
+double oracle_gpu_abs_tol(const double* expected, int64_t len, int64_t k);
+
+/* PARALLEL_MICRO restatement (the timed CPU baseline). Row-major A (lda), B (ldb), C (ldc). */
+void oracle_parallel_micro(int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+                           const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int max_threads,
+                           int* threads_used);
+/* Block sizes the restatement uses: mc, kc, nc, mr, nr. */
+void oracle_block_sizes(int* out5);
+
+void oracle_matvec(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x,
+                   double beta, double* y);
+
+/* Double-double dot products for selected entries (i,j) of op(A)op(B): hi+lo ~ exact sum
+ * with relative error ~ k*2^-104. abs_out (optional) receives sum |a||b| (rounded up slightly). */
+void oracle_dd_entries(int a_layout, int b_layout, int64_t k, const double* a, int64_t lda, const double* b,
+                       int64_t ldb, int64_t count, const int64_t* ii, const int64_t* jj, double* hi, double* lo,
+                       double* abs_out, int threads);
+/* Double-double full GEMM (small sizes): hi/lo arrays m*n row-major of alpha*AB + beta*C evaluated in dd. */
+void oracle_dd_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                    int64_t lda, const double* b, int64_t ldb, double beta, const double* c, int64_t ldc,
+                    double* hi, double* lo, int threads);
+
+/* Blocked Householder QR restatement (column-major in/out). */
+int oracle_qr_factor(int64_t m, int64_t n, double* a_colmajor, int64_t lda, double* tau, int64_t nb);
+void oracle_qr_form_q(int64_t m, int64_t n, const double* qr_colmajor, int64_t lda, const double* tau,
+                      double* q_colmajor, int64_t ldq);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
