@@ -187,7 +187,8 @@ __global__ __launch_bounds__(NT) void dgemm_f64_mfma(int M, int N, int K, double
         __syncthreads();
     }
 
-    // Accumulator element r of lane l holds C(4*(l/16) + r, l%16) of its 16 x 16 block.
+    // v_mfma_f64_16x16x4 accumulator layout: element r of lane l holds C(l/16 + 4*r, l%16)
+    // of its 16 x 16 block.
 #pragma unroll
     for (int i = 0; i < 4; i++) {
 #pragma unroll
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(NT) void dgemm_f64_mfma(int M, int N, int K, double
             if (col >= N) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm * 64 + i * 16 + g * 4 + r;
+                const int row = m0 + wm * 64 + i * 16 + g + 4 * r;
                 if (row < M) {
                     double* p = C + (int64_t)row * ldc + col;
                     double v = alpha * acc[i][j][r];
