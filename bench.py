@@ -1,0 +1,214 @@
+"""DGEMM 16384^3 benchmark on one MI355X: FP64 MFMA (and the split-precision
+form when requested), with the error of every timed form stated against a
+double-double oracle on sampled entries, the kernel's roofline fraction, and
+the CPU baseline (a restatement of the reference's PARALLEL_MICRO path) timed
+on this machine's host cores.
+
+Workload: C = A*B, M = N = K = 16384, FP64, row-major, with
+a_ij = (u - 0.5) * exp(phi * g), u ~ U[0,1), g ~ N(0,1) (same for B), phi = 0.5
+by default. Inputs are generated on the host from a fixed seed, copied to HBM
+once, and are resident when the timed region starts. A step is one full GEMM.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--phi P] [--size N]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from jlc_amd import native  # noqa: E402
+
+FP64_MFMA_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+METRIC = "DGEMM 16384³ TFLOP/s at stated max rel err: split-MFMA vs FP64 MFMA"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_operand(rng: np.random.Generator, rows: int, cols: int, phi: float) -> np.ndarray:
+    """a_ij = (u - 0.5) * exp(phi * g), generated in row chunks to bound temporaries."""
+    out = np.empty(rows * cols, dtype=np.float64)
+    chunk = max(1, (1 << 24) // cols)
+    for r0 in range(0, rows, chunk):
+        r1 = min(rows, r0 + chunk)
+        n = (r1 - r0) * cols
+        u = rng.random(n)
+        g = rng.standard_normal(n)
+        np.multiply(u - 0.5, np.exp(phi * g), out=out[r0 * cols:r1 * cols])
+    return out
+
+
+def sampled_error(a: np.ndarray, b: np.ndarray, c: np.ndarray, m: int, n: int, k: int, rows, cols) -> dict:
+    """Max relative and componentwise error of C on a rows x cols grid against dd sums."""
+    from oracle import oracle as orc
+    A = a.reshape(m, k)[rows, :].copy().reshape(-1)
+    B = np.ascontiguousarray(b.reshape(k, n)[:, cols]).reshape(-1)
+    ii, jj = np.meshgrid(np.arange(len(rows)), np.arange(len(cols)), indexing="ij")
+    hi, lo, absum = orc.dd_entries(0, 0, k, A, k, B, len(cols), ii.ravel(), jj.ravel())
+    exact = hi + lo
+    got = c.reshape(m, n)[np.ix_(rows, cols)].reshape(-1)
+    err = np.abs(got - hi - lo)
+    nz = exact != 0.0
+    rel = float(np.max(err[nz] / np.abs(exact[nz]))) if nz.any() else 0.0
+    u = 2.0 ** -53
+    comp = float(np.max(err / np.maximum(absum, 1e-300)) / u)
+    return {"max_rel_err": rel, "max_componentwise_err_units_u": comp, "entries": int(err.size),
+            "median_rel_err": float(np.median(err[nz] / np.abs(exact[nz]))) if nz.any() else 0.0}
+
+
+def read_traffic(kernel_tag: str) -> float | None:
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    p = ROOT / "profiles" / f"{kernel_tag}_traffic.txt"
+    if not p.exists():
+        return None
+    m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), flags=re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(max_seconds: float = 12.0) -> dict:
+    """PARALLEL_MICRO restatement (oracle) on a bounded sample of the same workload:
+    a 4096^3 FP64 GEMM with the same generator, median of the timed repeats."""
+    from oracle import oracle as orc
+    threads = min(16, os.cpu_count() or 1)
+    s = 4096
+    rng = np.random.default_rng(7)
+    a = make_operand(rng, s, s, 0.5)
+    b = make_operand(rng, s, s, 0.5)
+    c = np.zeros(s * s)
+    used = orc.parallel_micro(s, s, s, 1.0, a, s, b, s, 0.0, c, s, threads)  # warm-up
+    times = []
+    t_end = time.perf_counter() + max_seconds
+    while len(times) < 5 or (time.perf_counter() < t_end and len(times) < 20):
+        t0 = time.perf_counter()
+        orc.parallel_micro(s, s, s, 1.0, a, s, b, s, 0.0, c, s, threads)
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 5 and time.perf_counter() > t_end:
+            break
+    med = float(np.median(times))
+    return {"value": round(2.0 * s ** 3 / med / 1e12, 5), "unit": "TFLOP/s", "cores": used, "kind": "port",
+            "sample": f"PARALLEL_MICRO restatement (oracle/oracle.c), {s}^3 FP64 same generator, "
+                      f"median of {len(times)} runs, {used} threads of {os.cpu_count()} host cores"}
+
+
+def time_form(step, warmup: int, steps: int, stream: int) -> tuple[float, float]:
+    """(wall ms per step, device ms per step) for steps timed launches after warmup."""
+    for _ in range(warmup):
+        step()
+    native.synchronize()
+    timer = native.Timer(stream)
+    t0 = time.perf_counter()
+    timer.start()
+    for _ in range(steps):
+        step()
+    dev_ms = timer.stop()
+    native.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    return wall_ms / steps, dev_ms / steps
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--phi", type=float, default=0.5)
+    ap.add_argument("--size", type=int, default=16384)
+    ap.add_argument("--seed", type=int, default=20260603)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the copy kernel over MIB MiB three times (byte-counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        return 2
+    rc, count, name = native.probe()
+    if rc != native.JLC_OK:
+        print(f"no usable device: {native.last_error()}", file=sys.stderr)
+        return 1
+    log(f"device {name}")
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src, dst = native.DeviceArray(nbytes // 8), native.DeviceArray(nbytes // 8)
+        for _ in range(3):
+            native.dev_copy(dst.ptr, src.ptr, nbytes)
+        native.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}), flush=True)
+        return 0
+
+    m = n = k = args.size
+    rng = np.random.default_rng(args.seed)
+    t0 = time.perf_counter()
+    a = make_operand(rng, m, k, args.phi)
+    b = make_operand(rng, k, n, args.phi)
+    log(f"generated inputs in {time.perf_counter() - t0:.1f}s")
+    da, db = native.DeviceArray.from_numpy(a), native.DeviceArray.from_numpy(b)
+    dc = native.DeviceArray(m * n)
+    stream = native.stream()
+
+    def fp64_step():
+        native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, dc.ptr, n,
+                         native.MODE_FP64, stream, stats=False)
+
+    wall_ms, dev_ms = time_form(fp64_step, args.warmup, args.steps, stream)
+    flops = 2.0 * m * n * k
+    value = flops / (wall_ms * 1e-3) / 1e12
+    achieved = flops / (dev_ms * 1e-3) / 1e12
+    log(f"fp64 mfma: {wall_ms:.2f} ms/step wall, {dev_ms:.2f} ms device, {value:.2f} TFLOP/s")
+
+    result_err = None
+    if not args.no_error:
+        c = dc.to_numpy()
+        srng = np.random.default_rng(args.seed + 1)
+        rows = np.sort(srng.choice(m, 64, replace=False))
+        cols = np.sort(srng.choice(n, 64, replace=False))
+        result_err = sampled_error(a, b, c, m, n, k, rows, cols)
+        log(f"fp64 error vs dd oracle: {result_err}")
+        del c
+
+    traffic = read_traffic("dgemm_f64_mfma_16384")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"dgemm {m}x{n}x{k} row-major, a=(u-0.5)*exp(phi*g), phi={args.phi}",
+                   "M": m, "N": n, "K": k, "alpha": 1.0, "beta": 0.0, "phi": args.phi, "form": "fp64_mfma",
+                   "parallelism": "single_gpu"},
+        "error": result_err,
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
+                     "traffic": traffic, "kernel": "dgemm_f64_mfma", "kernel_ms": round(dev_ms, 3)},
+    }
+    if not args.no_cpu_baseline:
+        log("timing CPU baseline")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

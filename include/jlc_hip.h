@@ -118,6 +118,8 @@ int jlc_dev_free(void* ptr);
 int jlc_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int jlc_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int jlc_dev_synchronize(void);
+/* Device-to-device copy by a kernel on stream (NULL = the thread's stream), asynchronous. */
+int jlc_dev_copy(void* dst, const void* src, size_t bytes, void* stream);
 /* The calling thread's library stream (what a NULL stream argument means). */
 int jlc_stream_get(void** stream);
 int jlc_stream_synchronize(void* stream);

@@ -10,7 +10,7 @@
 //
 // Macro tile 128 x 128 x 16 per 256-thread workgroup, four wavefronts in a
 // 2 x 2 grid, each wavefront owning a 64 x 64 block of C as 4 x 4
-// v_mfma_f64_16x16x4 accumulators. Operand tiles live in LDS as [k][m] and
+// v_mfma_f64_16x16x4 accumulators; two workgroups resident per CU. Operand tiles live in LDS as [k][m] and
 // [k][n] (128 doubles per k row) in two buffers: while the MFMAs consume one
 // buffer, the next K tile is fetched into registers and written to the other,
 // one barrier per K tile. Column indices inside each 16-wide group are XOR
@@ -97,6 +97,27 @@ __device__ __forceinline__ void store_tile(double* __restrict__ s, int t, const 
     }
 }
 
+// Runs the 4-deep MFMA steps [ks0, ks1) of one staged K tile on a wavefront's
+// 4 x 4 accumulator blocks (rows wm*64.., cols wn*64..).
+__device__ __forceinline__ void mfma_steps(const double* __restrict__ as, const double* __restrict__ bs, int ks0,
+                                           int ks1, int wm, int wn, int g, int c16, d4 (&acc)[4][4]) {
+#pragma unroll
+    for (int ks = ks0; ks < ks1; ks++) {
+        const int kr = ks * 4 + g;
+        double af[4], bf[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            af[i] = as[kr * SROW + swz(kr, wm * 64 + i * 16 + c16)];
+            bf[i] = bs[kr * SROW + swz(kr, wn * 64 + i * 16 + c16)];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+}
+
 // Maps a linear workgroup id to a C tile. Workgroups are dealt to the eight
 // XCDs round-robin; the first step regroups ids so each XCD owns a contiguous
 // run of logical tiles, the second walks tiles in column strips of GROUP tile
@@ -116,8 +137,10 @@ __device__ __forceinline__ void tile_coords(int bid, int nblocks, int tiles_m, i
     tn = in / gsize;
 }
 
+// Two wavefronts per SIMD (two workgroups per CU) hide the barrier and LDS
+// latency of one group behind the other's MFMAs; the register budget is 256.
 template <bool A_COL, bool B_COL, bool VEC>
-__global__ __launch_bounds__(NT) void dgemm_f64_mfma(int M, int N, int K, double alpha,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void dgemm_f64_mfma(int M, int N, int K, double alpha,
                                                      const double* __restrict__ A, int64_t lda,
                                                      const double* __restrict__ B, int64_t ldb, double beta,
                                                      double* __restrict__ C, int64_t ldc, int tiles_m,
@@ -156,36 +179,22 @@ __global__ __launch_bounds__(NT) void dgemm_f64_mfma(int M, int N, int K, double
     const int nk = (K + BK - 1) / BK;
     const int g = lane >> 4;   // k row within a 4-deep MFMA step
     const int c16 = lane & 15; // column within a 16-wide fragment
-    for (int kt = 0; kt < nk; kt++) {
+    // Branch-free steady state: every iteration but the last prefetches the
+    // next K tile into registers, runs half of the current tile's MFMAs,
+    // writes the prefetched tile to the idle LDS buffer, then runs the other
+    // half, so the LDS writes sit between independent MFMAs.
+    int kt = 0;
+    for (; kt < nk - 1; kt++) {
         const int cur = kt & 1;
-        const bool more = kt + 1 < nk;
-        if (more) {
-            load_tile<A_KC, VEC>(A, lda, M, K, m0, (kt + 1) * BK, t, ra);
-            load_tile<B_KC, VEC>(B, ldb, N, K, n0, (kt + 1) * BK, t, rb);
-        }
-        const double* as = As[cur];
-        const double* bs = Bs[cur];
-#pragma unroll
-        for (int ks = 0; ks < BK / 4; ks++) {
-            const int kr = ks * 4 + g;
-            double af[4], bf[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                af[i] = as[kr * SROW + swz(kr, wm * 64 + i * 16 + c16)];
-                bf[i] = bs[kr * SROW + swz(kr, wn * 64 + i * 16 + c16)];
-            }
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
-        }
-        if (more) {
-            store_tile<A_KC>(As[cur ^ 1], t, ra);
-            store_tile<B_KC>(Bs[cur ^ 1], t, rb);
-        }
+        load_tile<A_KC, VEC>(A, lda, M, K, m0, (kt + 1) * BK, t, ra);
+        load_tile<B_KC, VEC>(B, ldb, N, K, n0, (kt + 1) * BK, t, rb);
+        mfma_steps(As[cur], Bs[cur], 0, BK / 8, wm, wn, g, c16, acc);
+        store_tile<A_KC>(As[cur ^ 1], t, ra);
+        store_tile<B_KC>(Bs[cur ^ 1], t, rb);
+        mfma_steps(As[cur], Bs[cur], BK / 8, BK / 4, wm, wn, g, c16, acc);
         __syncthreads();
     }
+    mfma_steps(As[kt & 1], Bs[kt & 1], 0, BK / 4, wm, wn, g, c16, acc);
 
     // v_mfma_f64_16x16x4 accumulator layout: element r of lane l holds C(l/16 + 4*r, l%16)
     // of its 16 x 16 block.

@@ -25,4 +25,7 @@ hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, in
 hipError_t launch_dgemv(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x,
                         int64_t incx, double beta, double* y, int64_t incy, hipStream_t stream, TileInfo* info);
 
+// Device-to-device copy kernel (bytes may be any size).
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 }  // namespace jlc

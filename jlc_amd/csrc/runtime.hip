@@ -445,6 +445,15 @@ int jlc_dev_synchronize(void) {
     return JLC_OK;
 }
 
+int jlc_dev_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes && (!dst || !src)) return fail(JLC_ERR_INVALID_ARG, "dst and src must not be null");
+    if (bytes == 0) return JLC_OK;
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(jlc::launch_copy(dst, src, bytes, stream ? static_cast<hipStream_t>(stream) : t_ctx.stream));
+    return JLC_OK;
+}
+
 int jlc_stream_get(void** stream) {
     if (!stream) return fail(JLC_ERR_INVALID_ARG, "stream must not be null");
     int rc = ensure_init();

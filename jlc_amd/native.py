@@ -89,6 +89,7 @@ _SIGNATURES = {
     "jlc_memcpy_h2d": (_i, [_p, _p, C.c_size_t]),
     "jlc_memcpy_d2h": (_i, [_p, _p, C.c_size_t]),
     "jlc_dev_synchronize": (_i, []),
+    "jlc_dev_copy": (_i, [_p, _p, C.c_size_t, _p]),
     "jlc_stream_get": (_i, [C.POINTER(_p)]),
     "jlc_stream_synchronize": (_i, [_p]),
     "jlc_timer_create": (_i, [C.POINTER(_p)]),
@@ -228,6 +229,10 @@ def dgemv_dev(m: int, k: int, alpha: float, a: int | None, lda: int, x: int | No
                               C.byref(st) if st is not None else None)
     check(rc)
     return st
+
+
+def dev_copy(dst: int, src: int, nbytes: int, stream_handle: int | None = None) -> None:
+    check(load().jlc_dev_copy(dst, src, nbytes, stream_handle))
 
 
 def synchronize() -> None:
