@@ -10,6 +10,7 @@ namespace jlc {
 struct TileInfo {
     int tile_m = 0, tile_n = 0, tile_k = 0;
     int launches = 0;
+    int slices = 0;
 };
 
 // C = beta*C on an m x n row-major block (beta == 0 writes zeros, C unread).
@@ -24,6 +25,36 @@ hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, in
 // y = alpha*A*x + beta*y, A row-major. Requires m > 0; k == 0 or alpha == 0 handled inside.
 hipError_t launch_dgemv(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x,
                         int64_t incx, double beta, double* y, int64_t incy, hipStream_t stream, TileInfo* info);
+
+constexpr int JLC_LAYOUT_ROW = 0;
+constexpr int JLC_LAYOUT_COL = 1;
+
+// Geometry and workspace of one split-precision (INT8 slice) GEMM.
+struct SplitPlan {
+    int64_t m = 0, n = 0, k = 0;
+    int64_t mpad = 0, npad = 0, kpad = 0;  // padded to the C tile (128 x 64) and the MFMA depth (32)
+    int slices = 0;                        // S, 3..9
+    int64_t a_slice_bytes = 0;             // bytes of one packed slice of op(A) (mpad * kpad)
+    int64_t b_slice_bytes = 0;             // bytes of one packed slice of op(B) (npad * kpad)
+    void* a_pack = nullptr;                // S * a_slice_bytes
+    void* b_pack = nullptr;                // S * b_slice_bytes
+    int* ea = nullptr;                     // mpad row exponents of op(A)
+    int* eb = nullptr;                     // npad column exponents of op(B)
+    float* spread_a = nullptr;             // mpad
+    float* spread_b = nullptr;             // npad
+};
+
+constexpr int SPLIT_MIN_SLICES = 3;
+constexpr int SPLIT_MAX_SLICES = 9;
+
+void split_plan_sizes(int64_t m, int64_t n, int64_t k, int slices, SplitPlan* p);
+// Row exponents / spreads of op(A) and column exponents / spreads of op(B).
+hipError_t launch_split_stats(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
+                              SplitPlan* p, hipStream_t stream);
+// Slices both operands and runs the slice-product GEMM with FP64 recombination into C.
+hipError_t launch_split_pack_and_gemm(int layout_a, int layout_b, const double* a, int64_t lda, const double* b,
+                                      int64_t ldb, const SplitPlan& p, double alpha, double beta, double* c,
+                                      int64_t ldc, hipStream_t stream, TileInfo* info);
 
 // Device-to-device copy kernel (bytes may be any size).
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);

@@ -17,7 +17,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <cmath>
 #include <string>
+#include <vector>
 
 #include "../../include/jlc_hip.h"
 #include "internal.h"
@@ -62,12 +64,14 @@ struct ThreadCtx {
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    void* buf[3] = {nullptr, nullptr, nullptr};
-    size_t cap[3] = {0, 0, 0};
+    // Staging: 0 = A, 1 = B, 2 = C; split workspace: 3 = exponents/spreads, 4 = A slices, 5 = B slices.
+    static constexpr int NBUF = 6;
+    void* buf[NBUF] = {};
+    size_t cap[NBUF] = {};
 
     ~ThreadCtx() {
         if (!g_alive.load() || device < 0) return;
-        for (int i = 0; i < 3; i++)
+        for (int i = 0; i < NBUF; i++)
             if (buf[i]) (void)hipFree(buf[i]);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -146,6 +150,12 @@ int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha,
     int form = mode & 0xff;
     if (form != JLC_MODE_FP64 && form != JLC_MODE_SPLIT_INT8 && form != JLC_MODE_SPLIT_BF16)
         return fail(JLC_ERR_INVALID_ARG, "unknown mode %d", mode);
+    int slices = (mode >> 8) & 0xff;
+    if (form == JLC_MODE_FP64 && slices != 0) return fail(JLC_ERR_INVALID_ARG, "slice count given for FP64 mode");
+    if (form == JLC_MODE_SPLIT_INT8 && slices != 0 &&
+        (slices < jlc::SPLIT_MIN_SLICES || slices > jlc::SPLIT_MAX_SLICES))
+        return fail(JLC_ERR_INVALID_ARG, "INT8 slice count %d outside [%d, %d]", slices, jlc::SPLIT_MIN_SLICES,
+                    jlc::SPLIT_MAX_SLICES);
     if (m == 0 || n == 0) return JLC_OK;
     if (!c) return fail(JLC_ERR_INVALID_ARG, "C must not be null");
     if (ldc < n) return fail(JLC_ERR_INVALID_ARG, "ldc=%lld < n=%lld", (long long)ldc, (long long)n);
@@ -159,6 +169,56 @@ int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha,
     return JLC_OK;
 }
 
+// Slice count for the INT8 split form from the inputs' exponent spread. The
+// truncation error of S 7-bit slices is about 2^(ea+eb-7S) per term while an
+// FP64 product-sum errs by about 2^-53 of the terms' typical size; spread_a /
+// spread_b are the largest row / column gaps (in bits) between the scale 2^e
+// and the mean |x|, so matching FP64 needs 7S >= 53 + spread_a + spread_b - 4.
+int choose_slices(const float* spread_a, int64_t m, const float* spread_b, int64_t n) {
+    float sa = 0.0f, sb = 0.0f;
+    for (int64_t i = 0; i < m; i++) sa = spread_a[i] > sa ? spread_a[i] : sa;
+    for (int64_t j = 0; j < n; j++) sb = spread_b[j] > sb ? spread_b[j] : sb;
+    double bits = 53.0 + sa + sb - 4.0;
+    int s = (int)std::ceil(bits / 7.0);
+    if (s < jlc::SPLIT_MIN_SLICES) s = jlc::SPLIT_MIN_SLICES;
+    if (s > jlc::SPLIT_MAX_SLICES) s = jlc::SPLIT_MAX_SLICES;
+    return s;
+}
+
+int ensure_buf(int slot, size_t bytes);
+
+int run_split_int8(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+                   const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int slices,
+                   hipStream_t stream, jlc::TileInfo* info) {
+    ThreadCtx& ctx = t_ctx;
+    jlc::SplitPlan p;
+    jlc::split_plan_sizes(m, n, k, slices ? slices : jlc::SPLIT_MAX_SLICES, &p);
+    const size_t stat_bytes = (size_t)(p.mpad + p.npad) * (sizeof(int) + sizeof(float));
+    int rc = ensure_buf(3, stat_bytes);
+    if (rc != JLC_OK) return rc;
+    char* sb = static_cast<char*>(ctx.buf[3]);
+    p.ea = reinterpret_cast<int*>(sb);
+    p.eb = p.ea + p.mpad;
+    p.spread_a = reinterpret_cast<float*>(p.eb + p.npad);
+    p.spread_b = p.spread_a + p.mpad;
+    JLC_HIP_CHECK(jlc::launch_split_stats(la, lb, a, lda, b, ldb, &p, stream));
+    if (slices == 0) {
+        std::vector<float> host((size_t)(p.mpad + p.npad));
+        JLC_HIP_CHECK(hipMemcpyAsync(host.data(), p.spread_a, host.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                     stream));
+        JLC_HIP_CHECK(hipStreamSynchronize(stream));
+        slices = choose_slices(host.data(), m, host.data() + p.mpad, n);
+        p.slices = slices;
+    }
+    if ((rc = ensure_buf(4, (size_t)(p.slices * p.a_slice_bytes))) != JLC_OK) return rc;
+    if ((rc = ensure_buf(5, (size_t)(p.slices * p.b_slice_bytes))) != JLC_OK) return rc;
+    p.a_pack = ctx.buf[4];
+    p.b_pack = ctx.buf[5];
+    info->slices = p.slices;
+    JLC_HIP_CHECK(jlc::launch_split_pack_and_gemm(la, lb, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info));
+    return JLC_OK;
+}
+
 // Device-side dispatch shared by the host and device entry points.
 int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
                     const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
@@ -169,7 +229,12 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
         return JLC_OK;
     }
     int form = mode & 0xff;
-    if (form != JLC_MODE_FP64) return fail(JLC_ERR_UNSUPPORTED, "split-precision modes are not built in this library");
+    if (form == JLC_MODE_SPLIT_INT8) {
+        *kernel = JLC_KERNEL_SPLIT_INT8;
+        return run_split_int8(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, (mode >> 8) & 0xff, stream,
+                              info);
+    }
+    if (form != JLC_MODE_FP64) return fail(JLC_ERR_UNSUPPORTED, "the BF16 split form is not built in this library");
     if (n == 1 && la == JLC_ROW) {
         // The n == 1 regime is the matvec (GemmDispatch.selectKernel, GemmDispatch.java:80-82):
         // x is the B column, y the C column.
@@ -191,7 +256,7 @@ void fill_stats(jlc_stats* s, const jlc::TileInfo& info, int kernel, double kms,
     s->tile_m = info.tile_m;
     s->tile_n = info.tile_n;
     s->tile_k = info.tile_k;
-    s->slices = 0;
+    s->slices = info.slices;
     s->launches = info.launches;
 }
 
@@ -236,7 +301,7 @@ int jlc_hip_init(int device) {
 int jlc_hip_shutdown(void) {
     ThreadCtx& c = t_ctx;
     if (c.device >= 0) {
-        for (int i = 0; i < 3; i++) {
+        for (int i = 0; i < ThreadCtx::NBUF; i++) {
             if (c.buf[i]) (void)hipFree(c.buf[i]);
             c.buf[i] = nullptr;
             c.cap[i] = 0;

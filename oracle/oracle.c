@@ -463,3 +463,93 @@ void oracle_qr_form_q(int64_t m, int64_t n, const double* qr, int64_t lda, const
         }
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Split-precision restatement (see oracle.h). Every FP64 step mirrors the
+ * device code: frexp of the max |x|, ldexp scaling, v*128 / trunc / subtract
+ * digit extraction, exact integer diagonal sums, sequential recombination. */
+#define SPLIT_NONFINITE (-100000)
+
+int oracle_split_exponent(const double* x, int64_t stride, int64_t count) {
+    double mx = 0.0;
+    for (int64_t i = 0; i < count; i++) {
+        double v = fabs(x[i * stride]);
+        if (!(v <= 1.7976931348623157e308)) return SPLIT_NONFINITE;
+        mx = fmax(mx, v);
+    }
+    int ex = 0;
+    if (mx > 0.0) frexp(mx, &ex);
+    return ex;
+}
+
+void oracle_split_slices(const double* x, int64_t stride, int64_t count, int e, int slices, int8_t* out,
+                         int64_t out_stride) {
+    for (int64_t i = 0; i < count; i++) {
+        double v = (e == SPLIT_NONFINITE) ? 0.0 : ldexp(x[i * stride], -e);
+        for (int t = 0; t < slices; t++) {
+            v *= 128.0;
+            double d = trunc(v);
+            v -= d;
+            out[t * out_stride + i] = (int8_t)(int)d;
+        }
+    }
+}
+
+/* Diagonal-sum chunk: S * chunk * 127^2 < 2^31 with chunk a multiple of 32. */
+int64_t oracle_split_chunk_k(int slices, int64_t k) {
+    int64_t steps = 2147483647LL / ((int64_t)slices * 32 * 127 * 127);
+    int64_t ksteps = (k + 31) / 32;
+    if (steps > ksteps) steps = ksteps;
+    return steps * 32;
+}
+
+void oracle_split_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                       int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int slices,
+                       int64_t chunk_k, int threads) {
+    const int S = slices;
+    int8_t* as = (int8_t*)calloc((size_t)(S * m * k + 1), 1);
+    int8_t* bs = (int8_t*)calloc((size_t)(S * n * k + 1), 1);
+    int* ea = (int*)malloc(sizeof(int) * (size_t)(m + 1));
+    int* eb = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+    for (int64_t i = 0; i < m; i++) {
+        const double* row = a_layout == ORACLE_ROW ? a + i * lda : a + i;
+        int64_t st = a_layout == ORACLE_ROW ? 1 : lda;
+        ea[i] = oracle_split_exponent(row, st, k);
+        oracle_split_slices(row, st, k, ea[i], S, as + i * k, m * k);
+    }
+    for (int64_t j = 0; j < n; j++) {
+        const double* col = b_layout == ORACLE_ROW ? b + j : b + j * ldb;
+        int64_t st = b_layout == ORACLE_ROW ? ldb : 1;
+        eb[j] = oracle_split_exponent(col, st, k);
+        oracle_split_slices(col, st, k, eb[j], S, bs + j * k, n * k);
+    }
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int64_t i = 0; i < m; i++) {
+        for (int64_t j = 0; j < n; j++) {
+            double* p = &c[i * ldc + j];
+            for (int64_t k0 = 0, first = 1; k0 < k || first; k0 += chunk_k, first = 0) {
+                int64_t k1 = k0 + chunk_k < k ? k0 + chunk_k : k;
+                int64_t acc[16] = {0};
+                for (int t = 0; t < S; t++)
+                    for (int u = 0; u + t <= S - 1; u++) {
+                        const int8_t* ar = as + (int64_t)t * m * k + i * k;
+                        const int8_t* br = bs + (int64_t)u * n * k + j * k;
+                        int64_t sdot = 0;
+                        for (int64_t q = k0; q < k1; q++) sdot += (int64_t)ar[q] * br[q];
+                        acc[t + u] += sdot;
+                    }
+                double sum = 0.0;
+                for (int d = S - 1; d >= 0; d--) sum += (double)acc[d] * ldexp(1.0, -7 * (d + 2));
+                double v = (ea[i] == SPLIT_NONFINITE || eb[j] == SPLIT_NONFINITE) ? NAN : ldexp(sum, ea[i] + eb[j]);
+                if (first) *p = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * *p);
+                else *p = fma(alpha, v, *p);
+                if (k1 >= k) break;
+            }
+        }
+    }
+    free(as);
+    free(bs);
+    free(ea);
+    free(eb);
+}

@@ -81,6 +81,20 @@ void oracle_dd_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k,
                     int64_t lda, const double* b, int64_t ldb, double beta, const double* c, int64_t ldc,
                     double* hi, double* lo, int threads);
 
+/* Split-precision (INT8 slice) GEMM restatement: the exact algorithm of the
+ * split form (row/column power-of-two scaling, S truncated 7-bit slices,
+ * triangular pair set t+u <= S-1, per-diagonal exact integer sums in K chunks of
+ * chunk_k, FP64 recombination from the smallest diagonal up, alpha/beta via fma).
+ * No reference counterpart: the north star's new component (Ozaki et al. 2012,
+ * Ootomo-Ozaki-Yokota 2024). Returns the exponent marker used for Inf/NaN rows. */
+int oracle_split_exponent(const double* x, int64_t stride, int64_t count);
+void oracle_split_slices(const double* x, int64_t stride, int64_t count, int e, int slices, int8_t* out,
+                         int64_t out_stride);
+void oracle_split_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                       int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int slices,
+                       int64_t chunk_k, int threads);
+int64_t oracle_split_chunk_k(int slices, int64_t k);
+
 /* Blocked Householder QR restatement (column-major in/out). */
 int oracle_qr_factor(int64_t m, int64_t n, double* a_colmajor, int64_t lda, double* tau, int64_t nb);
 void oracle_qr_form_q(int64_t m, int64_t n, const double* qr_colmajor, int64_t lda, const double* tau,

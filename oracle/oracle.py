@@ -57,6 +57,13 @@ def _load():
                                       C.c_int]
     lib.oracle_dd_gemm.argtypes = [C.c_int, C.c_int, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p,
                                    _p, C.c_int]
+    lib.oracle_split_exponent.argtypes = [_p, _i64, _i64]
+    lib.oracle_split_exponent.restype = C.c_int
+    lib.oracle_split_slices.argtypes = [_p, _i64, _i64, C.c_int, C.c_int, _p, _i64]
+    lib.oracle_split_gemm.argtypes = [C.c_int, C.c_int, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64,
+                                      C.c_int, _i64, C.c_int]
+    lib.oracle_split_chunk_k.argtypes = [C.c_int, _i64]
+    lib.oracle_split_chunk_k.restype = _i64
     lib.oracle_qr_factor.argtypes = [_i64, _i64, _p, _i64, _p, _i64]
     lib.oracle_qr_form_q.argtypes = [_i64, _i64, _p, _i64, _p, _p, _i64]
     _lib = lib
@@ -243,3 +250,33 @@ def qr_form_q(qr_colmajor: np.ndarray, tau: np.ndarray, m: int, n: int, lda: int
     q = np.zeros(m * n)
     _load().oracle_qr_form_q(m, n, _ptr(qr_colmajor), lda, _ptr(tau), _ptr(q), m)
     return q
+
+
+SPLIT_NONFINITE = -100000
+
+
+def split_exponent(x: np.ndarray) -> int:
+    """Row / column exponent of the split form: frexp exponent of max |x|."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return _load().oracle_split_exponent(_ptr(x), 1, x.size)
+
+
+def split_slices(x: np.ndarray, e: int, slices: int) -> np.ndarray:
+    """(slices, len(x)) int8 digits of x * 2^-e."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((slices, x.size), dtype=np.int8)
+    _load().oracle_split_slices(_ptr(x), 1, x.size, e, slices, out.ctypes.data, x.size)
+    return out
+
+
+def split_chunk_k(slices: int, k: int) -> int:
+    return _load().oracle_split_chunk_k(slices, k)
+
+
+def split_gemm(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, a, lda: int, b, ldb: int,
+               beta: float, c: np.ndarray, ldc: int, slices: int, chunk_k: int | None = None, a_off: int = 0,
+               b_off: int = 0, c_off: int = 0, nthreads: int | None = None) -> None:
+    """In-place split-precision GEMM restatement (bit-exact reference of the INT8 split form)."""
+    ck = chunk_k if chunk_k is not None else split_chunk_k(slices, k)
+    _load().oracle_split_gemm(layout_a, layout_b, m, n, k, alpha, _ptr(a, a_off), lda, _ptr(b, b_off), ldb, beta,
+                              _ptr(c, c_off), ldc, slices, ck, nthreads or threads())
