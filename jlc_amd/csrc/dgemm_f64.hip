@@ -11,12 +11,10 @@
 // Macro tile 128 x 128 x 16 per 256-thread workgroup, four wavefronts in a
 // 2 x 2 grid, each wavefront owning a 64 x 64 block of C as 4 x 4
 // v_mfma_f64_16x16x4 accumulators; two workgroups resident per CU. Operand tiles live in LDS as [k][m] and
-// [k][n] (128 doubles per k row) in two buffers: while the MFMAs consume one
-// buffer, the next K tile is fetched into registers and written to the other,
-// one barrier per K tile. Column indices inside each 16-wide group are XOR
-// swizzled with (k & 14) so both the transposing stores and the fragment
-// reads are free of LDS bank conflicts without padding (64 KiB per group,
-// two groups per CU).
+// [k][n] with k in pairs (16-byte entries, see slot()), two buffers: while
+// the MFMAs consume one buffer, the next K tile is fetched into registers and
+// written to the other, one barrier per K tile (64 KiB per group, two groups
+// per CU). Each ds_read_b128 fragment load feeds two MFMA steps.
 //
 // Edges are masked: loads outside [0,M) x [0,K) or [0,K) x [0,N) read as
 // zero and stores outside C are skipped, so any M, N, K >= 1 is valid.
@@ -34,27 +32,46 @@ constexpr int BK = 16;    // K per stage
 constexpr int NT = 256;   // threads per workgroup
 constexpr int SROW = 128; // doubles per k row of a staged tile (BM == BN)
 
-__device__ __forceinline__ int swz(int k, int r) { return r ^ (k & 14); }
+// Staged tiles hold k in pairs: entry (kp, r) is the 16-byte pair
+// (x(r, 2kp), x(r, 2kp+1)) at slot kp*128 + (r ^ (kp & 7)). One ds_read_b128
+// then feeds two MFMA steps, and the XOR spreads each 8-lane phase of a
+// transposing store or fragment read over 8 distinct 16-byte bank groups.
+__device__ __forceinline__ int slot(int kp, int r) { return kp * SROW + (r ^ (kp & 7)); }
 
-// Loads one 128 x 16 operand tile into 4 double pairs per thread.
-// KCONTIG: element (r, kk) at base[r*ld + kk] (k is the contiguous index).
-// !KCONTIG: element (r, kk) at base[kk*ld + r] (r is the contiguous index).
-// rdim / kdim bound r and kk; out-of-range elements read as zero.
-template <bool KCONTIG, bool VEC>
+// Loads one 128 x 16 operand tile as 4 pair entries per thread.
+// KCONTIG: element (r, k) at base[r*ld + k]: thread reads the pair (r, 2kp..2kp+1) directly.
+// !KCONTIG: element (r, k) at base[k*ld + r]: thread reads rows k = 2kp, 2kp+1 at r, r+1 and
+//           regroups them into the pairs of r and r+1.
+// Out-of-range elements read as zero.
+template <bool KCONTIG, bool VEC, bool MASK>
 __device__ __forceinline__ void load_tile(const double* __restrict__ base, int64_t ld, int rdim, int kdim, int r0,
                                           int k0, int t, d2 (&regs)[4]) {
+    if (!MASK) {
+        // Interior tile: every element is in range.
+        if (KCONTIG) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int r, kk;
-        if (KCONTIG) {
-            r = r0 + (t >> 3) + 32 * i;
-            kk = k0 + (t & 7) * 2;
+            for (int i = 0; i < 4; i++) {
+                const double* p = base + (int64_t)(r0 + (t >> 3) + 32 * i) * ld + k0 + (t & 7) * 2;
+                regs[i] = VEC ? *reinterpret_cast<const d2*>(p) : d2{p[0], p[1]};
+            }
         } else {
-            kk = k0 + (t >> 6) + 4 * i;
-            r = r0 + (t & 63) * 2;
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const double* p = base + (int64_t)(k0 + 2 * ((t >> 6) + 4 * i)) * ld + r0 + (t & 63) * 2;
+                const d2 lo = VEC ? *reinterpret_cast<const d2*>(p) : d2{p[0], p[1]};
+                const d2 hi = VEC ? *reinterpret_cast<const d2*>(p + ld) : d2{p[ld], p[ld + 1]};
+                regs[2 * i] = d2{lo.x, hi.x};
+                regs[2 * i + 1] = d2{lo.y, hi.y};
+            }
         }
-        d2 v = {0.0, 0.0};
-        if (KCONTIG) {
+        return;
+    }
+    if (KCONTIG) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = r0 + (t >> 3) + 32 * i;
+            const int kk = k0 + (t & 7) * 2;
+            d2 v = {0.0, 0.0};
             if (r < rdim) {
                 const double* p = base + (int64_t)r * ld + kk;
                 if (VEC && kk + 1 < kdim) {
@@ -64,57 +81,69 @@ __device__ __forceinline__ void load_tile(const double* __restrict__ base, int64
                     if (kk + 1 < kdim) v.y = p[1];
                 }
             }
-        } else {
-            if (kk < kdim) {
-                const double* p = base + (int64_t)kk * ld + r;
-                if (VEC && r + 1 < rdim) {
-                    v = *reinterpret_cast<const d2*>(p);
-                } else {
-                    if (r < rdim) v.x = p[0];
-                    if (r + 1 < rdim) v.y = p[1];
-                }
+            regs[i] = v;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int kk = k0 + 2 * ((t >> 6) + 4 * i);
+            const int r = r0 + (t & 63) * 2;
+            d2 lo = {0.0, 0.0}, hi = {0.0, 0.0};  // rows kk and kk+1, columns r and r+1
+            const double* p = base + (int64_t)kk * ld + r;
+            if (VEC && r + 1 < rdim) {
+                if (kk < kdim) lo = *reinterpret_cast<const d2*>(p);
+                if (kk + 1 < kdim) hi = *reinterpret_cast<const d2*>(p + ld);
+            } else {
+                if (kk < kdim && r < rdim) lo.x = p[0];
+                if (kk < kdim && r + 1 < rdim) lo.y = p[1];
+                if (kk + 1 < kdim && r < rdim) hi.x = p[ld];
+                if (kk + 1 < kdim && r + 1 < rdim) hi.y = p[ld + 1];
             }
+            regs[2 * i] = d2{lo.x, hi.x};
+            regs[2 * i + 1] = d2{lo.y, hi.y};
         }
-        regs[i] = v;
     }
 }
 
-// Writes the registers of load_tile into a staged [k][r] tile (swizzled).
+// Writes the pair entries of load_tile into a staged tile.
 template <bool KCONTIG>
-__device__ __forceinline__ void store_tile(double* __restrict__ s, int t, const d2 (&regs)[4]) {
+__device__ __forceinline__ void store_tile(d2* __restrict__ s, int t, const d2 (&regs)[4]) {
+    if (KCONTIG) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        if (KCONTIG) {
-            int r = (t >> 3) + 32 * i;
-            int kk = (t & 7) * 2;
-            s[kk * SROW + swz(kk, r)] = regs[i].x;
-            s[(kk + 1) * SROW + swz(kk + 1, r)] = regs[i].y;
-        } else {
-            int kk = (t >> 6) + 4 * i;
-            int r = (t & 63) * 2;
-            *reinterpret_cast<d2*>(&s[kk * SROW + swz(kk, r)]) = regs[i];
+        for (int i = 0; i < 4; i++) s[slot(t & 7, (t >> 3) + 32 * i)] = regs[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int kp = (t >> 6) + 4 * i;
+            const int r = (t & 63) * 2;
+            s[slot(kp, r)] = regs[2 * i];
+            s[slot(kp, r + 1)] = regs[2 * i + 1];
         }
     }
 }
 
-// Runs the 4-deep MFMA steps [ks0, ks1) of one staged K tile on a wavefront's
-// 4 x 4 accumulator blocks (rows wm*64.., cols wn*64..).
-__device__ __forceinline__ void mfma_steps(const double* __restrict__ as, const double* __restrict__ bs, int ks0,
-                                           int ks1, int wm, int wn, int g, int c16, d4 (&acc)[4][4]) {
+// Runs k pairs [kp0, kp1) (each two MFMA steps) of one staged K tile on a
+// wavefront's 4 x 4 accumulator blocks. Lane group g of step j takes
+// k = 8b + 2g + j, the same for A and B, so the products sum the right terms.
+__device__ __forceinline__ void mfma_steps(const d2* __restrict__ as, const d2* __restrict__ bs, int b0, int b1,
+                                           int wm, int wn, int g, int c16, d4 (&acc)[4][4]) {
 #pragma unroll
-    for (int ks = ks0; ks < ks1; ks++) {
-        const int kr = ks * 4 + g;
-        double af[4], bf[4];
+    for (int b = b0; b < b1; b++) {
+        const int kp = 4 * b + g;
+        d2 af[4], bf[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            af[i] = as[kr * SROW + swz(kr, wm * 64 + i * 16 + c16)];
-            bf[i] = bs[kr * SROW + swz(kr, wn * 64 + i * 16 + c16)];
+            af[i] = as[slot(kp, wm * 64 + i * 16 + c16)];
+            bf[i] = bs[slot(kp, wn * 64 + i * 16 + c16)];
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int j2 = 0; j2 < 2; j2++)
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(j2 ? af[i].y : af[i].x, j2 ? bf[j].y : bf[j].x,
+                                                                     acc[i][j], 0, 0, 0);
     }
 }
 
@@ -137,27 +166,18 @@ __device__ __forceinline__ void tile_coords(int bid, int nblocks, int tiles_m, i
     tn = in / gsize;
 }
 
-// Two wavefronts per SIMD (two workgroups per CU) hide the barrier and LDS
-// latency of one group behind the other's MFMAs; the register budget is 256.
-template <bool A_COL, bool B_COL, bool VEC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void dgemm_f64_mfma(int M, int N, int K, double alpha,
-                                                     const double* __restrict__ A, int64_t lda,
-                                                     const double* __restrict__ B, int64_t ldb, double beta,
-                                                     double* __restrict__ C, int64_t ldc, int tiles_m,
-                                                     int tiles_n) {
-    __shared__ __attribute__((aligned(16))) double As[2][BK * SROW];
-    __shared__ __attribute__((aligned(16))) double Bs[2][BK * SROW];
-
+// Main loop and epilogue of one C tile; MASK selects bounds-checked loads and
+// stores (edge tiles) or unchecked ones (interior tiles).
+template <bool A_COL, bool B_COL, bool VEC, bool MASK>
+__device__ __forceinline__ void gemm_tile(int M, int N, int K, double alpha, const double* __restrict__ A,
+                                          int64_t lda, const double* __restrict__ B, int64_t ldb, double beta,
+                                          double* __restrict__ C, int64_t ldc, int m0, int n0, d2 (*As)[BK / 2 * SROW],
+                                          d2 (*Bs)[BK / 2 * SROW]) {
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wid = t >> 6;
     const int wm = wid >> 1;
     const int wn = wid & 1;
-
-    int tm, tn;
-    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
-    const int m0 = tm * BM;
-    const int n0 = tn * BN;
 
     // op(A) row-major => k contiguous; op(B) column-major => k contiguous.
     constexpr bool A_KC = !A_COL;
@@ -170,14 +190,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         for (int j = 0; j < 4; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
     d2 ra[4], rb[4];
-    load_tile<A_KC, VEC>(A, lda, M, K, m0, 0, t, ra);
-    load_tile<B_KC, VEC>(B, ldb, N, K, n0, 0, t, rb);
+    load_tile<A_KC, VEC, MASK>(A, lda, M, K, m0, 0, t, ra);
+    load_tile<B_KC, VEC, MASK>(B, ldb, N, K, n0, 0, t, rb);
     store_tile<A_KC>(As[0], t, ra);
     store_tile<B_KC>(Bs[0], t, rb);
     __syncthreads();
 
     const int nk = (K + BK - 1) / BK;
-    const int g = lane >> 4;   // k row within a 4-deep MFMA step
+    const int g = lane >> 4;   // k group within an MFMA step
     const int c16 = lane & 15; // column within a 16-wide fragment
     // Branch-free steady state: every iteration but the last prefetches the
     // next K tile into registers, runs half of the current tile's MFMAs,
@@ -186,15 +206,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     int kt = 0;
     for (; kt < nk - 1; kt++) {
         const int cur = kt & 1;
-        load_tile<A_KC, VEC>(A, lda, M, K, m0, (kt + 1) * BK, t, ra);
-        load_tile<B_KC, VEC>(B, ldb, N, K, n0, (kt + 1) * BK, t, rb);
-        mfma_steps(As[cur], Bs[cur], 0, BK / 8, wm, wn, g, c16, acc);
+        load_tile<A_KC, VEC, MASK>(A, lda, M, K, m0, (kt + 1) * BK, t, ra);
+        load_tile<B_KC, VEC, MASK>(B, ldb, N, K, n0, (kt + 1) * BK, t, rb);
+        mfma_steps(As[cur], Bs[cur], 0, 1, wm, wn, g, c16, acc);
         store_tile<A_KC>(As[cur ^ 1], t, ra);
         store_tile<B_KC>(Bs[cur ^ 1], t, rb);
-        mfma_steps(As[cur], Bs[cur], BK / 8, BK / 4, wm, wn, g, c16, acc);
+        mfma_steps(As[cur], Bs[cur], 1, 2, wm, wn, g, c16, acc);
         __syncthreads();
     }
-    mfma_steps(As[kt & 1], Bs[kt & 1], 0, BK / 4, wm, wn, g, c16, acc);
+    mfma_steps(As[kt & 1], Bs[kt & 1], 0, 2, wm, wn, g, c16, acc);
 
     // v_mfma_f64_16x16x4 accumulator layout: element r of lane l holds C(l/16 + 4*r, l%16)
     // of its 16 x 16 block.
@@ -203,11 +223,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int col = n0 + wn * 64 + j * 16 + c16;
-            if (col >= N) continue;
+            if (MASK && col >= N) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = m0 + wm * 64 + i * 16 + g + 4 * r;
-                if (row < M) {
+                if (!MASK || row < M) {
                     double* p = C + (int64_t)row * ldc + col;
                     double v = alpha * acc[i][j][r];
                     if (beta != 0.0) v += beta * *p;
@@ -216,6 +236,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
             }
         }
     }
+}
+
+// Two wavefronts per SIMD (two workgroups per CU) hide the barrier and LDS
+// latency of one group behind the other's MFMAs; the register budget is 256.
+template <bool A_COL, bool B_COL, bool VEC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void dgemm_f64_mfma(int M, int N, int K, double alpha,
+                                                     const double* __restrict__ A, int64_t lda,
+                                                     const double* __restrict__ B, int64_t ldb, double beta,
+                                                     double* __restrict__ C, int64_t ldc, int tiles_m,
+                                                     int tiles_n) {
+    __shared__ __attribute__((aligned(16))) d2 As[2][BK / 2 * SROW];
+    __shared__ __attribute__((aligned(16))) d2 Bs[2][BK / 2 * SROW];
+    int tm, tn;
+    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * BM;
+    const int n0 = tn * BN;
+    // Uniform per workgroup: interior tiles take the unchecked path.
+    if (m0 + BM <= M && n0 + BN <= N && K % BK == 0)
+        gemm_tile<A_COL, B_COL, VEC, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
+    else
+        gemm_tile<A_COL, B_COL, VEC, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
 }
 
 __global__ __launch_bounds__(256) void scale_kernel(int M, int N, double beta, double* __restrict__ C, int64_t ldc) {

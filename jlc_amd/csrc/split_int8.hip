@@ -142,9 +142,11 @@ __global__ __launch_bounds__(256) void pack_slices(const double* __restrict__ X,
     }
 }
 
+// Tile order: ids regrouped per XCD, then column strips of 4 tile rows (with
+// 128 x 64 tiles this keeps the 32 resident tiles of an XCD on 4 A and 8 B panels).
 __device__ __forceinline__ void tile_coords(int bid, int nblocks, int tiles_m, int tiles_n, int& tm, int& tn) {
     constexpr int XCDS = 8;
-    constexpr int GROUP = 8;
+    constexpr int GROUP = 4;
     int full = nblocks - nblocks % XCDS;
     int lid = bid;
     if (bid < full) lid = (bid % XCDS) * (full / XCDS) + bid / XCDS;
@@ -235,14 +237,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int u = 0; u < S; u++) bf[u] = st[(NA + u * 2 + wn) * 64];
         // t outer, u descending: consecutive MFMAs always target different diagonals.
+        // The next stage's LDS store sits after the first two slice rows of MFMAs.
 #pragma unroll
         for (int t = 0; t < S; t++) {
+            if (t == 2 && more) store(cur ^ 1);
             const v4i af = st[(t * 4 + wm) * 64];
 #pragma unroll
             for (int u = S - 1 - t; u >= 0; u--)
                 acc[t + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[u], acc[t + u], 0, 0, 0);
         }
-        if (more) store(cur ^ 1);
         __syncthreads();
         if ((kc + 1) % chunk == 0 || !more) {
             flush(kc + 1 <= chunk);
