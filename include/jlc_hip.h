@@ -112,6 +112,34 @@ int jlc_dgemv(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, 
 int jlc_dgemv_dev(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x,
                   int64_t incx, double beta, double* y, int64_t incy, void* stream, jlc_stats* stats);
 
+/* ---- grouped (batched, uneven) GEMM: thousands of independent problems in
+ * one persistent launch. Serves the reference's TINY / SMALL regimes in bulk
+ * (SpecializedKernels.tinyGemm / smallGemm,
+ * src/main/java/net/faulj/compute/SpecializedKernels.java:79-158) and batched
+ * QR trailing updates. All pointers in a descriptor are device pointers. */
+typedef struct jlc_gemm_desc {
+    int32_t layout_a, layout_b; /* JLC_ROW / JLC_COL */
+    int32_t m, n, k;
+    int32_t flags;              /* reserved, 0 */
+    double alpha, beta;
+    const double* a;
+    const double* b;
+    double* c;                  /* row-major, ldc >= n */
+    int64_t lda, ldb, ldc;
+} jlc_gemm_desc;
+
+typedef struct jlc_grouped_plan jlc_grouped_plan;
+/* Validates the descriptors, builds the tile list (sorted longest-K first)
+ * and uploads both to the device. Only JLC_MODE_FP64 is served. */
+int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode, jlc_grouped_plan** plan);
+/* One persistent launch over every tile of the plan. */
+int jlc_grouped_plan_run(jlc_grouped_plan* plan, void* stream, jlc_stats* stats);
+/* Captures the launch into a HIP graph (once) and replays it on stream. */
+int jlc_grouped_plan_replay(jlc_grouped_plan* plan, void* stream, jlc_stats* stats);
+int jlc_grouped_plan_destroy(jlc_grouped_plan* plan);
+/* Create + run + destroy. */
+int jlc_dgemm_grouped_dev(int64_t count, const jlc_gemm_desc* descs, int mode, void* stream, jlc_stats* stats);
+
 /* ---- device memory helpers for callers that keep operands resident. */
 int jlc_dev_alloc(void** ptr, size_t bytes);
 int jlc_dev_free(void* ptr);

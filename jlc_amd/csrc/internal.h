@@ -56,6 +56,24 @@ hipError_t launch_split_pack_and_gemm(int layout_a, int layout_b, const double* 
                                       int64_t ldb, const SplitPlan& p, double alpha, double beta, double* c,
                                       int64_t ldc, hipStream_t stream, TileInfo* info);
 
+// One problem of a grouped launch; the same layout as jlc_gemm_desc.
+struct GroupedDesc {
+    int32_t layout_a, layout_b;
+    int32_t m, n, k;
+    int32_t flags;
+    double alpha, beta;
+    const double* a;
+    const double* b;
+    double* c;
+    int64_t lda, ldb, ldc;
+};
+
+// Persistent grid size (workgroups) for the grouped kernel on a device with device_cus CUs.
+int grouped_grid(int device_cus);
+// tiles: (problem << 32) | (tile_row << 16) | tile_col, 32 x 32 tiles.
+hipError_t launch_grouped_f64(const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
+                              hipStream_t stream);
+
 // Device-to-device copy kernel (bytes may be any size).
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 

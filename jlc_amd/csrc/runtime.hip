@@ -19,6 +19,7 @@
 #include <mutex>
 #include <cmath>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/jlc_hip.h"
@@ -532,6 +533,158 @@ int jlc_stream_synchronize(void* stream) {
     if (rc != JLC_OK) return rc;
     JLC_HIP_CHECK(hipStreamSynchronize(stream ? static_cast<hipStream_t>(stream) : t_ctx.stream));
     return JLC_OK;
+}
+
+static_assert(sizeof(jlc_gemm_desc) == sizeof(jlc::GroupedDesc), "descriptor layouts must match");
+
+struct jlc_grouped_plan {
+    int device = -1;
+    int64_t count = 0;
+    int ntiles = 0;
+    int grid = 0;
+    jlc::GroupedDesc* d_descs = nullptr;
+    uint64_t* d_tiles = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static int stats_for_stream(hipStream_t s, jlc_stats* stats, double t0, int ntiles, bool start) {
+    if (!stats) return JLC_OK;
+    if (start) {
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+        return JLC_OK;
+    }
+    JLC_HIP_CHECK(hipEventRecord(t_ctx.ev1, s));
+    JLC_HIP_CHECK(hipEventSynchronize(t_ctx.ev1));
+    float f = 0.0f;
+    JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
+    jlc::TileInfo info;
+    info.tile_m = 32;
+    info.tile_n = 32;
+    info.tile_k = 16;
+    info.launches = 1;
+    fill_stats(stats, info, JLC_KERNEL_GROUPED, f, now_ms() - t0);
+    (void)ntiles;
+    return JLC_OK;
+}
+
+int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode, jlc_grouped_plan** plan) {
+    if (!plan) return fail(JLC_ERR_INVALID_ARG, "plan must not be null");
+    *plan = nullptr;
+    if (count < 0 || (count > 0 && !descs)) return fail(JLC_ERR_INVALID_ARG, "bad descriptor array");
+    if (count >= (int64_t)1 << 31) return fail(JLC_ERR_INVALID_ARG, "more than 2^31-1 problems");
+    if ((mode & 0xff) != JLC_MODE_FP64 || (mode >> 8) != 0)
+        return fail(JLC_ERR_UNSUPPORTED, "grouped launches serve JLC_MODE_FP64 only");
+    // Tile list: one entry per 32 x 32 tile, sorted by decreasing k.
+    std::vector<std::pair<int32_t, uint64_t>> order;
+    for (int64_t p = 0; p < count; p++) {
+        const jlc_gemm_desc& d = descs[p];
+        int rc = validate_gemm(d.layout_a, d.layout_b, d.m, d.n, d.k, d.alpha, d.a, d.lda, d.b, d.ldb, d.c, d.ldc,
+                               JLC_MODE_FP64);
+        if (rc != JLC_OK) {
+            std::string msg = t_error;
+            return fail(rc, "descriptor %lld: %s", (long long)p, msg.c_str());
+        }
+        if (d.m > 32 * 65535 || d.n > 32 * 65535) return fail(JLC_ERR_INVALID_ARG, "descriptor %lld too large", (long long)p);
+        const int tm = (d.m + 31) / 32, tn = (d.n + 31) / 32;
+        for (int i = 0; i < tm; i++)
+            for (int j = 0; j < tn; j++)
+                order.emplace_back(d.k, ((uint64_t)p << 32) | ((uint64_t)i << 16) | (uint64_t)j);
+    }
+    if (order.size() >= ((size_t)1 << 31)) return fail(JLC_ERR_INVALID_ARG, "too many tiles");
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int32_t, uint64_t>& x, const std::pair<int32_t, uint64_t>& y) {
+                         return x.first > y.first;
+                     });
+    std::vector<uint64_t> tiles(order.size());
+    for (size_t i = 0; i < order.size(); i++) tiles[i] = order[i].second;
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    jlc_grouped_plan* pl = new jlc_grouped_plan;
+    pl->device = g_device.load();
+    pl->count = count;
+    pl->ntiles = (int)tiles.size();
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, pl->device);
+    pl->grid = jlc::grouped_grid(e == hipSuccess ? prop.multiProcessorCount : 256);
+    e = hipMalloc(&pl->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::GroupedDesc));
+    if (e == hipSuccess) e = hipMalloc(&pl->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(uint64_t));
+    if (e == hipSuccess && count)
+        e = hipMemcpy(pl->d_descs, descs, (size_t)count * sizeof(jlc::GroupedDesc), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !tiles.empty())
+        e = hipMemcpy(pl->d_tiles, tiles.data(), tiles.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        jlc_grouped_plan_destroy(pl);
+        return fail(JLC_ERR_ALLOC, "grouped plan upload failed: %s", hipGetErrorString(e));
+    }
+    *plan = pl;
+    return JLC_OK;
+}
+
+int jlc_grouped_plan_run(jlc_grouped_plan* plan, void* stream, jlc_stats* stats) {
+    if (!plan) return fail(JLC_ERR_INVALID_ARG, "plan must not be null");
+    double t0 = now_ms();
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    if ((rc = stats_for_stream(s, stats, t0, plan->ntiles, true)) != JLC_OK) return rc;
+    JLC_HIP_CHECK(jlc::launch_grouped_f64(plan->d_descs, plan->d_tiles, plan->ntiles, plan->grid, s));
+    return stats_for_stream(s, stats, t0, plan->ntiles, false);
+}
+
+int jlc_grouped_plan_replay(jlc_grouped_plan* plan, void* stream, jlc_stats* stats) {
+    if (!plan) return fail(JLC_ERR_INVALID_ARG, "plan must not be null");
+    double t0 = now_ms();
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    if (!plan->exec) {
+        // Capture on a private stream so the caller's stream state is never in capture mode.
+        hipStream_t cap;
+        JLC_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) e = jlc::launch_grouped_f64(plan->d_descs, plan->d_tiles, plan->ntiles, plan->grid, cap);
+        hipGraph_t graph = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cap, &graph);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cap);
+        if (e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            plan->exec = nullptr;
+            return hip_fail(e, "grouped graph capture");
+        }
+        plan->graph = graph;
+    }
+    if ((rc = stats_for_stream(s, stats, t0, plan->ntiles, true)) != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipGraphLaunch(plan->exec, s));
+    return stats_for_stream(s, stats, t0, plan->ntiles, false);
+}
+
+int jlc_grouped_plan_destroy(jlc_grouped_plan* plan) {
+    if (!plan) return JLC_OK;
+    if (g_alive.load()) {
+        if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+        if (plan->graph) (void)hipGraphDestroy(plan->graph);
+        if (plan->d_descs) (void)hipFree(plan->d_descs);
+        if (plan->d_tiles) (void)hipFree(plan->d_tiles);
+    }
+    delete plan;
+    return JLC_OK;
+}
+
+int jlc_dgemm_grouped_dev(int64_t count, const jlc_gemm_desc* descs, int mode, void* stream, jlc_stats* stats) {
+    jlc_grouped_plan* plan = nullptr;
+    int rc = jlc_grouped_plan_create(count, descs, mode, &plan);
+    if (rc != JLC_OK) return rc;
+    rc = jlc_grouped_plan_run(plan, stream, stats);
+    if (rc == JLC_OK) {
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+    }
+    jlc_grouped_plan_destroy(plan);
+    return rc;
 }
 
 struct jlc_timer {

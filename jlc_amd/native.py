@@ -73,6 +73,17 @@ _d = C.c_double
 _p = C.c_void_p
 _i = C.c_int
 
+class GemmDesc(C.Structure):
+    """jlc_gemm_desc: one problem of a grouped launch (device pointers)."""
+    _fields_ = [
+        ("layout_a", C.c_int32), ("layout_b", C.c_int32),
+        ("m", C.c_int32), ("n", C.c_int32), ("k", C.c_int32), ("flags", C.c_int32),
+        ("alpha", C.c_double), ("beta", C.c_double),
+        ("a", C.c_void_p), ("b", C.c_void_p), ("c", C.c_void_p),
+        ("lda", C.c_int64), ("ldb", C.c_int64), ("ldc", C.c_int64),
+    ]
+
+
 _SIGNATURES = {
     "jlc_hip_probe": (_i, [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "jlc_hip_init": (_i, [_i]),
@@ -96,6 +107,11 @@ _SIGNATURES = {
     "jlc_timer_start": (_i, [_p, _p]),
     "jlc_timer_stop": (_i, [_p, _p, C.POINTER(_d)]),
     "jlc_timer_destroy": (_i, [_p]),
+    "jlc_grouped_plan_create": (_i, [_i64, _p, _i, C.POINTER(_p)]),
+    "jlc_grouped_plan_run": (_i, [_p, _p, C.POINTER(Stats)]),
+    "jlc_grouped_plan_replay": (_i, [_p, _p, C.POINTER(Stats)]),
+    "jlc_grouped_plan_destroy": (_i, [_p]),
+    "jlc_dgemm_grouped_dev": (_i, [_i64, _p, _i, _p, C.POINTER(Stats)]),
 }
 
 _lock = threading.Lock()
@@ -268,6 +284,46 @@ class Timer:
                 _lib.jlc_timer_destroy(self.handle)
         except Exception:
             pass
+
+
+class GroupedPlan:
+    """A grouped launch over many problems (jlc_grouped_plan_*)."""
+
+    def __init__(self, descs, mode: int = MODE_FP64):
+        arr = (GemmDesc * len(descs))(*descs)
+        self._arr = arr
+        p = C.c_void_p()
+        check(load().jlc_grouped_plan_create(len(descs), C.cast(arr, C.c_void_p), mode, C.byref(p)))
+        self.handle = p.value
+
+    def run(self, stream_handle: int | None = None, stats: bool = False) -> Stats | None:
+        st = Stats() if stats else None
+        check(load().jlc_grouped_plan_run(self.handle, stream_handle, C.byref(st) if st is not None else None))
+        return st
+
+    def replay(self, stream_handle: int | None = None, stats: bool = False) -> Stats | None:
+        st = Stats() if stats else None
+        check(load().jlc_grouped_plan_replay(self.handle, stream_handle, C.byref(st) if st is not None else None))
+        return st
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(load().jlc_grouped_plan_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and _lib is not None:
+                _lib.jlc_grouped_plan_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def dgemm_grouped_dev(descs, mode: int = MODE_FP64, stream_handle: int | None = None) -> Stats:
+    arr = (GemmDesc * len(descs))(*descs)
+    st = Stats()
+    check(load().jlc_dgemm_grouped_dev(len(descs), C.cast(arr, C.c_void_p), mode, stream_handle, C.byref(st)))
+    return st
 
 
 def header_symbols() -> list[str]:
