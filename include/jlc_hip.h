@@ -61,6 +61,7 @@ extern "C" {
 #define JLC_KERNEL_SPLIT_INT8 4 /* INT8 slice products, FP64 recombination */
 #define JLC_KERNEL_SPLIT_BF16 5 /* BF16 slice products, FP64 recombination */
 #define JLC_KERNEL_GROUPED 6    /* persistent grouped kernel */
+#define JLC_KERNEL_QR 7         /* Householder QR (panel kernels + compact-WY GEMMs) */
 
 typedef struct jlc_stats {
     double kernel_ms;   /* device time of the call's kernels (HIP events) */
@@ -139,6 +140,39 @@ int jlc_grouped_plan_replay(jlc_grouped_plan* plan, void* stream, jlc_stats* sta
 int jlc_grouped_plan_destroy(jlc_grouped_plan* plan);
 /* Create + run + destroy. */
 int jlc_dgemm_grouped_dev(int64_t count, const jlc_gemm_desc* descs, int mode, void* stream, jlc_stats* stats);
+
+/* ---- blocked Householder QR (HouseholderQR.decompose / factorizeBlocked,
+ * src/main/java/net/faulj/decomposition/qr/HouseholderQR.java:52-209).
+ * a is column-major m x n (element (i,j) at a[j*lda + i], the reference's AT
+ * layout); on return R is on and above the diagonal and the Householder vectors
+ * (v0 = 1 implicit) below it, tau[min(m,n)] their scalars, as computeHouseholder
+ * (:219-291) stores them. Panels of nb <= 128 columns are factored in FP64 on
+ * the GPU; the compact-WY trailing updates (:167-209) run on the GEMM entry
+ * points in `mode` (JLC_MODE_FP64 or JLC_MODE_SPLIT_INT8[, slices]); T is FP64. */
+int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, int64_t nb, int mode, void* stream,
+                   jlc_stats* stats);
+/* Thin Q (column-major m x n, ldq) from the factors (buildQ, HouseholderQR.java:390-444), n <= m. */
+int jlc_dorgqr_dev(int64_t m, int64_t n, const double* a, int64_t lda, const double* tau, int64_t nb, double* q,
+                   int64_t ldq, void* stream);
+/* ||A - QR||_F / ||A||_F and ||Q^T Q - I||_F for a factorisation of a_orig (the
+ * reference's QR acceptance metrics, src/test/java/net/faulj/stress/HouseholderQRTest.java:26-121). */
+int jlc_dqr_errors_dev(int64_t m, int64_t n, const double* a_orig, int64_t lda_orig, const double* qr, int64_t lda,
+                       const double* tau, int64_t nb, double* recon_err, double* orth_err);
+
+/* Batched small QR: each matrix (m*n <= 20480, n <= 256) factored by one
+ * workgroup in LDS, unblocked, same reflector convention. */
+typedef struct jlc_qr_desc {
+    int32_t m, n;
+    int64_t lda;
+    double* a;   /* column-major, device */
+    double* tau; /* min(m, n), device */
+} jlc_qr_desc;
+typedef struct jlc_qr_batched_plan jlc_qr_batched_plan;
+int jlc_qr_batched_plan_create(int64_t count, const jlc_qr_desc* descs, jlc_qr_batched_plan** plan);
+/* Captures the single launch into a HIP graph (once) and replays it. */
+int jlc_qr_batched_plan_replay(jlc_qr_batched_plan* plan, void* stream, jlc_stats* stats);
+int jlc_qr_batched_plan_destroy(jlc_qr_batched_plan* plan);
+int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream, jlc_stats* stats);
 
 /* ---- device memory helpers for callers that keep operands resident. */
 int jlc_dev_alloc(void** ptr, size_t bytes);

@@ -245,9 +245,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
                                                      const double* __restrict__ A, int64_t lda,
                                                      const double* __restrict__ B, int64_t ldb, double beta,
                                                      double* __restrict__ C, int64_t ldc, int tiles_m,
-                                                     int tiles_n) {
+                                                     int tiles_n, int kslice, int64_t c_slice) {
     __shared__ __attribute__((aligned(16))) d2 As[2][BK / 2 * SROW];
     __shared__ __attribute__((aligned(16))) d2 Bs[2][BK / 2 * SROW];
+    // Split K: slice blockIdx.y covers k in [y*kslice, min(K, (y+1)*kslice)) and
+    // writes its partial product to C + y*c_slice (kslice is a multiple of BK).
+    if (gridDim.y > 1) {
+        const int64_t koff = (int64_t)blockIdx.y * kslice;
+        K = (int)min((int64_t)kslice, (int64_t)K - koff);
+        A += A_COL ? koff * lda : koff;
+        B += B_COL ? koff : koff * ldb;
+        C += blockIdx.y * c_slice;
+    }
     int tm, tn;
     tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
     const int m0 = tm * BM;
@@ -271,13 +280,28 @@ __global__ __launch_bounds__(256) void scale_kernel(int M, int N, double beta, d
 
 template <bool A_COL, bool B_COL, bool VEC>
 hipError_t launch_variant(int M, int N, int K, double alpha, const double* A, int64_t lda, const double* B,
-                          int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream) {
+                          int64_t ldb, double beta, double* C, int64_t ldc, int slices, int kslice, int64_t c_slice,
+                          hipStream_t stream) {
     int tiles_m = (M + BM - 1) / BM;
     int tiles_n = (N + BN - 1) / BN;
     int64_t nblocks = (int64_t)tiles_m * tiles_n;
-    hipLaunchKernelGGL((dgemm_f64_mfma<A_COL, B_COL, VEC>), dim3((unsigned)nblocks), dim3(NT), 0, stream, M, N, K,
-                       alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+    hipLaunchKernelGGL((dgemm_f64_mfma<A_COL, B_COL, VEC>), dim3((unsigned)nblocks, (unsigned)slices), dim3(NT), 0,
+                       stream, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, kslice, c_slice);
     return hipGetLastError();
+}
+
+// C = alpha * sum_s P_s + beta * C, slices summed in order (deterministic).
+__global__ __launch_bounds__(256) void splitk_reduce(int M, int N, int slices, const double* __restrict__ P,
+                                                     int64_t p_slice, double alpha, double beta,
+                                                     double* __restrict__ C, int64_t ldc) {
+    const int64_t total = (int64_t)M * N;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < slices; q++) s += P[q * p_slice + idx];
+        double* c = C + (idx / N) * ldc + idx % N;
+        *c = beta == 0.0 ? alpha * s : alpha * s + beta * *c;
+    }
 }
 
 }  // namespace
@@ -290,6 +314,62 @@ hipError_t launch_scale(int64_t m, int64_t n, double beta, double* c, int64_t ld
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(scale_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (int)m, (int)n, beta, c, ldc);
     if (info) info->launches++;
+    return hipGetLastError();
+}
+
+int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus) {
+    // Few output tiles and a long K: give every CU work by cutting K.
+    const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+    const int64_t want = (2 * (int64_t)device_cus + tiles - 1) / tiles;
+    int64_t s = want;
+    if (s > k / 1024) s = k / 1024;  // keep at least 1024 of K per slice
+    if (s > 64) s = 64;
+    return s < 2 ? 1 : (int)s;
+}
+
+int64_t dgemm_splitk_kslice(int64_t k, int slices) {
+    int64_t ks = (k + slices - 1) / slices;
+    return (ks + BK - 1) / BK * BK;
+}
+
+hipError_t launch_dgemm_f64_splitk(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha,
+                                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                                   double* c, int64_t ldc, int slices, double* work, hipStream_t stream,
+                                   TileInfo* info) {
+    const bool vec = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) % 16 == 0) &&
+                     (lda % 2 == 0) && (ldb % 2 == 0);
+    const int M = (int)m, N = (int)n, K = (int)k;
+    const int kslice = (int)dgemm_splitk_kslice(k, slices);
+    slices = (int)((k + kslice - 1) / kslice);
+    const int64_t ps = m * n;
+    if (info) {
+        info->tile_m = BM;
+        info->tile_n = BN;
+        info->tile_k = BK;
+        info->launches += 2;
+    }
+    const int code = (layout_a ? 4 : 0) | (layout_b ? 2 : 0) | (vec ? 1 : 0);
+    hipError_t e;
+#define JLC_SPLITK_CASE(c_, a_, b_, v_)                                                                         \
+    case c_:                                                                                                  \
+        e = launch_variant<a_, b_, v_>(M, N, K, 1.0, a, lda, b, ldb, 0.0, work, N, slices, kslice, ps, stream); \
+        break;
+    switch (code) {
+        JLC_SPLITK_CASE(0, false, false, false)
+        JLC_SPLITK_CASE(1, false, false, true)
+        JLC_SPLITK_CASE(2, false, true, false)
+        JLC_SPLITK_CASE(3, false, true, true)
+        JLC_SPLITK_CASE(4, true, false, false)
+        JLC_SPLITK_CASE(5, true, false, true)
+        JLC_SPLITK_CASE(6, true, true, false)
+        default: JLC_SPLITK_CASE(7, true, true, true)
+    }
+#undef JLC_SPLITK_CASE
+    if (e != hipSuccess) return e;
+    int64_t blocks = (m * n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(256), 0, stream, M, N, slices, work, ps, alpha,
+                       beta, c, ldc);
     return hipGetLastError();
 }
 
@@ -307,14 +387,14 @@ hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, in
     }
     const int code = (layout_a ? 4 : 0) | (layout_b ? 2 : 0) | (vec ? 1 : 0);
     switch (code) {
-        case 0: return launch_variant<false, false, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-        case 1: return launch_variant<false, false, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-        case 2: return launch_variant<false, true, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-        case 3: return launch_variant<false, true, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-        case 4: return launch_variant<true, false, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-        case 5: return launch_variant<true, false, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-        case 6: return launch_variant<true, true, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-        default: return launch_variant<true, true, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+        case 0: return launch_variant<false, false, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1, K, 0, stream);
+        case 1: return launch_variant<false, false, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1, K, 0, stream);
+        case 2: return launch_variant<false, true, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1, K, 0, stream);
+        case 3: return launch_variant<false, true, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1, K, 0, stream);
+        case 4: return launch_variant<true, false, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1, K, 0, stream);
+        case 5: return launch_variant<true, false, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1, K, 0, stream);
+        case 6: return launch_variant<true, true, false>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1, K, 0, stream);
+        default: return launch_variant<true, true, true>(M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1, K, 0, stream);
     }
 }
 

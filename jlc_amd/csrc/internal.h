@@ -22,6 +22,14 @@ hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, in
                             const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
                             int64_t ldc, hipStream_t stream, TileInfo* info);
 
+// Split-K variant: K cut into `slices` pieces whose partial products go to
+// work (slices * m * n doubles) and are summed in order into C.
+int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus);
+hipError_t launch_dgemm_f64_splitk(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha,
+                                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                                   double* c, int64_t ldc, int slices, double* work, hipStream_t stream,
+                                   TileInfo* info);
+
 // y = alpha*A*x + beta*y, A row-major. Requires m > 0; k == 0 or alpha == 0 handled inside.
 hipError_t launch_dgemv(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, const double* x,
                         int64_t incx, double beta, double* y, int64_t incy, hipStream_t stream, TileInfo* info);
@@ -73,6 +81,26 @@ int grouped_grid(int device_cus);
 // tiles: (problem << 32) | (tile_row << 16) | tile_col, 32 x 32 tiles.
 hipError_t launch_grouped_f64(const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
                               hipStream_t stream);
+
+// ---- QR
+struct QrBatchDesc {
+    int32_t m, n;
+    int64_t lda;
+    double* a;    // column-major m x n, factored in place
+    double* tau;  // min(m, n)
+};
+int qr_panel_chunk();
+// Factors panel columns [k0, k1) of the column-major m-row matrix at (in place, tau[k0..k1)).
+// part: 2*G+1 doubles, dots: G*128 doubles, G = ceil((m-k0)/qr_panel_chunk()); k1 - k0 <= 128.
+hipError_t launch_qr_panel(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part, double* dots,
+                           hipStream_t stream);
+hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb, double* vc, hipStream_t stream);
+hipError_t launch_build_t(const double* g, int nb, const double* tau, double* T, hipStream_t stream);
+hipError_t launch_extract_r(const double* at, int64_t lda, int n, double* r, hipStream_t stream);
+hipError_t launch_fill_eye(double* x, int64_t ld, int rows, int cols, double diag, hipStream_t stream);
+hipError_t launch_sumsq_partial(const double* x, int64_t ld, int rows, int cols, double* part, int nparts,
+                                hipStream_t stream);
+hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_mn, hipStream_t stream);
 
 // Device-to-device copy kernel (bytes may be any size).
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);

@@ -29,6 +29,7 @@ namespace {
 
 std::mutex g_mutex;
 std::atomic<int> g_device{-1};
+std::atomic<int> g_cus{256};
 std::atomic<bool> g_alive{true};
 
 thread_local std::string t_error;
@@ -65,8 +66,9 @@ struct ThreadCtx {
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // Staging: 0 = A, 1 = B, 2 = C; split workspace: 3 = exponents/spreads, 4 = A slices, 5 = B slices.
-    static constexpr int NBUF = 6;
+    // Staging: 0 = A, 1 = B, 2 = C; split workspace: 3 = exponents/spreads, 4 = A slices, 5 = B slices;
+    // 6 = split-K partial products.
+    static constexpr int NBUF = 7;
     void* buf[NBUF] = {};
     size_t cap[NBUF] = {};
 
@@ -245,6 +247,14 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
         return JLC_OK;
     }
     *kernel = JLC_KERNEL_MFMA_F64;
+    const int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load());
+    if (slices > 1) {
+        int rc = ensure_buf(6, (size_t)slices * (size_t)(m * n) * sizeof(double));
+        if (rc != JLC_OK) return rc;
+        JLC_HIP_CHECK(jlc::launch_dgemm_f64_splitk(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices,
+                                                   static_cast<double*>(t_ctx.buf[6]), stream, info));
+        return JLC_OK;
+    }
     JLC_HIP_CHECK(jlc::launch_dgemm_f64(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, info));
     return JLC_OK;
 }
@@ -290,6 +300,11 @@ int jlc_hip_init(int device) {
     int rc = check_arch(device, nullptr, 0);
     if (rc != JLC_OK) return rc;
     JLC_HIP_CHECK(hipSetDevice(device));
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            g_cus.store(prop.multiProcessorCount);
+    }
     static bool registered = false;
     if (!registered) {
         std::atexit(mark_dead);
@@ -684,6 +699,316 @@ int jlc_dgemm_grouped_dev(int64_t count, const jlc_gemm_desc* descs, int mode, v
         if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
     }
     jlc_grouped_plan_destroy(plan);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Householder QR (HouseholderQR.factorizeBlocked / updateTrailingMatrix,
+// src/main/java/net/faulj/decomposition/qr/HouseholderQR.java:133-209).
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p && g_alive.load()) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
+    double* d() { return static_cast<double*>(p); }
+};
+
+int gemm_on(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+            const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode, hipStream_t s,
+            jlc::TileInfo* info) {
+    int kernel = 0;
+    return run_gemm_device(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode, s, info, &kernel);
+}
+
+// V (as Vc = V^T, nb x mk) and T of the panel [k0, k0+nb) of a factored matrix.
+int panel_wy(const double* a, int64_t lda, int64_t m, int64_t k0, int64_t nb, const double* tau, double* vc,
+             double* g, double* t, hipStream_t s, jlc::TileInfo* info) {
+    const int64_t mk = m - k0;
+    JLC_HIP_CHECK(jlc::launch_build_vc(a, lda, (int)m, (int)k0, (int)nb, vc, s));
+    // G = V^T V: op(A) = Vc (row-major), op(B) = V = Vc^T (column-major view of Vc).
+    int rc = gemm_on(JLC_ROW, JLC_COL, nb, nb, mk, 1.0, vc, mk, vc, mk, 0.0, g, nb, JLC_MODE_FP64, s, info);
+    if (rc != JLC_OK) return rc;
+    JLC_HIP_CHECK(jlc::launch_build_t(g, (int)nb, tau + k0, t, s));
+    return JLC_OK;
+}
+
+}  // namespace
+
+int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, int64_t nb, int mode, void* stream,
+                   jlc_stats* stats) {
+    double t0 = now_ms();
+    if (m < 0 || n < 0 || m > INT32_MAX || n > INT32_MAX) return fail(JLC_ERR_INVALID_ARG, "bad dimensions");
+    if (nb < 1 || nb > 128) return fail(JLC_ERR_INVALID_ARG, "panel width nb must be in [1, 128]");
+    const int form = mode & 0xff;
+    if (form != JLC_MODE_FP64 && form != JLC_MODE_SPLIT_INT8)
+        return fail(JLC_ERR_INVALID_ARG, "trailing-update mode must be FP64 or SPLIT_INT8");
+    const int64_t kmax = m < n ? m : n;
+    if (kmax == 0) return JLC_OK;
+    if (!a || !tau || lda < m) return fail(JLC_ERR_INVALID_ARG, "a/tau must not be null and lda >= m");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    const int64_t G = (m + jlc::qr_panel_chunk() - 1) / jlc::qr_panel_chunk();
+    DevBuf part, dots, vc, g, t, w1, w2;
+    hipError_t e = part.alloc((size_t)(2 * G + 1) * 8);
+    if (e == hipSuccess) e = dots.alloc((size_t)G * 128 * 8);
+    if (e == hipSuccess) e = vc.alloc((size_t)nb * m * 8);
+    if (e == hipSuccess) e = g.alloc((size_t)nb * nb * 8);
+    if (e == hipSuccess) e = t.alloc((size_t)nb * nb * 8);
+    if (e == hipSuccess) e = w1.alloc((size_t)n * nb * 8);
+    if (e == hipSuccess) e = w2.alloc((size_t)n * nb * 8);
+    if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "QR workspace allocation failed: %s", hipGetErrorString(e));
+    jlc::TileInfo info;
+    if (stats) JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    for (int64_t k0 = 0; k0 < kmax; k0 += nb) {
+        const int64_t k1 = k0 + nb < kmax ? k0 + nb : kmax;
+        const int64_t nbk = k1 - k0, mk = m - k0, nt = n - k1;
+        JLC_HIP_CHECK(jlc::launch_qr_panel(a, lda, (int)m, (int)k0, (int)k1, tau, part.d(), dots.d(), s));
+        info.launches += 2 * (int)nbk;
+        if (nt <= 0) continue;
+        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
+        // Transposed compact-WY update of C = A[k0:m, k1:n] held as C^T (nt x mk, row-major, ld = lda):
+        // W^T = C^T V; W^T <- W^T T; C^T -= W^T V^T   (C <- (I - V T^T V^T) C).
+        double* ct = a + k1 * lda + k0;
+        if ((rc = gemm_on(JLC_ROW, JLC_COL, nt, nbk, mk, 1.0, ct, lda, vc.d(), mk, 0.0, w1.d(), nbk, mode, s,
+                          &info)) != JLC_OK)
+            return rc;
+        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nt, nbk, nbk, 1.0, w1.d(), nbk, t.d(), nbk, 0.0, w2.d(), nbk,
+                          JLC_MODE_FP64, s, &info)) != JLC_OK)
+            return rc;
+        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nt, mk, nbk, -1.0, w2.d(), nbk, vc.d(), mk, 1.0, ct, lda, mode, s,
+                          &info)) != JLC_OK)
+            return rc;
+    }
+    double kms = 0.0;
+    if (stats) {
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev1, s));
+        JLC_HIP_CHECK(hipEventSynchronize(t_ctx.ev1));
+        float f = 0.0f;
+        JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
+        kms = f;
+    } else {
+        JLC_HIP_CHECK(hipStreamSynchronize(s));  // workspace is freed on return
+    }
+    info.tile_m = (int)nb;
+    fill_stats(stats, info, JLC_KERNEL_QR, kms, now_ms() - t0);
+    return JLC_OK;
+}
+
+int jlc_dorgqr_dev(int64_t m, int64_t n, const double* a, int64_t lda, const double* tau, int64_t nb, double* q,
+                   int64_t ldq, void* stream) {
+    if (m < 0 || n < 0 || n > m || m > INT32_MAX) return fail(JLC_ERR_INVALID_ARG, "need 0 <= n <= m");
+    if (nb < 1 || nb > 128) return fail(JLC_ERR_INVALID_ARG, "panel width nb must be in [1, 128]");
+    if (n == 0) return JLC_OK;
+    if (!a || !tau || !q || lda < m || ldq < m) return fail(JLC_ERR_INVALID_ARG, "bad pointers or leading dims");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    DevBuf vc, g, t, y1, y2;
+    hipError_t e = vc.alloc((size_t)nb * m * 8);
+    if (e == hipSuccess) e = g.alloc((size_t)nb * nb * 8);
+    if (e == hipSuccess) e = t.alloc((size_t)nb * nb * 8);
+    if (e == hipSuccess) e = y1.alloc((size_t)n * nb * 8);
+    if (e == hipSuccess) e = y2.alloc((size_t)n * nb * 8);
+    if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "Q workspace allocation failed: %s", hipGetErrorString(e));
+    // Q^T (n x m, row-major, ld = ldq) starts as [I 0].
+    JLC_HIP_CHECK(jlc::launch_fill_eye(q, ldq, (int)n, (int)m, 1.0, s));
+    jlc::TileInfo info;
+    const int64_t last = ((n - 1) / nb) * nb;
+    for (int64_t k0 = last; k0 >= 0; k0 -= nb) {
+        const int64_t k1 = k0 + nb < n ? k0 + nb : n;
+        const int64_t nbk = k1 - k0, mk = m - k0, nq = n - k0;
+        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
+        // Q <- (I - V T V^T) Q on rows k0..m-1, columns k0..n-1:  Q^T -= ((Q^T V) T^T) V^T.
+        double* qt = q + k0 * ldq + k0;
+        if ((rc = gemm_on(JLC_ROW, JLC_COL, nq, nbk, mk, 1.0, qt, ldq, vc.d(), mk, 0.0, y1.d(), nbk, JLC_MODE_FP64, s,
+                          &info)) != JLC_OK)
+            return rc;
+        if ((rc = gemm_on(JLC_ROW, JLC_COL, nq, nbk, nbk, 1.0, y1.d(), nbk, t.d(), nbk, 0.0, y2.d(), nbk,
+                          JLC_MODE_FP64, s, &info)) != JLC_OK)
+            return rc;
+        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nq, mk, nbk, -1.0, y2.d(), nbk, vc.d(), mk, 1.0, qt, ldq, JLC_MODE_FP64,
+                          s, &info)) != JLC_OK)
+            return rc;
+    }
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    return JLC_OK;
+}
+
+namespace {
+int fro_norm(const double* x, int64_t ld, int64_t rows, int64_t cols, hipStream_t s, double* out) {
+    const int parts = 1024;
+    DevBuf p;
+    if (p.alloc(parts * 8) != hipSuccess) return fail(JLC_ERR_ALLOC, "norm workspace");
+    JLC_HIP_CHECK(jlc::launch_sumsq_partial(x, ld, (int)rows, (int)cols, p.d(), parts, s));
+    std::vector<double> h(parts);
+    JLC_HIP_CHECK(hipMemcpyAsync(h.data(), p.d(), parts * 8, hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    double sum = 0.0;
+    for (double v : h) sum += v;
+    *out = std::sqrt(sum);
+    return JLC_OK;
+}
+}  // namespace
+
+int jlc_dqr_errors_dev(int64_t m, int64_t n, const double* a_orig, int64_t lda_orig, const double* qr, int64_t lda,
+                       const double* tau, int64_t nb, double* recon_err, double* orth_err) {
+    if (!recon_err || !orth_err) return fail(JLC_ERR_INVALID_ARG, "outputs must not be null");
+    if (n > m || n <= 0) return fail(JLC_ERR_INVALID_ARG, "need 0 < n <= m");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = t_ctx.stream;
+    DevBuf q, r, d, e;
+    hipError_t he = q.alloc((size_t)m * n * 8);
+    if (he == hipSuccess) he = r.alloc((size_t)n * n * 8);
+    if (he == hipSuccess) he = d.alloc((size_t)m * n * 8);
+    if (he == hipSuccess) he = e.alloc((size_t)n * n * 8);
+    if (he != hipSuccess) return fail(JLC_ERR_ALLOC, "error-check workspace: %s", hipGetErrorString(he));
+    if ((rc = jlc_dorgqr_dev(m, n, qr, lda, tau, nb, q.d(), m, s)) != JLC_OK) return rc;
+    JLC_HIP_CHECK(jlc::launch_extract_r(qr, lda, (int)n, r.d(), s));
+    // D^T = A^T - R^T Q^T  (all n x m row-major views of column-major m x n matrices).
+    JLC_HIP_CHECK(hipMemcpy2DAsync(d.d(), (size_t)m * 8, a_orig, (size_t)lda_orig * 8, (size_t)m * 8, (size_t)n,
+                                   hipMemcpyDeviceToDevice, s));
+    jlc::TileInfo info;
+    if ((rc = gemm_on(JLC_COL, JLC_ROW, n, m, n, -1.0, r.d(), n, q.d(), m, 1.0, d.d(), m, JLC_MODE_FP64, s, &info)) !=
+        JLC_OK)
+        return rc;
+    double dn = 0.0, an = 0.0, en = 0.0;
+    if ((rc = fro_norm(d.d(), m, n, m, s, &dn)) != JLC_OK) return rc;
+    if ((rc = fro_norm(a_orig, lda_orig, n, m, s, &an)) != JLC_OK) return rc;
+    // E = Q^T Q - I.
+    JLC_HIP_CHECK(jlc::launch_fill_eye(e.d(), n, (int)n, (int)n, -1.0, s));
+    if ((rc = gemm_on(JLC_ROW, JLC_COL, n, n, m, 1.0, q.d(), m, q.d(), m, 1.0, e.d(), n, JLC_MODE_FP64, s, &info)) !=
+        JLC_OK)
+        return rc;
+    if ((rc = fro_norm(e.d(), n, n, n, s, &en)) != JLC_OK) return rc;
+    *recon_err = an > 0.0 ? dn / an : dn;
+    *orth_err = en;
+    return JLC_OK;
+}
+
+static_assert(sizeof(jlc_qr_desc) == sizeof(jlc::QrBatchDesc), "QR descriptor layouts must match");
+
+struct jlc_qr_batched_plan {
+    int count = 0;
+    int max_mn = 0;
+    jlc::QrBatchDesc* d_descs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int jlc_qr_batched_plan_create(int64_t count, const jlc_qr_desc* descs, jlc_qr_batched_plan** plan) {
+    if (!plan) return fail(JLC_ERR_INVALID_ARG, "plan must not be null");
+    *plan = nullptr;
+    if (count < 0 || count > INT32_MAX || (count && !descs)) return fail(JLC_ERR_INVALID_ARG, "bad descriptors");
+    int max_mn = 1;
+    for (int64_t i = 0; i < count; i++) {
+        const jlc_qr_desc& d = descs[i];
+        if (d.m < 1 || d.n < 1 || d.n > 256 || d.lda < d.m || !d.a || !d.tau)
+            return fail(JLC_ERR_INVALID_ARG, "descriptor %lld: need m, n >= 1, n <= 256, lda >= m", (long long)i);
+        if ((int64_t)d.m * d.n > 20480)
+            return fail(JLC_ERR_INVALID_ARG, "descriptor %lld: m*n = %lld exceeds the 20480-entry LDS tile",
+                        (long long)i, (long long)d.m * d.n);
+        max_mn = std::max(max_mn, d.m * d.n);
+    }
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    jlc_qr_batched_plan* p = new jlc_qr_batched_plan;
+    p->count = (int)count;
+    p->max_mn = max_mn;
+    hipError_t e = hipMalloc(&p->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::QrBatchDesc));
+    if (e == hipSuccess && count)
+        e = hipMemcpy(p->d_descs, descs, (size_t)count * sizeof(jlc::QrBatchDesc), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        jlc_qr_batched_plan_destroy(p);
+        return fail(JLC_ERR_ALLOC, "QR plan upload failed: %s", hipGetErrorString(e));
+    }
+    *plan = p;
+    return JLC_OK;
+}
+
+int jlc_qr_batched_plan_replay(jlc_qr_batched_plan* plan, void* stream, jlc_stats* stats) {
+    if (!plan) return fail(JLC_ERR_INVALID_ARG, "plan must not be null");
+    double t0 = now_ms();
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    if (plan->count == 0) return JLC_OK;
+    if (!plan->exec) {
+        hipStream_t cap;
+        JLC_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) e = jlc::launch_qr_batched_small(plan->d_descs, plan->count, plan->max_mn, cap);
+        hipGraph_t graph = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cap, &graph);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cap);
+        if (e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            plan->exec = nullptr;
+            return hip_fail(e, "batched QR graph capture");
+        }
+        plan->graph = graph;
+    }
+    if (stats) JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    JLC_HIP_CHECK(hipGraphLaunch(plan->exec, s));
+    if (stats) {
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev1, s));
+        JLC_HIP_CHECK(hipEventSynchronize(t_ctx.ev1));
+        float f = 0.0f;
+        JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
+        jlc::TileInfo info;
+        info.launches = 1;
+        fill_stats(stats, info, JLC_KERNEL_QR, f, now_ms() - t0);
+    }
+    return JLC_OK;
+}
+
+int jlc_qr_batched_plan_destroy(jlc_qr_batched_plan* plan) {
+    if (!plan) return JLC_OK;
+    if (g_alive.load()) {
+        if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+        if (plan->graph) (void)hipGraphDestroy(plan->graph);
+        if (plan->d_descs) (void)hipFree(plan->d_descs);
+    }
+    delete plan;
+    return JLC_OK;
+}
+
+int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream, jlc_stats* stats) {
+    jlc_qr_batched_plan* plan = nullptr;
+    int rc = jlc_qr_batched_plan_create(count, descs, &plan);
+    if (rc != JLC_OK) return rc;
+    double t0 = now_ms();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    if (stats) {
+        hipError_t e = hipEventRecord(t_ctx.ev0, s);
+        if (e != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+    }
+    if (rc == JLC_OK && count) {
+        hipError_t e = jlc::launch_qr_batched_small(plan->d_descs, plan->count, plan->max_mn, s);
+        if (e != hipSuccess) rc = hip_fail(e, "batched QR launch");
+    }
+    if (rc == JLC_OK) {
+        float f = 0.0f;
+        hipError_t e = hipSuccess;
+        if (stats) {
+            e = hipEventRecord(t_ctx.ev1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(t_ctx.ev1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = hip_fail(e, "batched QR");
+        jlc::TileInfo info;
+        info.launches = 1;
+        fill_stats(stats, info, JLC_KERNEL_QR, f, now_ms() - t0);
+    }
+    jlc_qr_batched_plan_destroy(plan);
     return rc;
 }
 

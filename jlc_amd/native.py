@@ -33,7 +33,8 @@ MODE_FP64 = 0
 MODE_SPLIT_INT8 = 1
 MODE_SPLIT_BF16 = 2
 
-KERNEL_NAMES = {0: "none", 1: "scale", 2: "mfma_f64", 3: "gemv", 4: "split_int8", 5: "split_bf16", 6: "grouped"}
+KERNEL_NAMES = {0: "none", 1: "scale", 2: "mfma_f64", 3: "gemv", 4: "split_int8", 5: "split_bf16", 6: "grouped",
+                7: "qr"}
 
 
 def mode_slices(mode: int, slices: int) -> int:
@@ -84,6 +85,11 @@ class GemmDesc(C.Structure):
     ]
 
 
+class QrDesc(C.Structure):
+    """jlc_qr_desc: one matrix of a batched QR (device pointers, column-major)."""
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("lda", C.c_int64), ("a", C.c_void_p), ("tau", C.c_void_p)]
+
+
 _SIGNATURES = {
     "jlc_hip_probe": (_i, [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "jlc_hip_init": (_i, [_i]),
@@ -112,6 +118,13 @@ _SIGNATURES = {
     "jlc_grouped_plan_replay": (_i, [_p, _p, C.POINTER(Stats)]),
     "jlc_grouped_plan_destroy": (_i, [_p]),
     "jlc_dgemm_grouped_dev": (_i, [_i64, _p, _i, _p, C.POINTER(Stats)]),
+    "jlc_dgeqrf_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _i, _p, C.POINTER(Stats)]),
+    "jlc_dorgqr_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p]),
+    "jlc_dqr_errors_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, C.POINTER(_d), C.POINTER(_d)]),
+    "jlc_qr_batched_plan_create": (_i, [_i64, _p, C.POINTER(_p)]),
+    "jlc_qr_batched_plan_replay": (_i, [_p, _p, C.POINTER(Stats)]),
+    "jlc_qr_batched_plan_destroy": (_i, [_p]),
+    "jlc_dgeqrf_batched_dev": (_i, [_i64, _p, _p, C.POINTER(Stats)]),
 }
 
 _lock = threading.Lock()
@@ -323,6 +336,58 @@ def dgemm_grouped_dev(descs, mode: int = MODE_FP64, stream_handle: int | None = 
     arr = (GemmDesc * len(descs))(*descs)
     st = Stats()
     check(load().jlc_dgemm_grouped_dev(len(descs), C.cast(arr, C.c_void_p), mode, stream_handle, C.byref(st)))
+    return st
+
+
+def dgeqrf_dev(m: int, n: int, a: int, lda: int, tau: int, nb: int = 128, mode: int = MODE_FP64,
+               stream_handle: int | None = None) -> Stats:
+    """Blocked Householder QR of the column-major m x n matrix at device pointer a (in place)."""
+    st = Stats()
+    check(load().jlc_dgeqrf_dev(m, n, a, lda, tau, nb, mode, stream_handle, C.byref(st)))
+    return st
+
+
+def dorgqr_dev(m: int, n: int, a: int, lda: int, tau: int, nb: int, q: int, ldq: int) -> None:
+    check(load().jlc_dorgqr_dev(m, n, a, lda, tau, nb, q, ldq, None))
+
+
+def dqr_errors_dev(m: int, n: int, a_orig: int, lda_orig: int, qr: int, lda: int, tau: int, nb: int) -> tuple:
+    """(||A - QR||_F / ||A||_F, ||Q^T Q - I||_F)."""
+    r, o = C.c_double(), C.c_double()
+    check(load().jlc_dqr_errors_dev(m, n, a_orig, lda_orig, qr, lda, tau, nb, C.byref(r), C.byref(o)))
+    return r.value, o.value
+
+
+class QrBatchedPlan:
+    def __init__(self, descs):
+        arr = (QrDesc * len(descs))(*descs)
+        self._arr = arr
+        p = C.c_void_p()
+        check(load().jlc_qr_batched_plan_create(len(descs), C.cast(arr, C.c_void_p), C.byref(p)))
+        self.handle = p.value
+
+    def replay(self, stream_handle: int | None = None, stats: bool = False) -> Stats | None:
+        st = Stats() if stats else None
+        check(load().jlc_qr_batched_plan_replay(self.handle, stream_handle, C.byref(st) if st is not None else None))
+        return st
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(load().jlc_qr_batched_plan_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and _lib is not None:
+                _lib.jlc_qr_batched_plan_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def dgeqrf_batched_dev(descs, stream_handle: int | None = None) -> Stats:
+    arr = (QrDesc * len(descs))(*descs)
+    st = Stats()
+    check(load().jlc_dgeqrf_batched_dev(len(descs), C.cast(arr, C.c_void_p), stream_handle, C.byref(st)))
     return st
 
 

@@ -1,0 +1,123 @@
+"""GPU blocked Householder QR and batched small QR against the oracle and the
+reference's QR acceptance tests (src/test/java/net/faulj/stress/HouseholderQRTest.java,
+src/test/java/net/faulj/decomposition/DecompositionTests.java:183-188, :459-527)."""
+import numpy as np
+import pytest
+
+from jlc_amd import native
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _factor(A, nb=128, mode=native.MODE_FP64):
+    m, n = A.shape
+    a_cm = np.asfortranarray(A).reshape(-1, order="F").copy()
+    da = native.DeviceArray.from_numpy(a_cm)
+    d0 = native.DeviceArray.from_numpy(a_cm)
+    dt = native.DeviceArray(min(m, n))
+    st = native.dgeqrf_dev(m, n, da.ptr, m, dt.ptr, nb, mode)
+    return da, d0, dt, st
+
+
+def _errors(A, da, d0, dt, nb):
+    m, n = A.shape
+    return native.dqr_errors_dev(m, n, d0.ptr, m, da.ptr, m, dt.ptr, nb)
+
+
+def test_reference_square_cases_n_1_to_100():
+    """HouseholderQRTest: n = 1..100 random square, recon and ortho < 1e-12."""
+    rng = np.random.default_rng(12)
+    for n in list(range(1, 21)) + [33, 64, 65, 100]:
+        A = rng.uniform(-1, 1, (n, n))
+        da, d0, dt, _ = _factor(A, nb=32)
+        recon, orth = _errors(A, da, d0, dt, 32)
+        assert recon < 1e-12 and orth < 1e-12, (n, recon, orth)
+
+
+def test_reference_thin_case_and_oracle_parity():
+    """Thin 100 x 40 (HouseholderQRTest.java:100-121): same R and tau as the oracle restatement."""
+    rng = np.random.default_rng(40)
+    for (m, n, nb) in [(100, 40, 16), (300, 300, 64), (513, 200, 128), (1000, 640, 128)]:
+        A = rng.standard_normal((m, n))
+        da, d0, dt, st = _factor(A, nb=nb)
+        assert st.kernel_name == "qr"
+        recon, orth = _errors(A, da, d0, dt, nb)
+        tol = 1e-10 * np.sqrt(n)  # size-graded (DecompositionTests.java:183-188)
+        assert recon < tol and orth < tol, (m, n, recon, orth)
+        ref = np.asfortranarray(A).reshape(-1, order="F").copy()
+        tau_ref = orc.qr_factor(ref, m, n, nb=nb)
+        tau = dt.to_numpy()
+        np.testing.assert_allclose(tau, tau_ref, rtol=1e-10, atol=1e-12)
+        got = da.to_numpy().reshape(n, m).T
+        want = ref.reshape(n, m).T
+        R, Rr = np.triu(got[:n]), np.triu(want[:n])
+        np.testing.assert_allclose(R, Rr, rtol=0, atol=1e-10 * np.max(np.abs(Rr)))
+
+
+def test_graded_spectrum_split_trailing_updates():
+    """A = U diag(s) V^T with s geometric 1 -> 1e-8 (the tall-QR construction) with INT8 split
+    trailing updates: reconstruction and orthogonality stay at FP64 level."""
+    rng = np.random.default_rng(5)
+    m, n = 4096, 1024
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.geomspace(1.0, 1e-8, n)
+    A = (U * s) @ V.T
+    res = {}
+    for name, mode in (("fp64", native.MODE_FP64), ("split", native.MODE_SPLIT_INT8)):
+        da, d0, dt, _ = _factor(A, nb=128, mode=mode)
+        res[name] = _errors(A, da, d0, dt, 128)
+    print(res, flush=True)
+    for name, (recon, orth) in res.items():
+        assert recon < 1e-13 and orth < 1e-12, (name, recon, orth)
+
+
+def _batch(rng, count, lo=16, hi=192):
+    ms = rng.integers(lo, hi + 1, count)
+    ns = np.maximum(1, ms // 2)
+    mats = [rng.standard_normal((int(m), int(n))) for m, n in zip(ms, ns)]
+    flat = np.concatenate([np.asfortranarray(M).reshape(-1, order="F") for M in mats])
+    offs = np.concatenate([[0], np.cumsum([M.size for M in mats])[:-1]])
+    tau_offs = np.concatenate([[0], np.cumsum([min(M.shape) for M in mats])[:-1]])
+    return mats, flat, offs, tau_offs
+
+
+def test_batched_small_qr_against_oracle_and_graph_replay():
+    rng = np.random.default_rng(77)
+    mats, flat, offs, toffs = _batch(rng, 600)
+    da = native.DeviceArray.from_numpy(flat)
+    dt = native.DeviceArray(int(sum(min(M.shape) for M in mats)))
+    descs = [native.QrDesc(M.shape[0], M.shape[1], M.shape[0], da.at(int(o)), dt.at(int(t)))
+             for M, o, t in zip(mats, offs, toffs)]
+    st = native.dgeqrf_batched_dev(descs)
+    assert st.kernel_name == "qr"
+    got = da.to_numpy()
+    taus = dt.to_numpy()
+    for i, M in enumerate(mats):
+        m, n = M.shape
+        f = got[offs[i]:offs[i] + m * n].copy()
+        tau = taus[toffs[i]:toffs[i] + n]
+        ref = np.asfortranarray(M).reshape(-1, order="F").copy()
+        tau_ref = orc.qr_factor(ref, m, n)
+        np.testing.assert_allclose(tau, tau_ref, rtol=1e-10, atol=1e-12)
+        Q = orc.qr_form_q(f, tau, m, n).reshape(n, m).T
+        R = np.triu(f.reshape(n, m).T[:n])
+        assert np.linalg.norm(M - Q @ R) / np.linalg.norm(M) < 1e-13
+        assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13
+    # graph replays give the eager result again, bit for bit
+    plan = native.QrBatchedPlan(descs)
+    for _ in range(3):
+        native.check(native.load().jlc_memcpy_h2d(da.ptr, flat.ctypes.data, flat.size * 8))
+        native.synchronize()
+        plan.replay()
+        native.synchronize()
+        assert np.array_equal(da.to_numpy(), got)
+    plan.destroy()
+
+
+def test_batched_rejects_oversized_matrix():
+    d = native.DeviceArray(8)
+    with pytest.raises(native.JlcError) as ei:
+        native.QrBatchedPlan([native.QrDesc(300, 100, 300, d.ptr, d.ptr)])
+    assert ei.value.status == native.JLC_ERR_INVALID_ARG
