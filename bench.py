@@ -166,6 +166,141 @@ def bench_batched(args) -> int:
     return 0
 
 
+def _graded(rng, m: int, n: int) -> np.ndarray:
+    """A = U diag(s) V^T, U and V from QR of Gaussian matrices, s geometric 1 -> 1e-8."""
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (U * np.geomspace(1.0, 1e-8, n)) @ V.T
+
+
+def bench_qr(args) -> int:
+    """Tall blocked Householder QR: 32768 x 4096, nb = 128, FP64 panels, FP64 or split trailing updates."""
+    m, n, nb = 32768, 4096, 128
+    rng = np.random.default_rng(args.seed)
+    stream = native.stream()
+    t0 = time.perf_counter()
+    # U: thin Q of a Gaussian m x n, V: Q of a Gaussian n x n, both by this library's QR.
+    def q_of_gaussian(r, c):
+        g = native.DeviceArray.from_numpy(rng.standard_normal((c, r)).reshape(-1))  # column-major r x c
+        tau = native.DeviceArray(c)
+        native.dgeqrf_dev(r, c, g.ptr, r, tau.ptr, nb)
+        q = native.DeviceArray(r * c)
+        native.dorgqr_dev(r, c, g.ptr, r, tau.ptr, nb, q.ptr, r)
+        return q
+    U = q_of_gaussian(m, n)
+    V = q_of_gaussian(n, n).to_numpy().reshape(n, n)          # row j = column j of V
+    Vs = (V.T * np.geomspace(1.0, 1e-8, n)).T.copy()          # columns of V scaled by s (column-major)
+    dVs = native.DeviceArray.from_numpy(Vs.reshape(-1))
+    A = native.DeviceArray(m * n)
+    # A^T (n x m row-major = A column-major) = (V diag(s)) U^T
+    native.dgemm_dev(native.COL, native.ROW, n, m, n, 1.0, dVs.ptr, n, U.ptr, m, 0.0, A.ptr, m)
+    native.synchronize()
+    log(f"qr: built A ({m}x{n}) in {time.perf_counter() - t0:.1f}s")
+    flops = 2.0 * m * n * n - 2.0 * n ** 3 / 3.0
+    work = native.DeviceArray(m * n)
+    tau = native.DeviceArray(n)
+    forms = {}
+    for name, mode in (("fp64", native.MODE_FP64), ("split_int8", native.MODE_SPLIT_INT8)):
+        times = []
+        for it in range(args.warmup + max(1, args.steps)):
+            native.dev_copy(work.ptr, A.ptr, m * n * 8, stream)
+            native.synchronize()
+            t1 = time.perf_counter()
+            native.dgeqrf_dev(m, n, work.ptr, m, tau.ptr, nb, mode, stream)
+            native.synchronize()
+            if it >= args.warmup:
+                times.append((time.perf_counter() - t1) * 1e3)
+        ms = float(np.median(times))
+        recon, orth = native.dqr_errors_dev(m, n, A.ptr, m, work.ptr, m, tau.ptr, nb)
+        forms[name] = {"ms": round(ms, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 3),
+                       "recon_err": recon, "orth_err": orth}
+        log(f"qr {name}: {ms:.1f} ms, {forms[name]['tflops']} TFLOP/s, recon {recon:.3e}, orth {orth:.3e}")
+    # CPU path: the oracle's Householder QR restatement on a bounded sample of the same construction.
+    from oracle import oracle as orc
+    cm, cn = 4096, 512
+    As = _graded(np.random.default_rng(args.seed + 2), cm, cn)
+    f = np.asfortranarray(As).reshape(-1, order="F").copy()
+    t1 = time.perf_counter()
+    ctau = orc.qr_factor(f, cm, cn)
+    cpu_s = time.perf_counter() - t1
+    Q = orc.qr_form_q(f, ctau, cm, cn).reshape(cn, cm).T
+    R = np.triu(f.reshape(cn, cm).T[:cn])
+    cpu = {"sample": f"{cm}x{cn} same construction, oracle HouseholderQR restatement, 1 thread",
+           "seconds": round(cpu_s, 2), "gflops": round((2.0 * cm * cn * cn - 2.0 * cn ** 3 / 3) / cpu_s / 1e9, 3),
+           "recon_err": float(np.linalg.norm(As - Q @ R) / np.linalg.norm(As)),
+           "orth_err": float(np.linalg.norm(Q.T @ Q - np.eye(cn)))}
+    best = min(forms.values(), key=lambda f: f["ms"])
+    print(json.dumps({
+        "metric": "blocked Householder QR 32768x4096 (nb=128), A = U diag(s) V^T, s geometric 1..1e-8",
+        "value": best["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic", "config": {"workload": "qr 32768x4096 nb=128", "flops": flops},
+        "forms": forms, "cpu_path": cpu}), flush=True)
+    return 0
+
+
+def bench_qr_batched(args) -> int:
+    """20,000 independent QRs, m ~ U[16,192], n = m/2, one graph-replayed launch."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(args.seed)
+    count = 20000
+    ms_ = rng.integers(16, 193, count)
+    mats = [None] * count
+    for mm in np.unique(ms_):  # stacked numpy QRs per shape
+        idx = np.nonzero(ms_ == mm)[0]
+        nn = max(1, int(mm) // 2)
+        U, _ = np.linalg.qr(rng.standard_normal((len(idx), int(mm), nn)))
+        V, _ = np.linalg.qr(rng.standard_normal((len(idx), nn, nn)))
+        A = (U * np.geomspace(1.0, 1e-8, nn)) @ np.swapaxes(V, 1, 2)
+        for t, i in enumerate(idx):
+            mats[i] = A[t]
+    flat = np.concatenate([np.asfortranarray(M).reshape(-1, order="F") for M in mats])
+    offs = np.concatenate([[0], np.cumsum([M.size for M in mats])[:-1]])
+    toffs = np.concatenate([[0], np.cumsum([min(M.shape) for M in mats])[:-1]])
+    da = native.DeviceArray.from_numpy(flat)
+    dt = native.DeviceArray(int(toffs[-1] + min(mats[-1].shape)))
+    descs = [native.QrDesc(M.shape[0], M.shape[1], M.shape[0], da.at(int(o)), dt.at(int(t)))
+             for M, o, t in zip(mats, offs, toffs)]
+    plan = native.QrBatchedPlan(descs)
+    stream = native.stream()
+    times = []
+    for it in range(args.warmup + max(1, args.steps)):
+        native.check(native.load().jlc_memcpy_h2d(da.ptr, flat.ctypes.data, flat.size * 8))
+        native.synchronize()
+        t1 = time.perf_counter()
+        plan.replay(stream)
+        native.synchronize()
+        if it >= args.warmup:
+            times.append((time.perf_counter() - t1) * 1e3)
+    ms = float(np.median(times))
+    got, taus = da.to_numpy(), dt.to_numpy()
+    flops = float(sum(2.0 * M.shape[0] * M.shape[1] ** 2 - 2.0 * M.shape[1] ** 3 / 3 for M in mats))
+    recon, orth = 0.0, 0.0
+    for i, M in enumerate(mats):
+        m, n = M.shape
+        f = got[offs[i]:offs[i] + m * n].copy()
+        Q = orc.qr_form_q(f, taus[toffs[i]:toffs[i] + n], m, n).reshape(n, m).T
+        R = np.triu(f.reshape(n, m).T[:n])
+        recon = max(recon, float(np.linalg.norm(M - Q @ R) / np.linalg.norm(M)))
+        orth = max(orth, float(np.linalg.norm(Q.T @ Q - np.eye(n))))
+    # CPU path on the first 2000 matrices
+    t1 = time.perf_counter()
+    for M in mats[:2000]:
+        f = np.asfortranarray(M).reshape(-1, order="F").copy()
+        orc.qr_factor(f, M.shape[0], M.shape[1])
+    cpu_ms = (time.perf_counter() - t1) * 1e3
+    print(json.dumps({
+        "metric": "batched Householder QR, 20000 matrices m~U[16,192], n=m/2, one graph launch",
+        "value": round(count / (ms * 1e-3), 1), "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "20000 QRs, A = U diag(s) V^T per matrix", "flops": flops},
+        "tflops": round(flops / (ms * 1e-3) / 1e12, 4), "max_recon_err": recon, "max_orth_err": orth,
+        "cpu_path": {"sample": "first 2000 matrices, oracle HouseholderQR restatement, 1 thread",
+                     "ms": round(cpu_ms, 1), "matrices_per_s": round(2000 / (cpu_ms * 1e-3), 1)}}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,7 +313,7 @@ def main() -> int:
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="time only the FP64 MFMA form")
     ap.add_argument("--slices", type=int, default=0, help="INT8 slice count for the split form (0 = auto)")
-    ap.add_argument("--workload", choices=["dgemm", "batched"], default="dgemm",
+    ap.add_argument("--workload", choices=["dgemm", "batched", "qr", "qr_batched"], default="dgemm",
                     help="dgemm: the 16384^3 metric workload; batched: the grouped-GEMM configuration")
     ap.add_argument("--count", type=int, default=100000, help="problems in the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -205,6 +340,10 @@ def main() -> int:
 
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "qr":
+        return bench_qr(args)
+    if args.workload == "qr_batched":
+        return bench_qr_batched(args)
 
     m = n = k = args.size
     rng = np.random.default_rng(args.seed)
