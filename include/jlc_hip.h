@@ -174,6 +174,15 @@ int jlc_qr_batched_plan_replay(jlc_qr_batched_plan* plan, void* stream, jlc_stat
 int jlc_qr_batched_plan_destroy(jlc_qr_batched_plan* plan);
 int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream, jlc_stats* stats);
 
+/* ---- tuning table: the autotuner's choices, read by dispatch. Keys:
+ * hip_min_flops (offload gate, the HIP analogue of GemmDispatch.CUDA_THRESHOLD,
+ * src/main/java/net/faulj/compute/GemmDispatch.java:32), splitk_tiles_per_cu,
+ * splitk_min_kslice, grouped_waves_per_simd. Text file, one "key value" per line. */
+int jlc_tune_load(const char* path);
+int jlc_tune_write(const char* path);
+int jlc_tune_get(const char* key, double* value);
+int jlc_tune_set(const char* key, double value);
+
 /* ---- device memory helpers for callers that keep operands resident. */
 int jlc_dev_alloc(void** ptr, size_t bytes);
 int jlc_dev_free(void* ptr);

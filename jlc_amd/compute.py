@@ -151,7 +151,8 @@ class DispatchPolicy:
 
     @staticmethod
     def defaultPolicy() -> "DispatchPolicy":
-        return DispatchPolicy()
+        """Default policy: the HIP gate comes from the loaded tuning table."""
+        return DispatchPolicy(hip_min_flops=int(native.tune_get("hip_min_flops")))
 
     def getParallelism(self) -> int:
         return self.parallelism

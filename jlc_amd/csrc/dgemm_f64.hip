@@ -317,12 +317,14 @@ hipError_t launch_scale(int64_t m, int64_t n, double beta, double* c, int64_t ld
     return hipGetLastError();
 }
 
-int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus) {
+int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus, int tiles_per_cu, int min_kslice) {
     // Few output tiles and a long K: give every CU work by cutting K.
+    if (tiles_per_cu <= 0) return 1;
     const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
-    const int64_t want = (2 * (int64_t)device_cus + tiles - 1) / tiles;
+    const int64_t want = ((int64_t)tiles_per_cu * device_cus + tiles - 1) / tiles;
     int64_t s = want;
-    if (s > k / 1024) s = k / 1024;  // keep at least 1024 of K per slice
+    if (min_kslice < BK) min_kslice = BK;
+    if (s > k / min_kslice) s = k / min_kslice;
     if (s > 64) s = 64;
     return s < 2 ? 1 : (int)s;
 }

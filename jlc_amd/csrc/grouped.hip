@@ -115,7 +115,7 @@ __global__ __launch_bounds__(64 * WAVES_PER_GROUP) void grouped_f64(const Groupe
 
 }  // namespace
 
-int grouped_grid(int device_cus) {
+int grouped_grid(int device_cus, int waves_per_simd) {
     // Exactly the resident capacity: with static round-robin dealing, a
     // workgroup that only starts after another finishes would run its whole
     // share late.
@@ -123,6 +123,8 @@ int grouped_grid(int device_cus) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_f64, 64 * WAVES_PER_GROUP, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 2;
+    // A tuned width below the occupancy limit (waves per SIMD == workgroups per CU here).
+    if (waves_per_simd > 0 && waves_per_simd < per_cu) per_cu = waves_per_simd;
     return device_cus * per_cu;
 }
 

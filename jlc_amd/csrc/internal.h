@@ -17,6 +17,15 @@ struct TileInfo {
 hipError_t launch_scale(int64_t m, int64_t n, double beta, double* c, int64_t ldc, hipStream_t stream,
                         TileInfo* info);
 
+// Tuned parameters (read by dispatch; written by the autotuner's table).
+struct TuneTable {
+    double hip_min_flops = 2.0e8;     // host-side offload gate (GemmDispatch reads it)
+    int splitk_tiles_per_cu = 2;      // split K until slices * tiles >= this * CUs (0 = never split)
+    int splitk_min_kslice = 1024;     // smallest K per slice
+    int grouped_waves_per_simd = 0;   // persistent grouped grid (0 = occupancy limit)
+};
+TuneTable tune_snapshot();
+
 // FP64 MFMA GEMM, device pointers, layouts as in jlc_hip.h. Requires m, n, k > 0 and alpha != 0.
 hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha,
                             const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
@@ -24,7 +33,7 @@ hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, in
 
 // Split-K variant: K cut into `slices` pieces whose partial products go to
 // work (slices * m * n doubles) and are summed in order into C.
-int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus);
+int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus, int tiles_per_cu, int min_kslice);
 hipError_t launch_dgemm_f64_splitk(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha,
                                    const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                                    double* c, int64_t ldc, int slices, double* work, hipStream_t stream,
@@ -77,7 +86,7 @@ struct GroupedDesc {
 };
 
 // Persistent grid size (workgroups) for the grouped kernel on a device with device_cus CUs.
-int grouped_grid(int device_cus);
+int grouped_grid(int device_cus, int waves_per_simd);
 // tiles: (problem << 32) | (tile_row << 16) | tile_col, 32 x 32 tiles.
 hipError_t launch_grouped_f64(const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
                               hipStream_t stream);

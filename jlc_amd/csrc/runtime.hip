@@ -34,6 +34,9 @@ std::atomic<bool> g_alive{true};
 
 thread_local std::string t_error;
 
+std::mutex g_tune_mutex;
+jlc::TuneTable g_tune;
+
 int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -71,9 +74,13 @@ struct ThreadCtx {
     static constexpr int NBUF = 7;
     void* buf[NBUF] = {};
     size_t cap[NBUF] = {};
+    // Pinned host staging for the host-pointer entry points.
+    void* pin = nullptr;
+    size_t pin_cap = 0;
 
     ~ThreadCtx() {
         if (!g_alive.load() || device < 0) return;
+        if (pin) (void)hipHostFree(pin);
         for (int i = 0; i < NBUF; i++)
             if (buf[i]) (void)hipFree(buf[i]);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -133,6 +140,43 @@ int ensure_buf(int slot, size_t bytes) {
         return fail(JLC_ERR_ALLOC, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
     }
     c.cap[slot] = want;
+    return JLC_OK;
+}
+
+int ensure_pin(size_t bytes) {
+    ThreadCtx& c = t_ctx;
+    if (bytes <= c.pin_cap) return JLC_OK;
+    if (c.pin) {
+        JLC_HIP_CHECK(hipHostFree(c.pin));
+        c.pin = nullptr;
+        c.pin_cap = 0;
+    }
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    hipError_t e = hipHostMalloc(&c.pin, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        c.pin = nullptr;
+        return fail(JLC_ERR_ALLOC, "hipHostMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    c.pin_cap = want;
+    return JLC_OK;
+}
+
+// Packs a rows x cols block (row pitch spitch elements) into pinned memory
+// and queues one copy to the device block (row pitch dpitch elements).
+int stage_to_device(double* dst, int64_t dpitch, const double* src, int64_t spitch, int64_t rows, int64_t cols,
+                    double* pin, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return JLC_OK;
+    if (spitch == cols) {
+        std::memcpy(pin, src, (size_t)(rows * cols) * sizeof(double));
+    } else {
+        for (int64_t r = 0; r < rows; r++)
+            std::memcpy(pin + r * cols, src + r * spitch, (size_t)cols * sizeof(double));
+    }
+    if (dpitch == cols)
+        JLC_HIP_CHECK(hipMemcpyAsync(dst, pin, (size_t)(rows * cols) * sizeof(double), hipMemcpyHostToDevice, s));
+    else
+        JLC_HIP_CHECK(hipMemcpy2DAsync(dst, dpitch * sizeof(double), pin, cols * sizeof(double),
+                                       cols * sizeof(double), rows, hipMemcpyHostToDevice, s));
     return JLC_OK;
 }
 
@@ -247,7 +291,8 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
         return JLC_OK;
     }
     *kernel = JLC_KERNEL_MFMA_F64;
-    const int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load());
+    const jlc::TuneTable tt = jlc::tune_snapshot();
+    const int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load(), tt.splitk_tiles_per_cu, tt.splitk_min_kslice);
     if (slices > 1) {
         int rc = ensure_buf(6, (size_t)slices * (size_t)(m * n) * sizeof(double));
         if (rc != JLC_OK) return rc;
@@ -273,7 +318,94 @@ void fill_stats(jlc_stats* s, const jlc::TileInfo& info, int kernel, double kms,
 
 }  // namespace
 
+namespace jlc {
+TuneTable tune_snapshot() {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    return g_tune;
+}
+}  // namespace jlc
+
+namespace {
+// Reads or writes one named entry of a table (false for an unknown key).
+bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* set) {
+    struct F {
+        const char* name;
+        double jlc::TuneTable::*d;
+        int jlc::TuneTable::*i;
+    };
+    static const F fields[] = {
+        {"hip_min_flops", &jlc::TuneTable::hip_min_flops, nullptr},
+        {"splitk_tiles_per_cu", nullptr, &jlc::TuneTable::splitk_tiles_per_cu},
+        {"splitk_min_kslice", nullptr, &jlc::TuneTable::splitk_min_kslice},
+        {"grouped_waves_per_simd", nullptr, &jlc::TuneTable::grouped_waves_per_simd},
+    };
+    for (const F& f : fields) {
+        if (std::strcmp(f.name, key) != 0) continue;
+        if (set) {
+            if (f.d) t.*(f.d) = *set;
+            else t.*(f.i) = (int)*set;
+        }
+        if (get) *get = f.d ? t.*(f.d) : (double)(t.*(f.i));
+        return true;
+    }
+    return false;
+}
+}  // namespace
+
 extern "C" {
+
+int jlc_tune_get(const char* key, double* value) {
+    if (!key || !value) return fail(JLC_ERR_INVALID_ARG, "key and value must not be null");
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (!tune_field(g_tune, key, value, nullptr)) return fail(JLC_ERR_INVALID_ARG, "unknown tuning key '%s'", key);
+    return JLC_OK;
+}
+
+int jlc_tune_set(const char* key, double value) {
+    if (!key) return fail(JLC_ERR_INVALID_ARG, "key must not be null");
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (!tune_field(g_tune, key, nullptr, &value)) return fail(JLC_ERR_INVALID_ARG, "unknown tuning key '%s'", key);
+    return JLC_OK;
+}
+
+int jlc_tune_load(const char* path) {
+    if (!path) return fail(JLC_ERR_INVALID_ARG, "path must not be null");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(JLC_ERR_INVALID_ARG, "cannot open tuning table %s", path);
+    jlc::TuneTable t = jlc::tune_snapshot();
+    char line[512];
+    int lineno = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        lineno++;
+        char* p = line;
+        while (*p == ' ' || *p == '\t') p++;
+        if (*p == '#' || *p == '\n' || *p == '\0') continue;
+        char key[128];
+        double v;
+        if (std::sscanf(p, "%127s %lf", key, &v) != 2 || !tune_field(t, key, nullptr, &v)) {
+            std::fclose(f);
+            return fail(JLC_ERR_INVALID_ARG, "%s:%d: bad tuning entry", path, lineno);
+        }
+    }
+    std::fclose(f);
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune = t;
+    return JLC_OK;
+}
+
+int jlc_tune_write(const char* path) {
+    if (!path) return fail(JLC_ERR_INVALID_ARG, "path must not be null");
+    const jlc::TuneTable t = jlc::tune_snapshot();
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(JLC_ERR_INVALID_ARG, "cannot write %s", path);
+    std::fprintf(f, "# libjlc_hip tuning table (key value); written by jlc_amd/tune.py\n");
+    std::fprintf(f, "hip_min_flops %.17g\n", t.hip_min_flops);
+    std::fprintf(f, "splitk_tiles_per_cu %d\n", t.splitk_tiles_per_cu);
+    std::fprintf(f, "splitk_min_kslice %d\n", t.splitk_min_kslice);
+    std::fprintf(f, "grouped_waves_per_simd %d\n", t.grouped_waves_per_simd);
+    std::fclose(f);
+    return JLC_OK;
+}
 
 const char* jlc_version(void) { return "libjlc_hip 0.1 gfx950"; }
 
@@ -387,23 +519,31 @@ int jlc_dgemm(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, doubl
     double* da = static_cast<double*>(ctx.buf[0]);
     double* db = static_cast<double*>(ctx.buf[1]);
     double* dc = static_cast<double*>(ctx.buf[2]);
+    // Pinned staging: A, B, C regions packed contiguously, one DMA copy each.
+    const int64_t a_el = use_ab ? a_rows * a_cols : 0, b_el = use_ab ? b_rows * b_cols : 0, c_el = m * n;
+    if ((rc = ensure_pin((size_t)(a_el + b_el + c_el) * sizeof(double))) != JLC_OK) return rc;
+    double* pin = static_cast<double*>(ctx.pin);
+    double* pin_a = pin;
+    double* pin_b = pin + a_el;
+    double* pin_c = pin + a_el + b_el;
     if (use_ab) {
-        JLC_HIP_CHECK(hipMemcpy2DAsync(da, dlda * sizeof(double), a, lda * sizeof(double), a_cols * sizeof(double),
-                                       a_rows, hipMemcpyHostToDevice, s));
-        JLC_HIP_CHECK(hipMemcpy2DAsync(db, dldb * sizeof(double), b, ldb * sizeof(double), b_cols * sizeof(double),
-                                       b_rows, hipMemcpyHostToDevice, s));
+        if ((rc = stage_to_device(da, dlda, a, lda, a_rows, a_cols, pin_a, s)) != JLC_OK) return rc;
+        if ((rc = stage_to_device(db, dldb, b, ldb, b_rows, b_cols, pin_b, s)) != JLC_OK) return rc;
     }
-    if (beta != 0.0)
-        JLC_HIP_CHECK(hipMemcpy2DAsync(dc, dldc * sizeof(double), c, ldc * sizeof(double), n * sizeof(double), m,
-                                       hipMemcpyHostToDevice, s));
+    if (beta != 0.0 && (rc = stage_to_device(dc, dldc, c, ldc, m, n, pin_c, s)) != JLC_OK) return rc;
     int kernel = JLC_KERNEL_NONE;
     JLC_HIP_CHECK(hipEventRecord(ctx.ev0, s));
     rc = run_gemm_device(layout_a, layout_b, m, n, k, alpha, da, dlda, db, dldb, beta, dc, dldc, mode, s, &info,
                          &kernel);
     if (rc != JLC_OK) return rc;
     JLC_HIP_CHECK(hipEventRecord(ctx.ev1, s));
-    JLC_HIP_CHECK(hipMemcpy2DAsync(c, ldc * sizeof(double), dc, dldc * sizeof(double), n * sizeof(double), m,
-                                   hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipMemcpyAsync(pin_c, dc, (size_t)(m * n) * sizeof(double), hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    if (ldc == n) {
+        std::memcpy(c, pin_c, (size_t)(m * n) * sizeof(double));
+    } else {
+        for (int64_t r = 0; r < m; r++) std::memcpy(c + r * ldc, pin_c + r * n, (size_t)n * sizeof(double));
+    }
     JLC_HIP_CHECK(hipStreamSynchronize(s));
     float f = 0.0f;
     JLC_HIP_CHECK(hipEventElapsedTime(&f, ctx.ev0, ctx.ev1));
@@ -467,20 +607,26 @@ int jlc_dgemv(int64_t m, int64_t k, double alpha, const double* a, int64_t lda, 
     double* da = static_cast<double*>(ctx.buf[0]);
     double* dx = static_cast<double*>(ctx.buf[1]);
     double* dy = static_cast<double*>(ctx.buf[2]);
+    const int64_t a_el = use_ax ? m * k : 0, x_el = use_ax ? k : 0;
+    if ((rc = ensure_pin((size_t)(a_el + x_el + m) * sizeof(double))) != JLC_OK) return rc;
+    double* pin_a = static_cast<double*>(ctx.pin);
+    double* pin_x = pin_a + a_el;
+    double* pin_y = pin_x + x_el;
     if (use_ax) {
-        JLC_HIP_CHECK(hipMemcpy2DAsync(da, dlda * sizeof(double), a, lda * sizeof(double), k * sizeof(double), m,
-                                       hipMemcpyHostToDevice, s));
-        JLC_HIP_CHECK(hipMemcpy2DAsync(dx, sizeof(double), x, incx * sizeof(double), sizeof(double), k,
-                                       hipMemcpyHostToDevice, s));
+        if ((rc = stage_to_device(da, dlda, a, lda, m, k, pin_a, s)) != JLC_OK) return rc;
+        for (int64_t i = 0; i < k; i++) pin_x[i] = x[i * incx];
+        JLC_HIP_CHECK(hipMemcpyAsync(dx, pin_x, (size_t)k * sizeof(double), hipMemcpyHostToDevice, s));
     }
-    if (beta != 0.0)
-        JLC_HIP_CHECK(hipMemcpy2DAsync(dy, sizeof(double), y, incy * sizeof(double), sizeof(double), m,
-                                       hipMemcpyHostToDevice, s));
+    if (beta != 0.0) {
+        for (int64_t i = 0; i < m; i++) pin_y[i] = y[i * incy];
+        JLC_HIP_CHECK(hipMemcpyAsync(dy, pin_y, (size_t)m * sizeof(double), hipMemcpyHostToDevice, s));
+    }
     JLC_HIP_CHECK(hipEventRecord(ctx.ev0, s));
     JLC_HIP_CHECK(jlc::launch_dgemv(m, k, alpha, da, dlda, dx, 1, beta, dy, 1, s, &info));
     JLC_HIP_CHECK(hipEventRecord(ctx.ev1, s));
-    JLC_HIP_CHECK(hipMemcpy2DAsync(y, incy * sizeof(double), dy, sizeof(double), sizeof(double), m,
-                                   hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipMemcpyAsync(pin_y, dy, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < m; i++) y[i * incy] = pin_y[i];
     JLC_HIP_CHECK(hipStreamSynchronize(s));
     float f = 0.0f;
     JLC_HIP_CHECK(hipEventElapsedTime(&f, ctx.ev0, ctx.ev1));
@@ -621,7 +767,8 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     pl->ntiles = (int)tiles.size();
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, pl->device);
-    pl->grid = jlc::grouped_grid(e == hipSuccess ? prop.multiProcessorCount : 256);
+    pl->grid = jlc::grouped_grid(e == hipSuccess ? prop.multiProcessorCount : 256,
+                                 jlc::tune_snapshot().grouped_waves_per_simd);
     e = hipMalloc(&pl->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::GroupedDesc));
     if (e == hipSuccess) e = hipMalloc(&pl->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(uint64_t));
     if (e == hipSuccess && count)

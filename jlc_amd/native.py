@@ -16,6 +16,8 @@ from pathlib import Path
 import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libjlc_hip.so"
+# The autotuner's table for MI355X; loaded with the library when present.
+TUNE_TABLE = Path(__file__).resolve().parent / "tuning" / "mi355x.tune"
 
 JLC_OK = 0
 JLC_ERR_INVALID_ARG = 1
@@ -118,6 +120,10 @@ _SIGNATURES = {
     "jlc_grouped_plan_replay": (_i, [_p, _p, C.POINTER(Stats)]),
     "jlc_grouped_plan_destroy": (_i, [_p]),
     "jlc_dgemm_grouped_dev": (_i, [_i64, _p, _i, _p, C.POINTER(Stats)]),
+    "jlc_tune_load": (_i, [C.c_char_p]),
+    "jlc_tune_write": (_i, [C.c_char_p]),
+    "jlc_tune_get": (_i, [C.c_char_p, C.POINTER(_d)]),
+    "jlc_tune_set": (_i, [C.c_char_p, _d]),
     "jlc_dgeqrf_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _i, _p, C.POINTER(Stats)]),
     "jlc_dorgqr_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p]),
     "jlc_dqr_errors_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, C.POINTER(_d), C.POINTER(_d)]),
@@ -148,6 +154,10 @@ def load() -> C.CDLL:
             fn.restype = res
             fn.argtypes = args
         _lib = lib
+        table = os.environ.get("JLC_TUNE_TABLE", str(TUNE_TABLE))
+        if table and Path(table).exists():
+            if lib.jlc_tune_load(table.encode()) != JLC_OK:
+                raise JlcError(JLC_ERR_INVALID_ARG, (lib.jlc_last_error() or b"").decode())
         return lib
 
 
@@ -389,6 +399,24 @@ def dgeqrf_batched_dev(descs, stream_handle: int | None = None) -> Stats:
     st = Stats()
     check(load().jlc_dgeqrf_batched_dev(len(descs), C.cast(arr, C.c_void_p), stream_handle, C.byref(st)))
     return st
+
+
+def tune_get(key: str) -> float:
+    v = C.c_double()
+    check(load().jlc_tune_get(key.encode(), C.byref(v)))
+    return v.value
+
+
+def tune_set(key: str, value: float) -> None:
+    check(load().jlc_tune_set(key.encode(), float(value)))
+
+
+def tune_write(path: str) -> None:
+    check(load().jlc_tune_write(str(path).encode()))
+
+
+def tune_load(path: str) -> None:
+    check(load().jlc_tune_load(str(path).encode()))
 
 
 def header_symbols() -> list[str]:

@@ -1,0 +1,142 @@
+"""Autotuner for libjlc_hip on MI355X: measures candidate settings on the GPU
+and writes the table (jlc_amd/tuning/mi355x.tune) that dispatch reads.
+
+Tuned entries:
+* splitk_tiles_per_cu / splitk_min_kslice: when a GEMM has few output tiles and
+  a long K (the Gram matrix and W = C^T V of blocked QR, tall-skinny products),
+  K is cut into slices; candidates are timed on such shapes and on square ones
+  (which must not regress).
+* grouped_waves_per_simd: width of the persistent grouped launch, timed on a
+  batch of uneven problems.
+* hip_min_flops: the offload gate of the host-side dispatch (the HIP analogue
+  of GemmDispatch.CUDA_THRESHOLD, src/main/java/net/faulj/compute/GemmDispatch.java:32).
+  The end-to-end host-pointer GEMM (staging included) is timed per size and
+  compared with the CPU cost model of GemmDispatch.preferCuda
+  (GemmDispatch.java:155-178: 100 GFLOP/s unless --cpu-gflops says otherwise);
+  the gate is the smallest flop count from which the GPU stays faster.
+
+Usage: python -m jlc_amd.tune [--out PATH] [--cpu-gflops G] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import native
+from .workloads import BatchedWorkload, batched_shapes
+
+
+def _time_dev(fn, iters: int) -> float:
+    fn()
+    native.synchronize()
+    t = native.Timer(native.stream())
+    t.start()
+    for _ in range(iters):
+        fn()
+    return t.stop() / iters
+
+
+def tune_splitk(quick: bool) -> dict:
+    shapes = [(128, 128, 32768), (3968, 128, 32768), (1024, 128, 16384), (2048, 2048, 16384), (8192, 8192, 4096)]
+    if quick:
+        shapes = shapes[:3]
+    cands = [(0, 1024), (1, 1024), (2, 1024), (4, 1024), (2, 512), (4, 512), (2, 2048), (8, 512)]
+    rng = np.random.default_rng(0)
+    bufs = {}
+    for (m, n, k) in shapes:
+        bufs[(m, n, k)] = (native.DeviceArray.from_numpy(rng.uniform(-1, 1, m * k)),
+                           native.DeviceArray.from_numpy(rng.uniform(-1, 1, k * n)), native.DeviceArray(m * n))
+    times = {}
+    for tpc, mks in cands:
+        native.tune_set("splitk_tiles_per_cu", tpc)
+        native.tune_set("splitk_min_kslice", mks)
+        for shp, (a, b, c) in bufs.items():
+            m, n, k = shp
+            times[(tpc, mks, shp)] = _time_dev(
+                lambda: native.dgemm_dev(0, 0, m, n, k, 1.0, a.ptr, k, b.ptr, n, 0.0, c.ptr, n, stream=native.stream(),
+                                         stats=False), 3)
+    best_per_shape = {shp: min(times[(t, k, shp)] for t, k in cands) for shp in bufs}
+    score = {}
+    for tpc, mks in cands:
+        score[(tpc, mks)] = math.exp(np.mean([math.log(times[(tpc, mks, s)] / best_per_shape[s]) for s in bufs]))
+    best = min(score, key=score.get)
+    native.tune_set("splitk_tiles_per_cu", best[0])
+    native.tune_set("splitk_min_kslice", best[1])
+    return {"chosen": {"splitk_tiles_per_cu": best[0], "splitk_min_kslice": best[1]},
+            "score_vs_best": {f"{t},{k}": round(v, 4) for (t, k), v in score.items()},
+            "ms": {f"{t},{k} {s}": round(v, 3) for (t, k, s), v in times.items()}}
+
+
+def tune_grouped(quick: bool) -> dict:
+    dims, alpha, beta = batched_shapes(5000 if quick else 20000, seed=3)
+    w = BatchedWorkload(dims, alpha, beta, seed=4)
+    descs = w.descs()
+    res = {}
+    for wps in (0, 1, 2, 3):
+        native.tune_set("grouped_waves_per_simd", wps)
+        plan = native.GroupedPlan(descs)
+        res[wps] = _time_dev(lambda: plan.run(native.stream()), 5)
+        plan.destroy()
+    best = min(res, key=res.get)
+    native.tune_set("grouped_waves_per_simd", best)
+    return {"chosen": {"grouped_waves_per_simd": best}, "ms": {str(k): round(v, 4) for k, v in res.items()}}
+
+
+def tune_offload(cpu_gflops: float, quick: bool) -> dict:
+    sizes = [32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024]
+    if quick:
+        sizes = sizes[::2]
+    rng = np.random.default_rng(1)
+    rows = []
+    for n in sizes:
+        a, b = rng.uniform(-1, 1, n * n), rng.uniform(-1, 1, n * n)
+        c = np.zeros(n * n)
+        native.dgemm(0, 0, n, n, n, 1.0, a, n, b, n, 0.0, c, n)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            native.dgemm(0, 0, n, n, n, 1.0, a, n, b, n, 0.0, c, n)
+        gpu_ms = (time.perf_counter() - t0) * 1e3 / reps
+        cpu_ms = 2.0 * n ** 3 / (cpu_gflops * 1e9) * 1e3
+        rows.append((n, 2 * n ** 3, gpu_ms, cpu_ms))
+    gate = rows[-1][1]
+    for i in range(len(rows)):
+        if all(r[2] < r[3] for r in rows[i:]):
+            gate = rows[i][1]
+            break
+    native.tune_set("hip_min_flops", float(gate))
+    return {"chosen": {"hip_min_flops": gate}, "cpu_model_gflops": cpu_gflops,
+            "rows": [{"n": n, "flops": f, "gpu_end_to_end_ms": round(g, 4), "cpu_model_ms": round(c, 4)}
+                     for n, f, g, c in rows]}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=str(native.TUNE_TABLE))
+    ap.add_argument("--cpu-gflops", type=float, default=100.0)
+    ap.add_argument("--quick", action="store_true")
+    args = ap.parse_args(argv)
+    rc, count, name = native.probe()
+    if rc != native.JLC_OK:
+        print(f"no usable device: {native.last_error()}", file=sys.stderr)
+        return 1
+    report = {"device": name}
+    report["splitk"] = tune_splitk(args.quick)
+    report["grouped"] = tune_grouped(args.quick)
+    report["offload"] = tune_offload(args.cpu_gflops, args.quick)
+    out = Path(args.out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    native.tune_write(str(out))
+    report["table"] = out.read_text()
+    print(json.dumps(report, indent=1), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
