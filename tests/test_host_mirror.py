@@ -25,7 +25,7 @@ def test_select_kernel_reference_order_without_hip():
 
 
 def test_select_kernel_hip_gate():
-    pol = DispatchPolicy.defaultPolicy()
+    pol = DispatchPolicy.builder().hipMinFlops(200_000_000).build()  # the CUDA_THRESHOLD value
     assert GemmDispatch.selectKernel(512, 512, 512, True, 8, pol) is K.HIP  # 2.68e8 >= 2e8
     assert GemmDispatch.selectKernel(256, 256, 256, True, 8, pol) is K.PARALLEL_MICRO  # 3.4e7 < 2e8
     big = DispatchPolicy.builder().hipMinFlops(100_000_000).build()
