@@ -32,6 +32,7 @@ from jlc_amd import native  # noqa: E402
 FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 INT8_MFMA_PEAK_TOPS = 5000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 METRIC = "DGEMM 16384³ TFLOP/s at stated max rel err: split-MFMA vs FP64 MFMA"
 
 
@@ -372,10 +373,12 @@ def main() -> int:
         if st.slices:
             pairs = st.slices * (st.slices + 1) // 2
             ops = pairs * flops
+            peak = BF16_MFMA_PEAK_TFLOPS if name == "split_bf16" else INT8_MFMA_PEAK_TOPS
+            unit = "TFLOP/s" if name == "split_bf16" else "TOP/s"
             form["roofline"] = {"bound": "mfma", "achieved": round(ops / (dev_ms * 1e-3) / 1e12, 1),
-                                "peak": INT8_MFMA_PEAK_TOPS, "unit": "TOP/s",
-                                "frac": round(ops / (dev_ms * 1e-3) / 1e12 / INT8_MFMA_PEAK_TOPS, 4),
-                                "note": f"{pairs} INT8 slice products of 2MNK ops; time includes slicing"}
+                                "peak": peak, "unit": unit,
+                                "frac": round(ops / (dev_ms * 1e-3) / 1e12 / peak, 4),
+                                "note": f"{pairs} slice products of 2MNK ops; time includes slicing"}
         log(f"{name}: {wall_ms:.2f} ms/step wall, {dev_ms:.2f} ms device, {form['tflops']:.2f} TFLOP/s"
             f"{' slices=%d' % st.slices if st.slices else ''}")
         if not args.no_error:
@@ -386,6 +389,7 @@ def main() -> int:
     fp64 = run_form("fp64_mfma", native.MODE_FP64)
     split = None if args.no_split else run_form("split_int8", native.mode_slices(native.MODE_SPLIT_INT8,
                                                                                  args.slices))
+    split_bf16 = None if args.no_split else run_form("split_bf16", native.mode_slices(native.MODE_SPLIT_BF16, 0))
     # value: best throughput among forms whose max relative error is at most the FP64 MFMA form's.
     best = fp64
     if split is not None and split["tflops"] > fp64["tflops"]:
@@ -411,7 +415,7 @@ def main() -> int:
                    "value_rule": "best TFLOP/s among forms with max rel err <= the FP64 MFMA form's",
                    "parallelism": "single_gpu"},
         "error": best.get("error"),
-        "forms": {f["form"]: f for f in (fp64, split) if f is not None},
+        "forms": {f["form"]: f for f in (fp64, split, split_bf16) if f is not None},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
                      "traffic": traffic, "kernel": "dgemm_f64_mfma", "kernel_ms": fp64["device_ms"]},

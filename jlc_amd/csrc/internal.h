@@ -111,6 +111,14 @@ hipError_t launch_sumsq_partial(const double* x, int64_t ld, int rows, int cols,
                                 hipStream_t stream);
 hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_mn, hipStream_t stream);
 
+// BF16-slice variant (8-bit digits, slices 3..7).
+constexpr int SPLIT_BF16_MAX_SLICES = 7;
+void split_bf16_plan_sizes(int64_t m, int64_t n, int64_t k, int slices, SplitPlan* p);
+int64_t split_bf16_chunk_k(int slices, int64_t k);
+hipError_t launch_split_bf16(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
+                             const SplitPlan& p, double alpha, double beta, double* c, int64_t ldc,
+                             hipStream_t stream, TileInfo* info);
+
 // Device-to-device copy kernel (bytes may be any size).
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 

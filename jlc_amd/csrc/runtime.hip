@@ -203,6 +203,10 @@ int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha,
         (slices < jlc::SPLIT_MIN_SLICES || slices > jlc::SPLIT_MAX_SLICES))
         return fail(JLC_ERR_INVALID_ARG, "INT8 slice count %d outside [%d, %d]", slices, jlc::SPLIT_MIN_SLICES,
                     jlc::SPLIT_MAX_SLICES);
+    if (form == JLC_MODE_SPLIT_BF16 && slices != 0 &&
+        (slices < jlc::SPLIT_MIN_SLICES || slices > jlc::SPLIT_BF16_MAX_SLICES))
+        return fail(JLC_ERR_INVALID_ARG, "BF16 slice count %d outside [%d, %d]", slices, jlc::SPLIT_MIN_SLICES,
+                    jlc::SPLIT_BF16_MAX_SLICES);
     if (m == 0 || n == 0) return JLC_OK;
     if (!c) return fail(JLC_ERR_INVALID_ARG, "C must not be null");
     if (ldc < n) return fail(JLC_ERR_INVALID_ARG, "ldc=%lld < n=%lld", (long long)ldc, (long long)n);
@@ -221,25 +225,31 @@ int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha,
 // FP64 product-sum errs by about 2^-53 of the terms' typical size; spread_a /
 // spread_b are the largest row / column gaps (in bits) between the scale 2^e
 // and the mean |x|, so matching FP64 needs 7S >= 53 + spread_a + spread_b - 4.
-int choose_slices(const float* spread_a, int64_t m, const float* spread_b, int64_t n) {
+int choose_slices(const float* spread_a, int64_t m, const float* spread_b, int64_t n, int digit_bits,
+                  int max_slices) {
     float sa = 0.0f, sb = 0.0f;
     for (int64_t i = 0; i < m; i++) sa = spread_a[i] > sa ? spread_a[i] : sa;
     for (int64_t j = 0; j < n; j++) sb = spread_b[j] > sb ? spread_b[j] : sb;
     double bits = 53.0 + sa + sb - 4.0;
-    int s = (int)std::ceil(bits / 7.0);
+    int s = (int)std::ceil(bits / digit_bits);
     if (s < jlc::SPLIT_MIN_SLICES) s = jlc::SPLIT_MIN_SLICES;
-    if (s > jlc::SPLIT_MAX_SLICES) s = jlc::SPLIT_MAX_SLICES;
+    if (s > max_slices) s = max_slices;
     return s;
 }
 
 int ensure_buf(int slot, size_t bytes);
 
-int run_split_int8(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
-                   const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int slices,
-                   hipStream_t stream, jlc::TileInfo* info) {
+int run_split(bool bf16, int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+              int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int slices,
+              hipStream_t stream, jlc::TileInfo* info) {
     ThreadCtx& ctx = t_ctx;
     jlc::SplitPlan p;
-    jlc::split_plan_sizes(m, n, k, slices ? slices : jlc::SPLIT_MAX_SLICES, &p);
+    const int max_slices = bf16 ? jlc::SPLIT_BF16_MAX_SLICES : jlc::SPLIT_MAX_SLICES;
+    auto sizes = [&](int s_) {
+        if (bf16) jlc::split_bf16_plan_sizes(m, n, k, s_, &p);
+        else jlc::split_plan_sizes(m, n, k, s_, &p);
+    };
+    sizes(slices ? slices : max_slices);
     const size_t stat_bytes = (size_t)(p.mpad + p.npad) * (sizeof(int) + sizeof(float));
     int rc = ensure_buf(3, stat_bytes);
     if (rc != JLC_OK) return rc;
@@ -254,7 +264,7 @@ int run_split_int8(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha
         JLC_HIP_CHECK(hipMemcpyAsync(host.data(), p.spread_a, host.size() * sizeof(float), hipMemcpyDeviceToHost,
                                      stream));
         JLC_HIP_CHECK(hipStreamSynchronize(stream));
-        slices = choose_slices(host.data(), m, host.data() + p.mpad, n);
+        slices = choose_slices(host.data(), m, host.data() + p.mpad, n, bf16 ? 8 : 7, max_slices);
         p.slices = slices;
     }
     if ((rc = ensure_buf(4, (size_t)(p.slices * p.a_slice_bytes))) != JLC_OK) return rc;
@@ -262,7 +272,10 @@ int run_split_int8(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha
     p.a_pack = ctx.buf[4];
     p.b_pack = ctx.buf[5];
     info->slices = p.slices;
-    JLC_HIP_CHECK(jlc::launch_split_pack_and_gemm(la, lb, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info));
+    if (bf16)
+        JLC_HIP_CHECK(jlc::launch_split_bf16(la, lb, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info));
+    else
+        JLC_HIP_CHECK(jlc::launch_split_pack_and_gemm(la, lb, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info));
     return JLC_OK;
 }
 
@@ -276,12 +289,12 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
         return JLC_OK;
     }
     int form = mode & 0xff;
-    if (form == JLC_MODE_SPLIT_INT8) {
-        *kernel = JLC_KERNEL_SPLIT_INT8;
-        return run_split_int8(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, (mode >> 8) & 0xff, stream,
-                              info);
+    if (form == JLC_MODE_SPLIT_INT8 || form == JLC_MODE_SPLIT_BF16) {
+        const bool bf16 = form == JLC_MODE_SPLIT_BF16;
+        *kernel = bf16 ? JLC_KERNEL_SPLIT_BF16 : JLC_KERNEL_SPLIT_INT8;
+        return run_split(bf16, la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, (mode >> 8) & 0xff, stream,
+                         info);
     }
-    if (form != JLC_MODE_FP64) return fail(JLC_ERR_UNSUPPORTED, "the BF16 split form is not built in this library");
     if (n == 1 && la == JLC_ROW) {
         // The n == 1 regime is the matvec (GemmDispatch.selectKernel, GemmDispatch.java:80-82):
         // x is the B column, y the C column.

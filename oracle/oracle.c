@@ -482,6 +482,20 @@ int oracle_split_exponent(const double* x, int64_t stride, int64_t count) {
     return ex;
 }
 
+static void split_digits(const double* x, int64_t stride, int64_t count, int e, int slices, int bits, int16_t* out,
+                         int64_t out_stride) {
+    const double radix = ldexp(1.0, bits);
+    for (int64_t i = 0; i < count; i++) {
+        double v = (e == SPLIT_NONFINITE) ? 0.0 : ldexp(x[i * stride], -e);
+        for (int t = 0; t < slices; t++) {
+            v *= radix;
+            double d = trunc(v);
+            v -= d;
+            out[t * out_stride + i] = (int16_t)(int)d;
+        }
+    }
+}
+
 void oracle_split_slices(const double* x, int64_t stride, int64_t count, int e, int slices, int8_t* out,
                          int64_t out_stride) {
     for (int64_t i = 0; i < count; i++) {
@@ -506,22 +520,40 @@ int64_t oracle_split_chunk_k(int slices, int64_t k) {
 void oracle_split_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
                        int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int slices,
                        int64_t chunk_k, int threads) {
+    oracle_split_gemm_digits(a_layout, b_layout, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices, 7, chunk_k,
+                             threads);
+}
+
+/* BF16 variant: 8-bit digits, FP64 flush every chunk_k (oracle_split_bf16_chunk_k). */
+int64_t oracle_split_bf16_chunk_k(int slices, int64_t k) {
+    int64_t fsteps = (1LL << 24) / ((int64_t)slices * 255 * 255 * 16);
+    if (fsteps < 1) fsteps = 1;
+    int64_t chunk = 2147483647LL / ((int64_t)slices * 16 * 255 * 255);
+    chunk = chunk / fsteps * fsteps;
+    int64_t ksteps = (k + 15) / 16;
+    if (chunk > ksteps) chunk = ksteps;
+    return chunk * 16;
+}
+
+void oracle_split_gemm_digits(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha,
+                              const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
+                              int64_t ldc, int slices, int bits, int64_t chunk_k, int threads) {
     const int S = slices;
-    int8_t* as = (int8_t*)calloc((size_t)(S * m * k + 1), 1);
-    int8_t* bs = (int8_t*)calloc((size_t)(S * n * k + 1), 1);
+    int16_t* as = (int16_t*)calloc((size_t)(S * m * k + 1), sizeof(int16_t));
+    int16_t* bs = (int16_t*)calloc((size_t)(S * n * k + 1), sizeof(int16_t));
     int* ea = (int*)malloc(sizeof(int) * (size_t)(m + 1));
     int* eb = (int*)malloc(sizeof(int) * (size_t)(n + 1));
     for (int64_t i = 0; i < m; i++) {
         const double* row = a_layout == ORACLE_ROW ? a + i * lda : a + i;
         int64_t st = a_layout == ORACLE_ROW ? 1 : lda;
         ea[i] = oracle_split_exponent(row, st, k);
-        oracle_split_slices(row, st, k, ea[i], S, as + i * k, m * k);
+        split_digits(row, st, k, ea[i], S, bits, as + i * k, m * k);
     }
     for (int64_t j = 0; j < n; j++) {
         const double* col = b_layout == ORACLE_ROW ? b + j : b + j * ldb;
         int64_t st = b_layout == ORACLE_ROW ? ldb : 1;
         eb[j] = oracle_split_exponent(col, st, k);
-        oracle_split_slices(col, st, k, eb[j], S, bs + j * k, n * k);
+        split_digits(col, st, k, eb[j], S, bits, bs + j * k, n * k);
     }
     if (threads < 1) threads = 1;
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
@@ -533,14 +565,14 @@ void oracle_split_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t
                 int64_t acc[16] = {0};
                 for (int t = 0; t < S; t++)
                     for (int u = 0; u + t <= S - 1; u++) {
-                        const int8_t* ar = as + (int64_t)t * m * k + i * k;
-                        const int8_t* br = bs + (int64_t)u * n * k + j * k;
+                        const int16_t* ar = as + (int64_t)t * m * k + i * k;
+                        const int16_t* br = bs + (int64_t)u * n * k + j * k;
                         int64_t sdot = 0;
                         for (int64_t q = k0; q < k1; q++) sdot += (int64_t)ar[q] * br[q];
                         acc[t + u] += sdot;
                     }
                 double sum = 0.0;
-                for (int d = S - 1; d >= 0; d--) sum += (double)acc[d] * ldexp(1.0, -7 * (d + 2));
+                for (int d = S - 1; d >= 0; d--) sum += (double)acc[d] * ldexp(1.0, -bits * (d + 2));
                 double v = (ea[i] == SPLIT_NONFINITE || eb[j] == SPLIT_NONFINITE) ? NAN : ldexp(sum, ea[i] + eb[j]);
                 if (first) *p = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * *p);
                 else *p = fma(alpha, v, *p);

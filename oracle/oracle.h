@@ -94,6 +94,11 @@ void oracle_split_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t
                        int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int slices,
                        int64_t chunk_k, int threads);
 int64_t oracle_split_chunk_k(int slices, int64_t k);
+/* Digit width 7 (INT8 form) or 8 (BF16 form); oracle_split_gemm is the 7-bit case. */
+void oracle_split_gemm_digits(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha,
+                              const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
+                              int64_t ldc, int slices, int bits, int64_t chunk_k, int threads);
+int64_t oracle_split_bf16_chunk_k(int slices, int64_t k);
 
 /* Blocked Householder QR restatement (column-major in/out). */
 int oracle_qr_factor(int64_t m, int64_t n, double* a_colmajor, int64_t lda, double* tau, int64_t nb);

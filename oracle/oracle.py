@@ -64,6 +64,10 @@ def _load():
                                       C.c_int, _i64, C.c_int]
     lib.oracle_split_chunk_k.argtypes = [C.c_int, _i64]
     lib.oracle_split_chunk_k.restype = _i64
+    lib.oracle_split_gemm_digits.argtypes = [C.c_int, C.c_int, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p,
+                                             _i64, C.c_int, C.c_int, _i64, C.c_int]
+    lib.oracle_split_bf16_chunk_k.argtypes = [C.c_int, _i64]
+    lib.oracle_split_bf16_chunk_k.restype = _i64
     lib.oracle_qr_factor.argtypes = [_i64, _i64, _p, _i64, _p, _i64]
     lib.oracle_qr_form_q.argtypes = [_i64, _i64, _p, _i64, _p, _p, _i64]
     _lib = lib
@@ -280,3 +284,16 @@ def split_gemm(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: floa
     ck = chunk_k if chunk_k is not None else split_chunk_k(slices, k)
     _load().oracle_split_gemm(layout_a, layout_b, m, n, k, alpha, _ptr(a, a_off), lda, _ptr(b, b_off), ldb, beta,
                               _ptr(c, c_off), ldc, slices, ck, nthreads or threads())
+
+
+def split_bf16_chunk_k(slices: int, k: int) -> int:
+    return _load().oracle_split_bf16_chunk_k(slices, k)
+
+
+def split_gemm_bf16(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, a, lda: int, b, ldb: int,
+                    beta: float, c: np.ndarray, ldc: int, slices: int, chunk_k: int | None = None, a_off: int = 0,
+                    b_off: int = 0, c_off: int = 0, nthreads: int | None = None) -> None:
+    """In-place restatement of the BF16-slice form (8-bit digits)."""
+    ck = chunk_k if chunk_k is not None else split_bf16_chunk_k(slices, k)
+    _load().oracle_split_gemm_digits(layout_a, layout_b, m, n, k, alpha, _ptr(a, a_off), lda, _ptr(b, b_off), ldb,
+                                     beta, _ptr(c, c_off), ldc, slices, 8, ck, nthreads or threads())

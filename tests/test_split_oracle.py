@@ -176,3 +176,39 @@ def test_nonfinite_rows_give_nan():
     orc.split_gemm(0, 0, 3, 2, 4, 1.0, A.ravel().copy(), 4, B.ravel().copy(), 2, 0.0, c, 2, 5)
     c = c.reshape(3, 2)
     assert np.all(np.isnan(c[1])) and np.all(c[[0, 2]] == 4.0)
+
+
+def test_bf16_restatement_matches_numpy_emulation():
+    """8-bit digits: the C restatement equals an independent numpy emulation."""
+    rng = np.random.default_rng(31)
+    m, n, k, S = 8, 6, 120, 5
+    A = _operand(rng, m, k, 1.0)
+    B = _operand(rng, k, n, 1.0)
+    ck = orc.split_bf16_chunk_k(S, k)
+
+    def slices(X):
+        mx = np.max(np.abs(X), axis=1)
+        e = np.array([np.frexp(v)[1] if v > 0 else 0 for v in mx])
+        v = X * np.ldexp(1.0, -e)[:, None]
+        out = []
+        for _ in range(S):
+            v = v * 256.0
+            q = np.trunc(v)
+            v = v - q
+            out.append(q)
+        return out, e
+    As, ea = slices(A)
+    Bs, eb = slices(B.T)
+    Bs = [x.T for x in Bs]
+    acc = [np.zeros((m, n)) for _ in range(S)]
+    for t in range(S):
+        for u in range(S - t):
+            acc[t + u] = acc[t + u] + As[t] @ Bs[u]
+    s = np.zeros((m, n))
+    for d in range(S - 1, -1, -1):
+        s = s + acc[d] * 2.0 ** (-8 * (d + 2))
+    ref = s * np.ldexp(1.0, ea)[:, None] * np.ldexp(1.0, eb)[None, :]
+    c = np.zeros(m * n)
+    orc.split_gemm_bf16(0, 0, m, n, k, 1.0, A.ravel().copy(), k, B.ravel().copy(), n, 0.0, c, n, S, ck)
+    assert ck >= k
+    assert np.array_equal(c.reshape(m, n), ref)
