@@ -1,8 +1,10 @@
-"""DGEMM 16384^3 benchmark on one MI355X: FP64 MFMA and the split-precision
-INT8 form (slice count chosen from the inputs' exponent spread), with the error of every timed form stated against a
-double-double oracle on sampled entries, the kernel's roofline fraction, and
-the CPU baseline (a restatement of the reference's PARALLEL_MICRO path) timed
-on this machine's host cores.
+"""DGEMM 16384^3 benchmark on one MI355X: the FP64 MFMA kernel and the
+split-precision forms (CRT form: modular INT8 products; INT8 and BF16 slice
+forms), each with its moduli / slice count chosen from the inputs' exponent
+spread and its error stated against a double-double oracle on sampled
+entries; the roofline fraction of the value form's dominant kernel; and the
+CPU baseline (a restatement of the reference's PARALLEL_MICRO path) timed on
+this machine's host cores.
 
 Workload: C = A*B, M = N = K = 16384, FP64, row-major, with
 a_ij = (u - 0.5) * exp(phi * g), u ~ U[0,1), g ~ N(0,1) (same for B), phi = 0.5
@@ -105,8 +107,10 @@ def cpu_baseline(max_seconds: float = 12.0) -> dict:
                       f"median of {len(times)} runs, {used} threads of {os.cpu_count()} host cores"}
 
 
-def time_form(step, warmup: int, steps: int, stream: int) -> tuple[float, float]:
-    """(wall ms per step, device ms per step) for steps timed launches after warmup."""
+def time_form(step, warmup: int, steps: int, stream: int, main: list | None = None) -> tuple[float, float]:
+    """(wall ms per step, device ms per step) for steps timed launches after warmup.
+    If main is a list, each step returns the call's Stats and main collects the
+    dominant kernel's own duration (HIP events on the launch stream)."""
     for _ in range(warmup):
         step()
     native.synchronize()
@@ -114,7 +118,9 @@ def time_form(step, warmup: int, steps: int, stream: int) -> tuple[float, float]
     t0 = time.perf_counter()
     timer.start()
     for _ in range(steps):
-        step()
+        st = step()
+        if main is not None:
+            main.append(st.main_ms)
     dev_ms = timer.stop()
     native.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3
@@ -164,6 +170,83 @@ def bench_batched(args) -> int:
         "per_call_ms": round(per_ms, 2), "speedup_vs_per_call": round(per_ms / wall_ms, 1),
         "plan_build_ms": round(plan_ms, 1),
     }), flush=True)
+    return 0
+
+
+def bench_layouts(args) -> int:
+    """FP64 MFMA across the reference's layouts and the bandwidth-bound matvec:
+    (a) 8192^3 row-major; (b) gemmStridedTransA shape M=4096, N=8192, K=16384 with A
+    stored K x M; (c) gemmStridedColMajorA / ColMajorB at 6000^3; matvec M=K=32768.
+    Entries uniform[-1,1], alpha=1, beta=0. Each case is checked on sampled entries
+    against double-double sums: componentwise error in units of u must stay <= k."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(args.seed)
+    stream = native.stream()
+    R, Cl = native.ROW, native.COL
+    # (name, layout_a, layout_b, m, n, k, lda, ldb, a rows x cols as stored, b rows x cols as stored)
+    cases = [("rowmajor_8192", R, R, 8192, 8192, 8192, 8192, 8192, (8192, 8192), (8192, 8192)),
+             ("transA_4096x8192x16384", Cl, R, 4096, 8192, 16384, 4096, 8192, (16384, 4096), (16384, 8192)),
+             ("colmajorA_6000", Cl, R, 6000, 6000, 6000, 6000, 6000, (6000, 6000), (6000, 6000)),
+             ("colmajorB_6000", R, Cl, 6000, 6000, 6000, 6000, 6000, (6000, 6000), (6000, 6000))]
+    results = {}
+    for name, la, lb, m, n, k, lda, ldb, ash, bsh in cases:
+        a = rng.uniform(-1.0, 1.0, ash[0] * ash[1])
+        b = rng.uniform(-1.0, 1.0, bsh[0] * bsh[1])
+        da, db, dc = native.DeviceArray.from_numpy(a), native.DeviceArray.from_numpy(b), native.DeviceArray(m * n)
+        step = lambda: native.dgemm_dev(la, lb, m, n, k, 1.0, da.ptr, lda, db.ptr, ldb, 0.0, dc.ptr, n,
+                                        native.MODE_FP64, stream, stats=False)
+        st = native.dgemm_dev(la, lb, m, n, k, 1.0, da.ptr, lda, db.ptr, ldb, 0.0, dc.ptr, n, native.MODE_FP64,
+                              stream, stats=True)
+        wall_ms, dev_ms = time_form(step, args.warmup, args.steps, stream)
+        tf = 2.0 * m * n * k / (dev_ms * 1e-3) / 1e12
+        srng = np.random.default_rng(args.seed + len(results))
+        ii = srng.integers(0, m, 256)
+        jj = srng.integers(0, n, 256)
+        hi, lo, absum = orc.dd_entries(la, lb, k, a, lda, b, ldb, ii, jj)
+        got = dc.to_numpy().reshape(m, n)[ii, jj]
+        comp = float(np.max(np.abs(got - hi - lo) / np.maximum(absum, 1e-300)) / 2.0 ** -53)
+        results[name] = {"kernel": st.kernel_name, "ms_per_step": round(wall_ms, 3), "device_ms": round(dev_ms, 3),
+                         "tflops": round(tf, 3), "frac_of_fp64_peak": round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
+                         "max_componentwise_err_units_u": round(comp, 3), "bound_units_u": k,
+                         "within_bound": comp <= k}
+        log(f"{name}: {dev_ms:.2f} ms device, {tf:.2f} TFLOP/s, componentwise err {comp:.2f} u (bound {k} u)")
+        del da, db, dc
+    # matvec y = A x, A 32768 x 32768 row-major (8 GiB), beta = 0: streams A once.
+    mv = 32768
+    t0 = time.perf_counter()
+    A = np.empty(mv * mv)
+    for r0 in range(0, mv, 4096):
+        A[r0 * mv:(r0 + 4096) * mv] = rng.uniform(-1.0, 1.0, 4096 * mv)
+    x = rng.uniform(-1.0, 1.0, mv)
+    dA, dx, dy = native.DeviceArray.from_numpy(A), native.DeviceArray.from_numpy(x), native.DeviceArray(mv)
+    log(f"matvec operands ready in {time.perf_counter() - t0:.1f}s")
+    step = lambda: native.dgemv_dev(mv, mv, 1.0, dA.ptr, mv, dx.ptr, 1, 0.0, dy.ptr, 1, stream, stats=False)
+    wall_ms, dev_ms = time_form(step, args.warmup, max(args.steps, 10), stream)
+    algo_bytes = 8.0 * (mv * mv + mv + mv)
+    gbs = algo_bytes / (dev_ms * 1e-3) / 1e9
+    ii = np.random.default_rng(args.seed + 99).integers(0, mv, 256)
+    hi, lo, absum = orc.dd_entries(native.ROW, native.ROW, mv, A, mv, x, 1, ii, np.zeros(256, dtype=np.int64))
+    got = dy.to_numpy()[ii]
+    comp = float(np.max(np.abs(got - hi - lo) / np.maximum(absum, 1e-300)) / 2.0 ** -53)
+    results["matvec_32768"] = {"ms_per_step": round(wall_ms, 4), "device_ms": round(dev_ms, 4),
+                               "algorithmic_bytes": algo_bytes, "gbs": round(gbs, 1),
+                               "max_componentwise_err_units_u": round(comp, 3), "bound_units_u": mv,
+                               "within_bound": comp <= mv,
+                               "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                            "traffic": read_traffic("dgemv_32768")}}
+    log(f"matvec_32768: {dev_ms:.3f} ms device, {gbs:.0f} GB/s, componentwise err {comp:.2f} u")
+    head = results["rowmajor_8192"]
+    print(json.dumps({
+        "metric": "FP64 MFMA DGEMM TFLOP/s across layouts (8192^3 row-major headline) + matvec GB/s",
+        "value": head["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "layouts: 8192^3 row-major, transA 4096x8192x16384, colmajorA/B 6000^3, "
+                               "matvec 32768^2; uniform[-1,1], alpha=1, beta=0"},
+        "roofline": {"bound": "mfma", "achieved": head["tflops"], "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": head["frac_of_fp64_peak"], "traffic": None},
+        "cases": results}), flush=True)
     return 0
 
 
@@ -314,7 +397,8 @@ def main() -> int:
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="time only the FP64 MFMA form")
     ap.add_argument("--slices", type=int, default=0, help="INT8 slice count for the split form (0 = auto)")
-    ap.add_argument("--workload", choices=["dgemm", "batched", "qr", "qr_batched"], default="dgemm",
+    ap.add_argument("--moduli", type=int, default=0, help="moduli count for the CRT form (0 = auto)")
+    ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched"], default="dgemm",
                     help="dgemm: the 16384^3 metric workload; batched: the grouped-GEMM configuration")
     ap.add_argument("--count", type=int, default=100000, help="problems in the batched workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -341,6 +425,8 @@ def main() -> int:
 
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "layouts":
+        return bench_layouts(args)
     if args.workload == "qr":
         return bench_qr(args)
     if args.workload == "qr_batched":
@@ -363,22 +449,27 @@ def main() -> int:
 
     def run_form(name: str, mode: int) -> dict:
         def step():
-            native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, dc.ptr, n, mode,
-                             stream, stats=False)
-        st = native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, dc.ptr, n, mode,
-                              stream, stats=True)
-        wall_ms, dev_ms = time_form(step, args.warmup, args.steps, stream)
+            return native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, dc.ptr, n, mode,
+                                    stream, stats=True)
+        st = step()
+        main = []
+        wall_ms, dev_ms = time_form(step, args.warmup, args.steps, stream, main)
         form = {"form": name, "kernel": st.kernel_name, "slices": st.slices, "ms_per_step": round(wall_ms, 3),
-                "device_ms": round(dev_ms, 3), "tflops": round(flops / (wall_ms * 1e-3) / 1e12, 3)}
+                "device_ms": round(dev_ms, 3), "main_kernel_ms": round(float(np.mean(main)), 3),
+                "tflops": round(flops / (wall_ms * 1e-3) / 1e12, 3)}
         if st.slices:
-            pairs = st.slices * (st.slices + 1) // 2
+            crt = name == "split_crt"
+            pairs = st.slices if crt else st.slices * (st.slices + 1) // 2
             ops = pairs * flops
             peak = BF16_MFMA_PEAK_TFLOPS if name == "split_bf16" else INT8_MFMA_PEAK_TOPS
             unit = "TFLOP/s" if name == "split_bf16" else "TOP/s"
-            form["roofline"] = {"bound": "mfma", "achieved": round(ops / (dev_ms * 1e-3) / 1e12, 1),
-                                "peak": peak, "unit": unit,
-                                "frac": round(ops / (dev_ms * 1e-3) / 1e12 / peak, 4),
-                                "note": f"{pairs} slice products of 2MNK ops; time includes slicing"}
+            what = (f"{pairs} modular INT8 products of 2MNK ops (bits {st.bits_a}/{st.bits_b}); time includes "
+                    f"residues and CRT reconstruction") if crt else \
+                f"{pairs} slice products of 2MNK ops; time includes slicing"
+            kms = form["main_kernel_ms"]
+            form["roofline"] = {"bound": "mfma", "achieved": round(ops / (kms * 1e-3) / 1e12, 1),
+                                "peak": peak, "unit": unit, "frac": round(ops / (kms * 1e-3) / 1e12 / peak, 4),
+                                "kernel_ms": kms, "note": what + "; achieved over the product kernel alone"}
         log(f"{name}: {wall_ms:.2f} ms/step wall, {dev_ms:.2f} ms device, {form['tflops']:.2f} TFLOP/s"
             f"{' slices=%d' % st.slices if st.slices else ''}")
         if not args.no_error:
@@ -387,16 +478,29 @@ def main() -> int:
         return form
 
     fp64 = run_form("fp64_mfma", native.MODE_FP64)
-    split = None if args.no_split else run_form("split_int8", native.mode_slices(native.MODE_SPLIT_INT8,
-                                                                                 args.slices))
-    split_bf16 = None if args.no_split else run_form("split_bf16", native.mode_slices(native.MODE_SPLIT_BF16, 0))
+    splits = []
+    if not args.no_split:
+        splits.append(run_form("split_crt", native.mode_slices(native.MODE_SPLIT_CRT, args.moduli)))
+        splits.append(run_form("split_int8", native.mode_slices(native.MODE_SPLIT_INT8, args.slices)))
+        splits.append(run_form("split_bf16", native.mode_slices(native.MODE_SPLIT_BF16, 0)))
     # value: best throughput among forms whose max relative error is at most the FP64 MFMA form's.
     best = fp64
-    if split is not None and split["tflops"] > fp64["tflops"]:
-        if args.no_error or split["error"]["max_rel_err"] <= fp64["error"]["max_rel_err"]:
-            best = split
-    achieved = flops / (fp64["device_ms"] * 1e-3) / 1e12
-    traffic = read_traffic("dgemm_f64_mfma_16384")
+    for sf in splits:
+        if sf["tflops"] > best["tflops"] and (args.no_error or
+                                              sf["error"]["max_rel_err"] <= fp64["error"]["max_rel_err"]):
+            best = sf
+    # roofline of the value form's dominant kernel, timed alone with HIP events in the timed loop
+    if best is fp64:
+        achieved = flops / (fp64["main_kernel_ms"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
+                    "traffic": read_traffic("dgemm_f64_mfma_16384"), "kernel": "dgemm_f64_mfma",
+                    "kernel_ms": fp64["main_kernel_ms"]}
+    else:
+        kname = {"split_crt": "crt_gemm_i8", "split_int8": "split_gemm_i8", "split_bf16": "split_gemm_bf16"}
+        roofline = dict(best["roofline"])
+        roofline["kernel"] = kname[best["form"]]
+        roofline["traffic"] = read_traffic(f"{kname[best['form']]}_16384")
     out = {
         "metric": METRIC,
         "value": best["tflops"],
@@ -415,13 +519,10 @@ def main() -> int:
                    "value_rule": "best TFLOP/s among forms with max rel err <= the FP64 MFMA form's",
                    "parallelism": "single_gpu"},
         "error": best.get("error"),
-        "forms": {f["form"]: f for f in (fp64, split, split_bf16) if f is not None},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
-                     "traffic": traffic, "kernel": "dgemm_f64_mfma", "kernel_ms": fp64["device_ms"]},
+        "forms": {f["form"]: f for f in [fp64] + splits},
+        "roofline": roofline,
+        "speedup_vs_fp64": {f["form"]: round(f["tflops"] / fp64["tflops"], 3) for f in splits},
     }
-    if split is not None:
-        out["split_vs_fp64_speedup"] = round(split["tflops"] / fp64["tflops"], 3)
     if not args.no_cpu_baseline:
         log("timing CPU baseline")
         out["cpu_baseline"] = cpu_baseline()

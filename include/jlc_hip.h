@@ -47,10 +47,15 @@ extern "C" {
 #define JLC_COL 1
 
 /* Arithmetic modes. The low byte selects the form; for the split forms bits
- * 8..15 carry the slice count (0 = choose from the inputs' exponent spread). */
+ * 8..15 carry the slice count, or for the CRT form the moduli count (0 = choose
+ * from the inputs' exponent spread). JLC_MODE_SPLIT_CRT: modular INT8 products
+ * recombined by the Chinese remainder theorem (2..20 moduli); in automatic
+ * mode a spread too wide for 20 moduli runs the FP64 MFMA kernel instead
+ * (jlc_stats.kernel says which ran). */
 #define JLC_MODE_FP64 0
 #define JLC_MODE_SPLIT_INT8 1
 #define JLC_MODE_SPLIT_BF16 2
+#define JLC_MODE_SPLIT_CRT 3
 #define JLC_MODE_SLICES(mode_, s_) ((mode_) | ((s_) << 8))
 
 /* Kernel identifiers reported in jlc_stats.kernel. */
@@ -62,6 +67,7 @@ extern "C" {
 #define JLC_KERNEL_SPLIT_BF16 5 /* BF16 slice products, FP64 recombination */
 #define JLC_KERNEL_GROUPED 6    /* persistent grouped kernel */
 #define JLC_KERNEL_QR 7         /* Householder QR (panel kernels + compact-WY GEMMs) */
+#define JLC_KERNEL_SPLIT_CRT 8  /* modular INT8 products, CRT reconstruction in FP64 */
 
 typedef struct jlc_stats {
     double kernel_ms;   /* device time of the call's kernels (HIP events) */
@@ -70,8 +76,11 @@ typedef struct jlc_stats {
     int32_t tile_m;     /* macro tile chosen */
     int32_t tile_n;
     int32_t tile_k;
-    int32_t slices;     /* split-precision slice count (0 for FP64) */
+    int32_t slices;     /* split-precision slice count / CRT moduli count (0 for FP64) */
     int32_t launches;   /* kernel launches issued */
+    int32_t bits_a;     /* CRT form: integer bits of op(A) rows (0 otherwise) */
+    int32_t bits_b;     /* CRT form: integer bits of op(B) columns */
+    double main_ms;     /* device time of the dominant kernel alone (the GEMM / GEMV launch) */
 } jlc_stats;
 
 /* ---- device detection and lifetime (analogue of CudaSupport / CudaContext,
@@ -177,7 +186,9 @@ int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream
 /* ---- tuning table: the autotuner's choices, read by dispatch. Keys:
  * hip_min_flops (offload gate, the HIP analogue of GemmDispatch.CUDA_THRESHOLD,
  * src/main/java/net/faulj/compute/GemmDispatch.java:32), splitk_tiles_per_cu,
- * splitk_min_kslice, grouped_waves_per_simd. Text file, one "key value" per line. */
+ * splitk_min_kslice, grouped_waves_per_simd. Text file, one "key value" per line.
+ * On first use the library loads $JLC_TUNE_TABLE, or else the table shipped
+ * beside it (<libdir>/../tuning/mi355x.tune); jlc_tune_load replaces it later. */
 int jlc_tune_load(const char* path);
 int jlc_tune_write(const char* path);
 int jlc_tune_get(const char* key, double* value);

@@ -78,15 +78,55 @@ def build_oracle(verbose: bool = True) -> Path:
     return out
 
 
+HOST_LIB = "libjlc_compute.so"
+CXX = os.environ.get("CXX", "g++")
+
+
+def build_host(verbose: bool = True) -> Path:
+    """C++ host mirror of the reference's GEMM classes, linked against libjlc_hip."""
+    LIBDIR.mkdir(exist_ok=True)
+    out = LIBDIR / HOST_LIB
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-shared", "-o", str(out),
+           str(PKG / "host" / "jlc_compute.cpp"), f"-L{LIBDIR}", "-ljlc_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host mirror build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"[build] {out}", flush=True)
+    return out
+
+
+def build_cpp_tests(verbose: bool = True) -> Path:
+    """The C++ parity test program (test infrastructure: links the oracle)."""
+    outdir = ROOT / "tests" / "cpp" / "build"
+    outdir.mkdir(parents=True, exist_ok=True)
+    out = outdir / "gemm_parity_test"
+    oracle_dir = ROOT / "oracle" / "build"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-o", str(out), str(ROOT / "tests" / "cpp" / "gemm_parity_test.cpp"),
+           f"-L{LIBDIR}", "-ljlc_compute", "-ljlc_hip", f"-L{oracle_dir}", "-loracle",
+           f"-Wl,-rpath,{LIBDIR}:{oracle_dir}:$ORIGIN/../../../jlc_amd/lib:$ORIGIN/../../../oracle/build"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"[build] {out}", flush=True)
+    return out
+
+
 def build_all(verbose: bool = True) -> None:
     build_hip(verbose)
+    build_host(verbose)
     build_oracle(verbose)
+    build_cpp_tests(verbose)
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     if "hip" in what:
         build_hip()
+    elif "host" in what:
+        build_host()
+        build_cpp_tests()
     elif "oracle" in what:
         build_oracle()
     else:

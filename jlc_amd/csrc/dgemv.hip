@@ -123,12 +123,14 @@ hipError_t launch_dgemv(int64_t m, int64_t k, double alpha, const double* a, int
         info->tile_k = 256;
         info->launches++;
     }
+    mark_main(info, true, stream);
     if (vec)
         hipLaunchKernelGGL(dgemv_rows<true>, dim3((unsigned)blocks), dim3(64 * WAVES), 0, stream, (int)m, (int)k,
                            alpha, a, lda, x, incx, beta, y, incy);
     else
         hipLaunchKernelGGL(dgemv_rows<false>, dim3((unsigned)blocks), dim3(64 * WAVES), 0, stream, (int)m, (int)k,
                            alpha, a, lda, x, incx, beta, y, incy);
+    mark_main(info, false, stream);
     return hipGetLastError();
 }
 

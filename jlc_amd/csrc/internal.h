@@ -11,7 +11,17 @@ struct TileInfo {
     int tile_m = 0, tile_n = 0, tile_k = 0;
     int launches = 0;
     int slices = 0;
+    int bits_a = 0, bits_b = 0;
+    // When set, recorded on the launch stream right before / after the dominant kernel.
+    hipEvent_t main_begin = nullptr, main_end = nullptr;
+    double main_ms = 0.0;
 };
+
+inline void mark_main(const TileInfo* info, bool begin, hipStream_t stream) {
+    if (!info) return;
+    hipEvent_t e = begin ? info->main_begin : info->main_end;
+    if (e) (void)hipEventRecord(e, stream);
+}
 
 // C = beta*C on an m x n row-major block (beta == 0 writes zeros, C unread).
 hipError_t launch_scale(int64_t m, int64_t n, double beta, double* c, int64_t ldc, hipStream_t stream,
@@ -23,6 +33,7 @@ struct TuneTable {
     int splitk_tiles_per_cu = 2;      // split K until slices * tiles >= this * CUs (0 = never split)
     int splitk_min_kslice = 1024;     // smallest K per slice
     int grouped_waves_per_simd = 0;   // persistent grouped grid (0 = occupancy limit)
+    double crt_bits_offset = 48.0;    // CRT form: bits of a row = ceil(offset + row spread)
 };
 TuneTable tune_snapshot();
 
@@ -59,7 +70,13 @@ struct SplitPlan {
     int* eb = nullptr;                     // npad column exponents of op(B)
     float* spread_a = nullptr;             // mpad
     float* spread_b = nullptr;             // npad
+    double* stats_part = nullptr;          // 4 * split_stats_part_elems doubles (column statistics partials)
 };
+
+// K ranges of the two-pass column statistics, and the partial-buffer size
+// (in 32-byte records) that covers both operands of an m x n x k product.
+int split_stats_kparts(int64_t rows, int64_t k);
+int64_t split_stats_part_elems(int64_t m, int64_t n, int64_t k);
 
 constexpr int SPLIT_MIN_SLICES = 3;
 constexpr int SPLIT_MAX_SLICES = 9;
@@ -112,6 +129,48 @@ hipError_t launch_sumsq_partial(const double* x, int64_t ld, int rows, int cols,
 hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_mn, hipStream_t stream);
 
 // BF16-slice variant (8-bit digits, slices 3..7).
+// CRT split form: N pairwise coprime moduli <= 256, residues in int8.
+constexpr int CRT_MAX_MODULI = 20;
+constexpr int CRT_MIN_MODULI = 2;
+constexpr int CRT_MAX_BITS = 66;  // integer bits per operand: three signed 22-bit chunks
+extern const int CRT_MODULI[CRT_MAX_MODULI];
+
+// Per-modulus constants and the reconstruction tables (passed by value to kernels).
+struct CrtTables {
+    int nmod = 0;
+    int pad_ = 0;
+    double mod[CRT_MAX_MODULI];
+    double inv[CRT_MAX_MODULI];     // 1/m rounded
+    int k22[CRT_MAX_MODULI];        // 2^22 mod m, symmetric
+    int k44[CRT_MAX_MODULI];        // 2^44 mod m, symmetric
+    float invf[CRT_MAX_MODULI];     // 1/m in FP32
+    int pad2_ = 0;
+    double hi_lim[CRT_MAX_MODULI];  // symmetric residue range [lo_lim, hi_lim]
+    double lo_lim[CRT_MAX_MODULI];
+    double w[CRT_MAX_MODULI][4];    // 40-bit limbs of W_t = (M/m_t) * ((M/m_t)^-1 mod m_t)
+    double mlimb[4];                // 40-bit limbs of M
+    double finv[4];                 // 2^(40 j) / M
+};
+
+struct CrtPlan {
+    int64_t m = 0, n = 0, k = 0;
+    int64_t mpad = 0, npad = 0, kpad = 0;  // 256 x 256 C tile, 64-deep K stage
+    int nmod = 0, pa = 0, pb = 0;          // moduli, bits of op(A) rows / op(B) columns
+    int64_t a_plane_bytes = 0, b_plane_bytes = 0, c_plane_bytes = 0;
+    void* a_pack = nullptr;   // nmod planes of op(A) residues (fragment order)
+    void* b_pack = nullptr;   // nmod planes of op(B) residues
+    void* c_res = nullptr;    // nmod planes of C' residues (accumulator order)
+    int* ea = nullptr;        // mpad row exponents of op(A)
+    int* eb = nullptr;        // npad column exponents of op(B)
+};
+
+double crt_log2_product(int nmod);
+void crt_tables(int nmod, CrtTables* T);
+void crt_plan_sizes(int64_t m, int64_t n, int64_t k, int nmod, CrtPlan* p);
+hipError_t launch_crt(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
+                      const CrtPlan& p, double alpha, double beta, double* c, int64_t ldc, hipStream_t stream,
+                      TileInfo* info);
+
 constexpr int SPLIT_BF16_MAX_SLICES = 7;
 void split_bf16_plan_sizes(int64_t m, int64_t n, int64_t k, int slices, SplitPlan* p);
 int64_t split_bf16_chunk_k(int slices, int64_t k);

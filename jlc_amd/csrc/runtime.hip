@@ -10,6 +10,8 @@
 // selection at initialisation.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <atomic>
 #include <chrono>
 #include <cstdarg>
@@ -36,6 +38,7 @@ thread_local std::string t_error;
 
 std::mutex g_tune_mutex;
 jlc::TuneTable g_tune;
+std::once_flag g_tune_autoload;
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -68,10 +71,10 @@ bool env_flag(const char* name, bool dflt) {
 struct ThreadCtx {
     int device = -1;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // Staging: 0 = A, 1 = B, 2 = C; split workspace: 3 = exponents/spreads, 4 = A slices, 5 = B slices;
-    // 6 = split-K partial products.
-    static constexpr int NBUF = 7;
+    // 6 = split-K partial products; 7 = residue planes of C (CRT form).
+    static constexpr int NBUF = 8;
     void* buf[NBUF] = {};
     size_t cap[NBUF] = {};
     // Pinned host staging for the host-pointer entry points.
@@ -85,6 +88,8 @@ struct ThreadCtx {
             if (buf[i]) (void)hipFree(buf[i]);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev2) (void)hipEventDestroy(ev2);
+        if (ev3) (void)hipEventDestroy(ev3);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -118,6 +123,8 @@ int ensure_init() {
         JLC_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         JLC_HIP_CHECK(hipEventCreate(&c.ev0));
         JLC_HIP_CHECK(hipEventCreate(&c.ev1));
+        JLC_HIP_CHECK(hipEventCreate(&c.ev2));
+        JLC_HIP_CHECK(hipEventCreate(&c.ev3));
         c.device = dev;
     } else {
         JLC_HIP_CHECK(hipSetDevice(dev));
@@ -195,7 +202,8 @@ int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha,
     if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX)
         return fail(JLC_ERR_INVALID_ARG, "dimension above 2^31-1");
     int form = mode & 0xff;
-    if (form != JLC_MODE_FP64 && form != JLC_MODE_SPLIT_INT8 && form != JLC_MODE_SPLIT_BF16)
+    if (form != JLC_MODE_FP64 && form != JLC_MODE_SPLIT_INT8 && form != JLC_MODE_SPLIT_BF16 &&
+        form != JLC_MODE_SPLIT_CRT)
         return fail(JLC_ERR_INVALID_ARG, "unknown mode %d", mode);
     int slices = (mode >> 8) & 0xff;
     if (form == JLC_MODE_FP64 && slices != 0) return fail(JLC_ERR_INVALID_ARG, "slice count given for FP64 mode");
@@ -207,6 +215,10 @@ int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha,
         (slices < jlc::SPLIT_MIN_SLICES || slices > jlc::SPLIT_BF16_MAX_SLICES))
         return fail(JLC_ERR_INVALID_ARG, "BF16 slice count %d outside [%d, %d]", slices, jlc::SPLIT_MIN_SLICES,
                     jlc::SPLIT_BF16_MAX_SLICES);
+    if (form == JLC_MODE_SPLIT_CRT && slices != 0 &&
+        (slices < jlc::CRT_MIN_MODULI || slices > jlc::CRT_MAX_MODULI))
+        return fail(JLC_ERR_INVALID_ARG, "CRT moduli count %d outside [%d, %d]", slices, jlc::CRT_MIN_MODULI,
+                    jlc::CRT_MAX_MODULI);
     if (m == 0 || n == 0) return JLC_OK;
     if (!c) return fail(JLC_ERR_INVALID_ARG, "C must not be null");
     if (ldc < n) return fail(JLC_ERR_INVALID_ARG, "ldc=%lld < n=%lld", (long long)ldc, (long long)n);
@@ -224,7 +236,8 @@ int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha,
 // truncation error of S 7-bit slices is about 2^(ea+eb-7S) per term while an
 // FP64 product-sum errs by about 2^-53 of the terms' typical size; spread_a /
 // spread_b are the largest row / column gaps (in bits) between the scale 2^e
-// and the mean |x|, so matching FP64 needs 7S >= 53 + spread_a + spread_b - 4.
+// and the typical |x| (geometric mean), so matching FP64 needs
+// 7S >= 53 + spread_a + spread_b - 4.
 int choose_slices(const float* spread_a, int64_t m, const float* spread_b, int64_t n, int digit_bits,
                   int max_slices) {
     float sa = 0.0f, sb = 0.0f;
@@ -250,14 +263,16 @@ int run_split(bool bf16, int la, int lb, int64_t m, int64_t n, int64_t k, double
         else jlc::split_plan_sizes(m, n, k, s_, &p);
     };
     sizes(slices ? slices : max_slices);
-    const size_t stat_bytes = (size_t)(p.mpad + p.npad) * (sizeof(int) + sizeof(float));
-    int rc = ensure_buf(3, stat_bytes);
+    const size_t stat_bytes = ((size_t)(p.mpad + p.npad) * (sizeof(int) + sizeof(float)) + 255) / 256 * 256;
+    const size_t part_bytes = (size_t)jlc::split_stats_part_elems(m, n, k) * 4 * sizeof(double);
+    int rc = ensure_buf(3, stat_bytes + part_bytes);
     if (rc != JLC_OK) return rc;
     char* sb = static_cast<char*>(ctx.buf[3]);
     p.ea = reinterpret_cast<int*>(sb);
     p.eb = p.ea + p.mpad;
     p.spread_a = reinterpret_cast<float*>(p.eb + p.npad);
     p.spread_b = p.spread_a + p.mpad;
+    p.stats_part = reinterpret_cast<double*>(sb + stat_bytes);
     JLC_HIP_CHECK(jlc::launch_split_stats(la, lb, a, lda, b, ldb, &p, stream));
     if (slices == 0) {
         std::vector<float> host((size_t)(p.mpad + p.npad));
@@ -279,6 +294,87 @@ int run_split(bool bf16, int la, int lb, int64_t m, int64_t n, int64_t k, double
     return JLC_OK;
 }
 
+// Integer bits per operand and moduli count of the CRT form. A row with
+// spread s (bits between its scale 2^e and its geometric-mean |x|) gets
+// ceil(offset + s) bits, so its entries near the mean keep about offset bits;
+// the moduli must cover pa + pb + ceil(log2 k) + 2 bits (|C'| < M/4).
+int crt_choose(double spread_a, double spread_b, int64_t k, int* pa, int* pb) {
+    const double off = jlc::tune_snapshot().crt_bits_offset;
+    *pa = std::max(2, (int)std::ceil(off + spread_a));
+    *pb = std::max(2, (int)std::ceil(off + spread_b));
+    const int lk = k > 1 ? (int)std::ceil(std::log2((double)k)) : 0;
+    if (*pa > jlc::CRT_MAX_BITS || *pb > jlc::CRT_MAX_BITS) return 0;
+    const double need = *pa + *pb + lk + 2;
+    for (int nm = jlc::CRT_MIN_MODULI; nm <= jlc::CRT_MAX_MODULI; nm++)
+        if (jlc::crt_log2_product(nm) >= need) return nm;
+    return 0;
+}
+
+// Runs the CRT form; *fallback is set (and nothing launched beyond the
+// statistics) when automatic mode finds the spread too wide for 20 moduli.
+int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+            const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int nmod, hipStream_t stream,
+            jlc::TileInfo* info, bool* fallback) {
+    ThreadCtx& ctx = t_ctx;
+    *fallback = false;
+    jlc::CrtPlan p;
+    jlc::crt_plan_sizes(m, n, k, jlc::CRT_MAX_MODULI, &p);
+    jlc::SplitPlan sp;
+    sp.m = m;
+    sp.n = n;
+    sp.k = k;
+    sp.mpad = p.mpad;
+    sp.npad = p.npad;
+    const size_t stat_bytes = ((size_t)(p.mpad + p.npad) * (sizeof(int) + sizeof(float)) + 255) / 256 * 256;
+    const size_t part_bytes = (size_t)jlc::split_stats_part_elems(m, n, k) * 4 * sizeof(double);
+    int rc = ensure_buf(3, stat_bytes + part_bytes);
+    if (rc != JLC_OK) return rc;
+    char* sb = static_cast<char*>(ctx.buf[3]);
+    sp.ea = reinterpret_cast<int*>(sb);
+    sp.eb = sp.ea + p.mpad;
+    sp.spread_a = reinterpret_cast<float*>(sp.eb + p.npad);
+    sp.spread_b = sp.spread_a + p.mpad;
+    sp.stats_part = reinterpret_cast<double*>(sb + stat_bytes);
+    JLC_HIP_CHECK(jlc::launch_split_stats(la, lb, a, lda, b, ldb, &sp, stream));
+    int pa, pb;
+    const int lk = k > 1 ? (int)std::ceil(std::log2((double)k)) : 0;
+    if (nmod == 0) {
+        std::vector<float> host((size_t)(p.mpad + p.npad));
+        JLC_HIP_CHECK(hipMemcpyAsync(host.data(), sp.spread_a, host.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                     stream));
+        JLC_HIP_CHECK(hipStreamSynchronize(stream));
+        float sa = 0.0f, sbm = 0.0f;
+        for (int64_t i = 0; i < m; i++) sa = std::max(sa, host[i]);
+        for (int64_t j = 0; j < n; j++) sbm = std::max(sbm, host[p.mpad + j]);
+        nmod = crt_choose(sa, sbm, k, &pa, &pb);
+        if (nmod == 0) {
+            *fallback = true;
+            return JLC_OK;
+        }
+    } else {
+        const int avail = (int)std::floor(jlc::crt_log2_product(nmod)) - lk - 2;
+        if (avail < 2) return fail(JLC_ERR_INVALID_ARG, "%d moduli leave no integer bits for k=%lld", nmod,
+                                   (long long)k);
+        pa = std::min(avail / 2, jlc::CRT_MAX_BITS);
+        pb = std::min(avail - pa, jlc::CRT_MAX_BITS);
+    }
+    jlc::crt_plan_sizes(m, n, k, nmod, &p);
+    p.pa = pa;
+    p.pb = pb;
+    p.ea = sp.ea;
+    p.eb = sp.eb;
+    if ((rc = ensure_buf(4, (size_t)(nmod * p.a_plane_bytes))) != JLC_OK) return rc;
+    if ((rc = ensure_buf(5, (size_t)(nmod * p.b_plane_bytes))) != JLC_OK) return rc;
+    if ((rc = ensure_buf(7, (size_t)(nmod * p.c_plane_bytes))) != JLC_OK) return rc;
+    p.a_pack = ctx.buf[4];
+    p.b_pack = ctx.buf[5];
+    p.c_res = ctx.buf[7];
+    info->bits_a = pa;
+    info->bits_b = pb;
+    JLC_HIP_CHECK(jlc::launch_crt(la, lb, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info));
+    return JLC_OK;
+}
+
 // Device-side dispatch shared by the host and device entry points.
 int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
                     const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
@@ -289,6 +385,14 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
         return JLC_OK;
     }
     int form = mode & 0xff;
+    if (form == JLC_MODE_SPLIT_CRT) {
+        bool fallback = false;
+        *kernel = JLC_KERNEL_SPLIT_CRT;
+        int rc = run_crt(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, (mode >> 8) & 0xff, stream, info,
+                         &fallback);
+        if (rc != JLC_OK || !fallback) return rc;
+        form = JLC_MODE_FP64;  // spread too wide for the moduli: the FP64 kernel serves the call
+    }
     if (form == JLC_MODE_SPLIT_INT8 || form == JLC_MODE_SPLIT_BF16) {
         const bool bf16 = form == JLC_MODE_SPLIT_BF16;
         *kernel = bf16 ? JLC_KERNEL_SPLIT_BF16 : JLC_KERNEL_SPLIT_INT8;
@@ -326,13 +430,72 @@ void fill_stats(jlc_stats* s, const jlc::TileInfo& info, int kernel, double kms,
     s->tile_n = info.tile_n;
     s->tile_k = info.tile_k;
     s->slices = info.slices;
+    s->bits_a = info.bits_a;
+    s->bits_b = info.bits_b;
+    s->main_ms = info.main_ms;
     s->launches = info.launches;
 }
 
 }  // namespace
 
+namespace {
+bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* set);
+
+// Parses "key value" lines into t; JLC_OK or an error naming the file and line.
+int parse_tune_file(const char* path, jlc::TuneTable& t) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(JLC_ERR_INVALID_ARG, "cannot open tuning table %s", path);
+    char line[512];
+    int lineno = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        lineno++;
+        char* p = line;
+        while (*p == ' ' || *p == '\t') p++;
+        if (*p == '#' || *p == '\n' || *p == '\0') continue;
+        char key[128];
+        double v;
+        if (std::sscanf(p, "%127s %lf", key, &v) != 2 || !tune_field(t, key, nullptr, &v)) {
+            std::fclose(f);
+            return fail(JLC_ERR_INVALID_ARG, "%s:%d: bad tuning entry", path, lineno);
+        }
+    }
+    std::fclose(f);
+    return JLC_OK;
+}
+
+// First use of the table: load $JLC_TUNE_TABLE if set, else the table shipped
+// beside the library (<libdir>/../tuning/mi355x.tune), so every host (FFM, C++,
+// Python) sees the same tuned values. A missing shipped table keeps the
+// built-in defaults; a malformed one is ignored the same way.
+void tune_autoload() {
+    std::call_once(g_tune_autoload, [] {
+        std::string path;
+        if (const char* env = std::getenv("JLC_TUNE_TABLE")) {
+            path = env;
+        } else {
+            Dl_info info{};
+            if (dladdr(reinterpret_cast<void*>(&tune_autoload), &info) && info.dli_fname) {
+                path = info.dli_fname;
+                size_t slash = path.rfind('/');
+                path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+                       "/../tuning/mi355x.tune";
+            }
+        }
+        if (path.empty()) return;
+        jlc::TuneTable t;
+        std::string saved = t_error;
+        if (parse_tune_file(path.c_str(), t) == JLC_OK) {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            g_tune = t;
+        }
+        t_error = saved;
+    });
+}
+}  // namespace
+
 namespace jlc {
 TuneTable tune_snapshot() {
+    tune_autoload();
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     return g_tune;
 }
@@ -351,6 +514,7 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"splitk_tiles_per_cu", nullptr, &jlc::TuneTable::splitk_tiles_per_cu},
         {"splitk_min_kslice", nullptr, &jlc::TuneTable::splitk_min_kslice},
         {"grouped_waves_per_simd", nullptr, &jlc::TuneTable::grouped_waves_per_simd},
+        {"crt_bits_offset", &jlc::TuneTable::crt_bits_offset, nullptr},
     };
     for (const F& f : fields) {
         if (std::strcmp(f.name, key) != 0) continue;
@@ -369,6 +533,7 @@ extern "C" {
 
 int jlc_tune_get(const char* key, double* value) {
     if (!key || !value) return fail(JLC_ERR_INVALID_ARG, "key and value must not be null");
+    tune_autoload();
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     if (!tune_field(g_tune, key, value, nullptr)) return fail(JLC_ERR_INVALID_ARG, "unknown tuning key '%s'", key);
     return JLC_OK;
@@ -376,6 +541,7 @@ int jlc_tune_get(const char* key, double* value) {
 
 int jlc_tune_set(const char* key, double value) {
     if (!key) return fail(JLC_ERR_INVALID_ARG, "key must not be null");
+    tune_autoload();
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     if (!tune_field(g_tune, key, nullptr, &value)) return fail(JLC_ERR_INVALID_ARG, "unknown tuning key '%s'", key);
     return JLC_OK;
@@ -383,24 +549,9 @@ int jlc_tune_set(const char* key, double value) {
 
 int jlc_tune_load(const char* path) {
     if (!path) return fail(JLC_ERR_INVALID_ARG, "path must not be null");
-    FILE* f = std::fopen(path, "r");
-    if (!f) return fail(JLC_ERR_INVALID_ARG, "cannot open tuning table %s", path);
     jlc::TuneTable t = jlc::tune_snapshot();
-    char line[512];
-    int lineno = 0;
-    while (std::fgets(line, sizeof line, f)) {
-        lineno++;
-        char* p = line;
-        while (*p == ' ' || *p == '\t') p++;
-        if (*p == '#' || *p == '\n' || *p == '\0') continue;
-        char key[128];
-        double v;
-        if (std::sscanf(p, "%127s %lf", key, &v) != 2 || !tune_field(t, key, nullptr, &v)) {
-            std::fclose(f);
-            return fail(JLC_ERR_INVALID_ARG, "%s:%d: bad tuning entry", path, lineno);
-        }
-    }
-    std::fclose(f);
+    int rc = parse_tune_file(path, t);
+    if (rc != JLC_OK) return rc;
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     g_tune = t;
     return JLC_OK;
@@ -416,6 +567,7 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "splitk_tiles_per_cu %d\n", t.splitk_tiles_per_cu);
     std::fprintf(f, "splitk_min_kslice %d\n", t.splitk_min_kslice);
     std::fprintf(f, "grouped_waves_per_simd %d\n", t.grouped_waves_per_simd);
+    std::fprintf(f, "crt_bits_offset %.17g\n", t.crt_bits_offset);
     std::fclose(f);
     return JLC_OK;
 }
@@ -487,7 +639,11 @@ int jlc_dgemm_dev(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, d
     if (rc != JLC_OK) return rc;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
     int kernel = JLC_KERNEL_NONE;
-    if (stats) JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    if (stats) {
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+        info.main_begin = t_ctx.ev2;
+        info.main_end = t_ctx.ev3;
+    }
     rc = run_gemm_device(layout_a, layout_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode, s, &info, &kernel);
     if (rc != JLC_OK) return rc;
     double kms = 0.0;
@@ -497,6 +653,10 @@ int jlc_dgemm_dev(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, d
         float f = 0.0f;
         JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
         kms = f;
+        if (kernel != JLC_KERNEL_SCALE) {
+            JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev2, t_ctx.ev3));
+            info.main_ms = f;
+        }
     }
     fill_stats(stats, info, kernel, kms, now_ms() - t0);
     return JLC_OK;

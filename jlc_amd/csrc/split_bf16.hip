@@ -193,7 +193,8 @@ __global__ __launch_bounds__(256) void split_gemm_bf16(int M, int N, int ksteps,
 
 template <int S>
 hipError_t run_bf16(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
-                    const SplitPlan& p, double alpha, double beta, double* c, int64_t ldc, hipStream_t stream) {
+                    const SplitPlan& p, double alpha, double beta, double* c, int64_t ldc, hipStream_t stream,
+                    const TileInfo* info) {
     const int M = (int)p.m, N = (int)p.n, K = (int)p.k;
     const int64_t asr = layout_a == JLC_LAYOUT_ROW ? lda : 1, ask = layout_a == JLC_LAYOUT_ROW ? 1 : lda;
     const int64_t bsr = layout_b == JLC_LAYOUT_COL ? ldb : 1, bsk = layout_b == JLC_LAYOUT_COL ? 1 : ldb;
@@ -215,10 +216,12 @@ hipError_t run_bf16(int layout_a, int layout_b, const double* a, int64_t lda, co
     const int tiles_m = (int)(p.mpad / TM), tiles_n = (int)(p.npad / TN);
     const size_t lds = (size_t)2 * 4 * S * 1024;
     (void)hipFuncSetAttribute((const void*)split_gemm_bf16<S>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    mark_main(info, true, stream);
     hipLaunchKernelGGL(split_gemm_bf16<S>, dim3(tiles_m * tiles_n), dim3(256), lds, stream, M, N, ksteps, fsteps,
                        chunk, static_cast<const v4i*>(p.a_pack), p.a_slice_bytes / 16,
                        static_cast<const v4i*>(p.b_pack), p.b_slice_bytes / 16, p.ea, p.eb, alpha, beta, c, ldc,
                        tiles_m, tiles_n);
+    mark_main(info, false, stream);
     return hipGetLastError();
 }
 
@@ -246,11 +249,11 @@ hipError_t launch_split_bf16(int layout_a, int layout_b, const double* a, int64_
                              hipStream_t stream, TileInfo* info) {
     hipError_t err;
     switch (p.slices) {
-        case 3: err = run_bf16<3>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream); break;
-        case 4: err = run_bf16<4>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream); break;
-        case 5: err = run_bf16<5>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream); break;
-        case 6: err = run_bf16<6>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream); break;
-        case 7: err = run_bf16<7>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream); break;
+        case 3: err = run_bf16<3>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
+        case 4: err = run_bf16<4>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
+        case 5: err = run_bf16<5>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
+        case 6: err = run_bf16<6>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
+        case 7: err = run_bf16<7>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
         default: return hipErrorInvalidValue;
     }
     if (info) {

@@ -1,0 +1,484 @@
+// CRT split form of DGEMM on INT8 MFMA (gfx950): modular integer products
+// recombined by the Chinese remainder theorem (Ozaki, Uchino, Imamura 2025,
+// "Ozaki Scheme II: a GEMM-oriented emulation of floating-point matrix
+// multiplication using an integer modular technique").
+//
+//  1. Scaling. Row i of op(A) is mapped to integers a' = rint(a * 2^(pa-ea[i]))
+//     with 2^ea[i] just above the row max |a| (frexp exponent), so |a'| <= 2^pa;
+//     column j of op(B) likewise with pb, eb[j]. Power-of-two scaling is exact,
+//     and the only rounding of the whole method is this one per input entry
+//     (plus the final conversion of the exact integer result to FP64).
+//  2. Residues. For N pairwise coprime moduli m_t <= 256 the symmetric
+//     residues a' mod m_t and b' mod m_t fit in int8. They are written in the
+//     MFMA fragment order of the slice form: one 1 KiB block per (modulus,
+//     32-row block, 32-deep k chunk), lane l holding row l%32, k in
+//     [16*(l/32), +16).
+//  3. Products. Each modulus is one INT8 GEMM with exact INT32 accumulation
+//     (|sum| <= k * 2^14, folded modulo m_t every 65536 k when k is larger);
+//     the epilogue reduces the sum modulo m_t and stores the int8 residue of
+//     C' = A'B' in the accumulator's own lane order (1 KiB per 32 x 32 block).
+//     Unlike the slice form there is no operand reuse between moduli, so the
+//     GEMM is a plain large-tile INT8 GEMM: 256 x 256 C tile, 8 wavefronts of
+//     128 x 64, K staged 64 deep through a three-buffer LDS ring (96 KiB).
+//  4. Reconstruction. With M = prod m_t > 4 k 2^(pa+pb), C' is the unique
+//     integer in (-M/2, M/2) with those residues. X = sum_t r_t W_t
+//     (W_t = (M/m_t) * ((M/m_t)^-1 mod m_t)) is formed exactly in four 40-bit
+//     limbs (every partial sum is an integer below 2^53), q = rint(X/M),
+//     C' = X - qM limb by limb (exact fma), the limbs are balanced and summed
+//     from the top in double-double and rounded once; then
+//     C = alpha * C' * 2^(ea+eb-pa-pb) (+ beta*C through fma).
+//
+// Rows / columns holding Inf or NaN give NaN outputs, as in the slice form.
+#include <math.h>
+
+#include "internal.h"
+
+// The reconstruction must round exactly as written: no contraction of a*b+c.
+#pragma clang fp contract(off)
+
+namespace jlc {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int NONFINITE = -100000;
+constexpr int CT = 256;        // C tile rows and columns
+constexpr int KSTAGE = 64;     // K per LDS stage (two 32-deep MFMA steps)
+constexpr int FOLD_STEPS = 2048;  // 32-deep steps between modular folds (65536 k)
+
+// Symmetric residue of an integer-valued double |y| < 2^53 modulo m.
+__device__ __forceinline__ double red(double y, double m, double inv_m, double hi_lim, double lo_lim) {
+    const double q = rint(y * inv_m);
+    double r = fma(-q, m, y);
+    r = r > hi_lim ? r - m : r;
+    r = r < lo_lim ? r + m : r;
+    return r;
+}
+
+// Residues of 16 integer images s = c2*2^44 + c1*2^22 + c0 (22-bit signed
+// chunks, |s| < 2^66) packed as bytes. t = c2*(2^44 mod m) + c1*(2^22 mod m) + c0
+// is exact in INT32 (|t| < 2^31) and congruent to s; it is reduced with an
+// FP32 quotient estimate (|t/m| < 2^23, three FP32 roundings: the estimate is
+// within 1.62 of t/m) and two correction rounds.
+__device__ __forceinline__ v4i residues16(const int (&c2)[16], const int (&c1)[16], const int (&c0)[16],
+                                          const CrtTables& T, int t) {
+    const int m = (int)T.mod[t], k44 = T.k44[t], k22 = T.k22[t];
+    const int hl = (int)T.hi_lim[t], ll = (int)T.lo_lim[t];
+    const float invf = T.invf[t];
+    unsigned char q[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int x = c2[j] * k44 + c1[j] * k22 + c0[j];
+        const int qq = (int)rintf((float)x * invf);
+        int r = x - qq * m;
+        r = r > hl ? r - m : r;
+        r = r < ll ? r + m : r;
+        r = r > hl ? r - m : r;
+        r = r < ll ? r + m : r;
+        q[j] = (unsigned char)(signed char)r;
+    }
+    v4i w;
+    w.x = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+    w.y = q[4] | (q[5] << 8) | (q[6] << 16) | (q[7] << 24);
+    w.z = q[8] | (q[9] << 8) | (q[10] << 16) | (q[11] << 24);
+    w.w = q[12] | (q[13] << 8) | (q[14] << 16) | (q[15] << 24);
+    return w;
+}
+
+// Residue planes of op(X) (element (r, k) at X[r*sr + k*sk], R x Kd, zero
+// padded): one wavefront per 1 KiB block; lane l handles row l%32,
+// k in [16*(l/32), +16) and writes 16 bytes per modulus.
+__global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, int64_t sr, int64_t sk, int R, int Kd,
+                                                int kchunks, int nblocks, const int* __restrict__ e, int p,
+                                                CrtTables T, v4i* __restrict__ out, int64_t plane_stride) {
+    const int lane = threadIdx.x & 63;
+    const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blk >= nblocks) return;
+    const int rb = blk / kchunks, kc = blk % kchunks;
+    const int r = rb * 32 + (lane & 31);
+    const int k0 = kc * 32 + 16 * (lane >> 5);
+    const int ex = r < R ? e[r] : 0;
+    const bool live = r < R && ex != NONFINITE;
+    int c2[16], c1[16], c0[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int k = k0 + j;
+        double s = 0.0;
+        if (live && k < Kd) {
+            s = ldexp(X[(int64_t)r * sr + (int64_t)k * sk], p - ex);
+            s = fabs(s) < 9007199254740992.0 ? rint(s) : s;
+        }
+        // s = c2 * 2^44 + c1 * 2^22 + c0 exactly; every chunk has the sign of s and |chunk| < 2^22.
+        const double h = trunc(s * 0x1p-44);
+        const double rest = fma(-h, 0x1p44, s);
+        const double mid = trunc(rest * 0x1p-22);
+        c2[j] = (int)h;
+        c1[j] = (int)mid;
+        c0[j] = (int)fma(-mid, 0x1p22, rest);
+    }
+    const int64_t off = (int64_t)blk * 64 + lane;
+    for (int t = 0; t < T.nmod; t++) out[t * plane_stride + off] = residues16(c2, c1, c0, T, t);
+}
+
+// Column strips of 16 tile rows (256 x 256 tiles), planes in blockIdx.y.
+__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
+    constexpr int GROUP = 16;
+    const int per_group = GROUP * tiles_n;
+    const int g = bid / per_group;
+    const int first_m = g * GROUP;
+    const int gsize = min(tiles_m - first_m, GROUP);
+    const int in = bid % per_group;
+    tm = first_m + in % gsize;
+    tn = in / gsize;
+}
+
+__device__ __forceinline__ v16i reduce_acc(v16i acc, double m, double inv_m, double hl, double ll) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = (int)red((double)acc[r], m, inv_m, hl, ll);
+    return acc;
+}
+
+// One INT8 GEMM per modulus (blockIdx.y); C residues out in accumulator order.
+// Wavefront w: rows (w>>2)*128 + [0,128) as 4 blocks, columns (w&3)*64 + [0,64) as 2 blocks.
+// K runs in 64-deep stages through a ring of three 32 KiB LDS buffers fed from
+// two register sets: stage s+2 is fetched from global memory while stage s is
+// multiplied and stage s+1 (fetched one stage earlier) is written to LDS, so a
+// global load has two stages of MFMA work to arrive. One barrier per stage.
+constexpr int NBUF_LDS = 3;
+constexpr int STAGE_V4 = 32 * 64;  // v4i per stage (32 blocks of 1 KiB)
+
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void crt_gemm_i8(
+    int kchunks, const v4i* __restrict__ Ap, int64_t a_plane, const v4i* __restrict__ Bp, int64_t b_plane,
+    v4i* __restrict__ Cr, int64_t c_plane, int cblocks_n, int tiles_m, int tiles_n, CrtTables T) {
+    extern __shared__ v4i lds[];  // [NBUF_LDS][32 blocks: A kc*8+rb (0..15), B 16 + kc*8+cb][64 lanes]
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w >> 2, wn = w & 3;
+    const int t = blockIdx.y;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const v4i* A = Ap + t * a_plane + (int64_t)tm * 8 * kchunks * 64;
+    const v4i* B = Bp + t * b_plane + (int64_t)tn * 8 * kchunks * 64;
+    // Loads of a stage: 32 blocks of 1 KiB; wavefront w moves blocks w, w+8, w+16, w+24.
+    const v4i* src[4];
+    int dst[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int b = w + 8 * i;  // 0..15 A, 16..31 B; block b: row/col block (b&15)>>1, kc b&1
+        const int rb = (b & 15) >> 1, kc = b & 1;
+        src[i] = (b < 16 ? A : B) + ((int64_t)rb * kchunks + kc) * 64 + lane;
+        dst[i] = ((b & 16) + kc * 8 + rb) * 64 + lane;
+    }
+    const int stages = kchunks / 2;
+    v4i r0[4], r1[4];
+    auto load = [&](int s, v4i (&r)[4]) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = src[i][(int64_t)s * 128];
+    };
+    auto store = [&](int buf, const v4i (&r)[4]) {
+        v4i* base = lds + buf * STAGE_V4;
+#pragma unroll
+        for (int i = 0; i < 4; i++) base[dst[i]] = r[i];
+    };
+    v16i acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double m = T.mod[t], inv_m = T.inv[t], hl = T.hi_lim[t], ll = T.lo_lim[t];
+
+    auto compute = [&](int buf) {
+        const v4i* base = lds + buf * STAGE_V4 + lane;
+#pragma unroll
+        for (int kc = 0; kc < 2; kc++) {
+            v4i af[4], bf[2];
+#pragma unroll
+            for (int i = 0; i < 4; i++) af[i] = base[(kc * 8 + wm * 4 + i) * 64];
+#pragma unroll
+            for (int j = 0; j < 2; j++) bf[j] = base[(16 + kc * 8 + wn * 2 + j) * 64];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    };
+    auto fold = [&](int s) {
+        if (((s + 1) * 2) % FOLD_STEPS == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = reduce_acc(acc[i][j], m, inv_m, hl, ll);
+        }
+    };
+
+    load(0, r0);
+    store(0, r0);
+    if (stages > 1) load(1, r1);
+    __syncthreads();
+    // Iteration s: fetch stage s+2, multiply stage s, write stage s+1 to LDS.
+    // Unrolled by two so the register sets are named statically: even s uses
+    // r0 for the fetch and r1 for the LDS write, odd s the reverse.
+    int buf = 0;
+    for (int s = 0; s < stages; s += 2) {
+        {
+            if (s + 2 < stages) load(s + 2, r0);
+            compute(buf);
+            const int nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
+            if (s + 1 < stages) store(nb, r1);
+            __syncthreads();
+            fold(s);
+            buf = nb;
+        }
+        if (s + 1 < stages) {
+            if (s + 3 < stages) load(s + 3, r1);
+            compute(buf);
+            const int nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
+            if (s + 2 < stages) store(nb, r0);
+            __syncthreads();
+            fold(s + 1);
+            buf = nb;
+        }
+    }
+    // Residues of C' in accumulator order: block (row block, col block) of the plane.
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const v16i r = reduce_acc(acc[i][j], m, inv_m, hl, ll);
+            v4i o;
+            o.x = (r[0] & 255) | ((r[1] & 255) << 8) | ((r[2] & 255) << 16) | ((r[3] & 255) << 24);
+            o.y = (r[4] & 255) | ((r[5] & 255) << 8) | ((r[6] & 255) << 16) | ((r[7] & 255) << 24);
+            o.z = (r[8] & 255) | ((r[9] & 255) << 8) | ((r[10] & 255) << 16) | ((r[11] & 255) << 24);
+            o.w = (r[12] & 255) | ((r[13] & 255) << 8) | ((r[14] & 255) << 16) | ((r[15] & 255) << 24);
+            const int64_t rbc = tm * 8 + wm * 4 + i, cbc = tn * 8 + wn * 2 + j;
+            Cr[t * c_plane + (rbc * cblocks_n + cbc) * 64 + lane] = o;
+        }
+}
+
+// CRT reconstruction of one residue vector (exact up to the final rounding).
+template <int NM>
+__device__ __forceinline__ double crt_value(const int (&r)[NM], const CrtTables& T) {
+    double X[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) X[j] = 0.0;
+#pragma unroll
+    for (int t = 0; t < NM; t++) {
+        const double rt = (double)r[t];
+#pragma unroll
+        for (int j = 0; j < 4; j++) X[j] = X[j] + rt * T.w[t][j];
+    }
+    double qf = 0.0;
+#pragma unroll
+    for (int j = 3; j >= 0; j--) qf = qf + X[j] * T.finv[j];
+    const double q = rint(qf);
+    double D[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) D[j] = fma(-q, T.mlimb[j], X[j]);
+    const double base = 0x1p40;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const double c = rint(D[j] * 0x1p-40);
+        D[j] = D[j] - c * base;
+        D[j + 1] = D[j + 1] + c;
+    }
+    double h = D[3], l = 0.0;
+#pragma unroll
+    for (int j = 2; j >= 0; j--) {
+        const double hb = h * base;
+        const double s = hb + D[j];
+        const double bb = s - hb;
+        const double e = (hb - (s - bb)) + (D[j] - bb);
+        l = l * base + e;
+        h = s;
+    }
+    return h + l;
+}
+
+// One thread per (32 x 32 block, lane): 16 outputs of column l%32.
+template <int NM>
+__global__ __launch_bounds__(256) void crt_combine(const v4i* __restrict__ Cr, int64_t c_plane, int cblocks_n,
+                                                   int nblocks, int M, int N, const int* __restrict__ ea,
+                                                   const int* __restrict__ eb, int pa, int pb, double alpha,
+                                                   double beta, double* __restrict__ C, int64_t ldc, CrtTables T) {
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int blk = (int)(gid >> 6), lane = (int)(gid & 63);
+    if (blk >= nblocks) return;
+    const int rbc = blk / cblocks_n, cbc = blk % cblocks_n;
+    const int col = cbc * 32 + (lane & 31);
+    const int rbase = rbc * 32 + 4 * (lane >> 5);
+    if (col >= N || rbase >= M) return;
+    v4i vv[NM];
+#pragma unroll
+    for (int t = 0; t < NM; t++) vv[t] = Cr[t * c_plane + (int64_t)blk * 64 + lane];
+    const int fj = eb[col];
+#pragma unroll
+    for (int b = 0; b < 16; b++) {
+        const int row = rbase + 8 * (b >> 2) + (b & 3);
+        if (row >= M) continue;
+        int r[NM];
+#pragma unroll
+        for (int t = 0; t < NM; t++) r[t] = (signed char)((vv[t][b >> 2] >> (8 * (b & 3))) & 255);
+        const int ei = ea[row];
+        const double v = (ei == NONFINITE || fj == NONFINITE) ? __builtin_nan("")
+                                                              : ldexp(crt_value<NM>(r, T), ei + fj - pa - pb);
+        double* p = C + (int64_t)row * ldc + col;
+        *p = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * *p);
+    }
+}
+
+// Little-endian base-2^32 integers of 8 words for the host tables.
+struct Big {
+    uint32_t w[8] = {};
+    void mul(uint32_t v) {
+        uint64_t carry = 0;
+        for (int i = 0; i < 8; i++) {
+            const uint64_t x = (uint64_t)w[i] * v + carry;
+            w[i] = (uint32_t)x;
+            carry = x >> 32;
+        }
+    }
+    uint32_t mod(uint32_t m) const {
+        uint64_t r = 0;
+        for (int i = 7; i >= 0; i--) r = ((r << 32) | w[i]) % m;
+        return (uint32_t)r;
+    }
+    double limb(int j) const {  // bits [40j, 40j+40)
+        double v = 0.0;
+        for (int b = 39; b >= 0; b--) {
+            const int bit = 40 * j + b;
+            v = v * 2.0 + (double)((w[bit >> 5] >> (bit & 31)) & 1u);
+        }
+        return v;
+    }
+    double value() const {
+        double v = 0.0;
+        for (int i = 7; i >= 0; i--) v = v * 4294967296.0 + (double)w[i];
+        return v;
+    }
+};
+
+}  // namespace
+
+const int CRT_MODULI[CRT_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                        223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+
+double crt_log2_product(int nmod) {
+    double s = 0.0;
+    for (int t = 0; t < nmod && t < CRT_MAX_MODULI; t++) s += std::log2((double)CRT_MODULI[t]);
+    return s;
+}
+
+void crt_tables(int nmod, CrtTables* T) {
+    *T = CrtTables{};
+    T->nmod = nmod;
+    Big M;
+    M.w[0] = 1;
+    for (int t = 0; t < nmod; t++) M.mul((uint32_t)CRT_MODULI[t]);
+    for (int t = 0; t < nmod; t++) {
+        const int m = CRT_MODULI[t];
+        T->mod[t] = m;
+        T->inv[t] = 1.0 / m;
+        T->hi_lim[t] = (double)((m - 1) / 2);
+        T->lo_lim[t] = -(double)(m / 2);
+        // 2^22 and 2^44 mod m in the symmetric range
+        auto sym = [m](long long v) {
+            int c = (int)(v % m);
+            return c > (m - 1) / 2 ? c - m : c;
+        };
+        T->k22[t] = sym(1LL << 22);
+        T->k44[t] = sym(1LL << 44);
+        T->invf[t] = 1.0f / (float)m;
+        Big Mi;
+        Mi.w[0] = 1;
+        for (int u = 0; u < nmod; u++)
+            if (u != t) Mi.mul((uint32_t)CRT_MODULI[u]);
+        const uint32_t r = Mi.mod((uint32_t)m);
+        uint32_t y = 0;
+        for (uint32_t cand = 1; cand < (uint32_t)m; cand++)
+            if ((r * cand) % (uint32_t)m == 1) {
+                y = cand;
+                break;
+            }
+        Mi.mul(y);
+        for (int j = 0; j < 4; j++) T->w[t][j] = Mi.limb(j);
+    }
+    const double md = M.value();
+    for (int j = 0; j < 4; j++) {
+        T->mlimb[j] = M.limb(j);
+        T->finv[j] = std::ldexp(1.0, 40 * j) / md;
+    }
+}
+
+void crt_plan_sizes(int64_t m, int64_t n, int64_t k, int nmod, CrtPlan* p) {
+    p->m = m;
+    p->n = n;
+    p->k = k;
+    p->nmod = nmod;
+    p->mpad = (m + CT - 1) / CT * CT;
+    p->npad = (n + CT - 1) / CT * CT;
+    p->kpad = (k + KSTAGE - 1) / KSTAGE * KSTAGE;
+    p->a_plane_bytes = p->mpad * p->kpad;
+    p->b_plane_bytes = p->npad * p->kpad;
+    p->c_plane_bytes = p->mpad * p->npad;
+}
+
+hipError_t launch_crt(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
+                      const CrtPlan& p, double alpha, double beta, double* c, int64_t ldc, hipStream_t stream,
+                      TileInfo* info) {
+    const int M = (int)p.m, N = (int)p.n, K = (int)p.k;
+    CrtTables T;
+    crt_tables(p.nmod, &T);
+    const int64_t asr = layout_a == JLC_LAYOUT_ROW ? lda : 1, ask = layout_a == JLC_LAYOUT_ROW ? 1 : lda;
+    const int64_t bsr = layout_b == JLC_LAYOUT_COL ? ldb : 1, bsk = layout_b == JLC_LAYOUT_COL ? 1 : ldb;
+    const int kchunks = (int)(p.kpad / 32);
+    const int ablocks = (int)(p.mpad / 32) * kchunks, bblocks = (int)(p.npad / 32) * kchunks;
+    v4i* ap = static_cast<v4i*>(p.a_pack);
+    v4i* bp = static_cast<v4i*>(p.b_pack);
+    v4i* cr = static_cast<v4i*>(p.c_res);
+    hipLaunchKernelGGL(crt_pack, dim3((ablocks + 3) / 4), dim3(256), 0, stream, a, asr, ask, M, K, kchunks, ablocks,
+                       p.ea, p.pa, T, ap, p.a_plane_bytes / 16);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(crt_pack, dim3((bblocks + 3) / 4), dim3(256), 0, stream, b, bsr, bsk, N, K, kchunks, bblocks,
+                       p.eb, p.pb, T, bp, p.b_plane_bytes / 16);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    const int tiles_m = (int)(p.mpad / CT), tiles_n = (int)(p.npad / CT);
+    const int cblocks_n = (int)(p.npad / 32);
+    const size_t lds = (size_t)NBUF_LDS * STAGE_V4 * sizeof(v4i);
+    (void)hipFuncSetAttribute((const void*)crt_gemm_i8, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    mark_main(info, true, stream);
+    hipLaunchKernelGGL(crt_gemm_i8, dim3(tiles_m * tiles_n, p.nmod), dim3(512), lds, stream, kchunks, ap,
+                       p.a_plane_bytes / 16, bp, p.b_plane_bytes / 16, cr, p.c_plane_bytes / 16, cblocks_n, tiles_m,
+                       tiles_n, T);
+    mark_main(info, false, stream);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    const int cblocks = (int)(p.mpad / 32) * cblocks_n;
+    const dim3 cgrid((int)(((int64_t)cblocks * 64 + 255) / 256));
+    switch (p.nmod) {
+#define JLC_CRT_CASE(NM_)                                                                                         \
+    case NM_:                                                                                                     \
+        hipLaunchKernelGGL(crt_combine<NM_>, cgrid, dim3(256), 0, stream, cr, p.c_plane_bytes / 16, cblocks_n,    \
+                           cblocks, M, N, p.ea, p.eb, p.pa, p.pb, alpha, beta, c, ldc, T);                        \
+        break;
+        JLC_CRT_CASE(2) JLC_CRT_CASE(3) JLC_CRT_CASE(4) JLC_CRT_CASE(5) JLC_CRT_CASE(6) JLC_CRT_CASE(7)
+        JLC_CRT_CASE(8) JLC_CRT_CASE(9) JLC_CRT_CASE(10) JLC_CRT_CASE(11) JLC_CRT_CASE(12) JLC_CRT_CASE(13)
+        JLC_CRT_CASE(14) JLC_CRT_CASE(15) JLC_CRT_CASE(16) JLC_CRT_CASE(17) JLC_CRT_CASE(18) JLC_CRT_CASE(19)
+        JLC_CRT_CASE(20)
+#undef JLC_CRT_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    if (info) {
+        info->tile_m = CT;
+        info->tile_n = CT;
+        info->tile_k = KSTAGE;
+        info->slices = p.nmod;
+        info->launches += 4;
+    }
+    return hipSuccess;
+}
+
+}  // namespace jlc

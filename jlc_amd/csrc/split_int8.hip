@@ -53,52 +53,121 @@ constexpr int KS = 32;              // K per MFMA step
 
 // Row statistics of op(X) where element (r, k) is X[r*sr + k*sk]:
 // e[r] = frexp exponent of max_k |x| (0 for an all-zero row, NONFINITE if any
-// entry is Inf/NaN) and spread[r] = e[r] - log2(mean_k |x|) (bits between the
-// scale and the typical magnitude, used by the slice-count chooser).
+// entry is Inf/NaN) and spread[r] = e[r] - mean_k log2|x| over the non-zero
+// entries: the bits between the row's scale and its typical entry (geometric
+// mean, so heavy-tailed rows show their full width), read by the slice-count
+// and moduli-count choosers.
+__device__ __forceinline__ double log2_abs(double v) {
+    int fe;
+    const double mnt = frexp(v, &fe);  // v = mnt * 2^fe, mnt in [0.5, 1)
+    return (double)fe + (double)__log2f((float)mnt);
+}
+
 __global__ __launch_bounds__(256) void row_stats_kcontig(const double* __restrict__ X, int64_t sr, int R, int Kd,
                                                          int* __restrict__ e, float* __restrict__ spread) {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= R) return;
     const double* row = X + (int64_t)r * sr;
-    double mx = 0.0, sum = 0.0;
+    double mx = 0.0, slog = 0.0, cnt = 0.0;
     bool bad = false;
     for (int k = lane; k < Kd; k += 64) {
         double v = fabs(row[k]);
         bad |= !(v <= 1.7976931348623157e308);
         mx = fmax(mx, v);
-        sum += v;
+        if (v > 0.0 && v <= 1.7976931348623157e308) {
+            slog += log2_abs(v);
+            cnt += 1.0;
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         mx = fmax(mx, __shfl_xor(mx, off, 64));
-        sum += __shfl_xor(sum, off, 64);
+        slog += __shfl_xor(slog, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
         bad |= __shfl_xor((int)bad, off, 64) != 0;
     }
     if (lane == 0) {
         int ex = 0;
         if (mx > 0.0) frexp(mx, &ex);
         e[r] = bad ? NONFINITE : ex;
-        spread[r] = (bad || sum == 0.0) ? 0.0f : (float)(ex - log2(sum / Kd));
+        spread[r] = (bad || cnt == 0.0) ? 0.0f : (float)(ex - slog / cnt);
     }
 }
 
-__global__ __launch_bounds__(256) void row_stats_rcontig(const double* __restrict__ X, int64_t sk, int R, int Kd,
-                                                         int* __restrict__ e, float* __restrict__ spread) {
+// Same statistics when consecutive rows r are contiguous (element (r, k) at
+// X[k*sk + r]). Two passes so that a long K fills the chip: pass 1 gives each
+// block 64 rows and one of kparts K ranges (wave w of the block takes every
+// fourth k of the range), and writes that range's {max or -1 for
+// non-finite, sum of log2|x|, non-zero count} to part[kp*R + r]; pass 2
+// combines the ranges in order, so the result does not depend on scheduling.
+struct StatPart {
+    double mx, slog, cnt, pad;
+};
+
+__device__ __forceinline__ void stat_add(double v, double& mx, double& slog, double& cnt, bool& bad) {
+    bad |= !(v <= 1.7976931348623157e308);
+    mx = fmax(mx, v);
+    if (v > 0.0 && v <= 1.7976931348623157e308) {
+        slog += log2_abs(v);
+        cnt += 1.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void col_stats_partial(const double* __restrict__ X, int64_t sk, int R, int Kd,
+                                                         int krange, StatPart* __restrict__ part) {
+    __shared__ StatPart red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = blockIdx.x * 64 + lane;
+    const int k0 = blockIdx.y * krange;
+    const int k1 = min(Kd, k0 + krange);
+    double mx = 0.0, slog = 0.0, cnt = 0.0;
+    bool bad = false;
+    if (r < R) {
+        const double* col = X + r;
+        int k = k0 + w;
+        for (; k + 12 < k1; k += 16) {
+            const double v0 = fabs(col[(int64_t)k * sk]), v1 = fabs(col[(int64_t)(k + 4) * sk]);
+            const double v2 = fabs(col[(int64_t)(k + 8) * sk]), v3 = fabs(col[(int64_t)(k + 12) * sk]);
+            stat_add(v0, mx, slog, cnt, bad);
+            stat_add(v1, mx, slog, cnt, bad);
+            stat_add(v2, mx, slog, cnt, bad);
+            stat_add(v3, mx, slog, cnt, bad);
+        }
+        for (; k < k1; k += 4) stat_add(fabs(col[(int64_t)k * sk]), mx, slog, cnt, bad);
+    }
+    red[w][lane] = StatPart{bad ? -1.0 : mx, slog, cnt, 0.0};
+    __syncthreads();
+    if (w == 0 && r < R) {
+        StatPart acc = red[0][lane];
+#pragma unroll
+        for (int i = 1; i < 4; i++) {
+            const StatPart t = red[i][lane];
+            acc.mx = (acc.mx < 0.0 || t.mx < 0.0) ? -1.0 : fmax(acc.mx, t.mx);
+            acc.slog += t.slog;
+            acc.cnt += t.cnt;
+        }
+        part[(int64_t)blockIdx.y * R + r] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void col_stats_finish(const StatPart* __restrict__ part, int R, int kparts,
+                                                        int* __restrict__ e, float* __restrict__ spread) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
-    double mx = 0.0, sum = 0.0;
+    double mx = 0.0, slog = 0.0, cnt = 0.0;
     bool bad = false;
-    for (int k = 0; k < Kd; k++) {
-        double v = fabs(X[(int64_t)k * sk + r]);
-        bad |= !(v <= 1.7976931348623157e308);
-        mx = fmax(mx, v);
-        sum += v;
+    for (int kp = 0; kp < kparts; kp++) {
+        const StatPart t = part[(int64_t)kp * R + r];
+        bad |= t.mx < 0.0;
+        mx = fmax(mx, t.mx);
+        slog += t.slog;
+        cnt += t.cnt;
     }
     int ex = 0;
     if (mx > 0.0) frexp(mx, &ex);
     e[r] = bad ? NONFINITE : ex;
-    spread[r] = (bad || sum == 0.0) ? 0.0f : (float)(ex - log2(sum / Kd));
+    spread[r] = (bad || cnt == 0.0) ? 0.0f : (float)(ex - slog / cnt);
 }
 
 // Writes the S slices of op(X) (element (r, k) at X[r*sr + k*sk], R x Kd,
@@ -295,6 +364,32 @@ void split_plan_sizes(int64_t m, int64_t n, int64_t k, int slices, SplitPlan* p)
     p->b_slice_bytes = p->npad * p->kpad;
 }
 
+int split_stats_kparts(int64_t rows, int64_t k) {
+    const int64_t col_blocks = (rows + 63) / 64;
+    int64_t kp = (2048 + col_blocks - 1) / col_blocks;       // aim for >= 2048 workgroups
+    kp = std::min<int64_t>(kp, std::max<int64_t>(1, k / 256));  // at least 256 k per range
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kp, 256));
+}
+
+int64_t split_stats_part_elems(int64_t m, int64_t n, int64_t k) {
+    return std::max(split_stats_kparts(m, k) * m, split_stats_kparts(n, k) * n);
+}
+
+namespace {
+hipError_t launch_col_stats(const double* x, int64_t sk, int R, int K, double* part, int* e, float* spread,
+                            hipStream_t stream) {
+    const int kparts = split_stats_kparts(R, K);
+    const int krange = (K + kparts - 1) / kparts;
+    StatPart* pp = reinterpret_cast<StatPart*>(part);
+    hipLaunchKernelGGL(col_stats_partial, dim3((R + 63) / 64, kparts), dim3(256), 0, stream, x, sk, R, K, krange,
+                       pp);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(col_stats_finish, dim3((R + 255) / 256), dim3(256), 0, stream, pp, R, kparts, e, spread);
+    return hipGetLastError();
+}
+}  // namespace
+
 hipError_t launch_split_stats(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
                               SplitPlan* p, hipStream_t stream) {
     const int M = (int)p->m, N = (int)p->n, K = (int)p->k;
@@ -303,8 +398,10 @@ hipError_t launch_split_stats(int layout_a, int layout_b, const double* a, int64
         hipLaunchKernelGGL(row_stats_kcontig, dim3((M + 3) / 4), dim3(256), 0, stream, a, lda, M, K, p->ea,
                            p->spread_a);
     else
-        hipLaunchKernelGGL(row_stats_rcontig, dim3((M + 255) / 256), dim3(256), 0, stream, a, lda, M, K, p->ea,
-                           p->spread_a);
+    {
+        hipError_t err = launch_col_stats(a, lda, M, K, p->stats_part, p->ea, p->spread_a, stream);
+        if (err != hipSuccess) return err;
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     // Columns of op(B): ROW -> element (k,j) at b[k*ldb + j]; COL -> b[j*ldb + k].
@@ -312,8 +409,7 @@ hipError_t launch_split_stats(int layout_a, int layout_b, const double* a, int64
         hipLaunchKernelGGL(row_stats_kcontig, dim3((N + 3) / 4), dim3(256), 0, stream, b, ldb, N, K, p->eb,
                            p->spread_b);
     else
-        hipLaunchKernelGGL(row_stats_rcontig, dim3((N + 255) / 256), dim3(256), 0, stream, b, ldb, N, K, p->eb,
-                           p->spread_b);
+        return launch_col_stats(b, ldb, N, K, p->stats_part, p->eb, p->spread_b, stream);
     return hipGetLastError();
 }
 
@@ -332,7 +428,9 @@ hipError_t launch_split_pack_and_gemm(int layout_a, int layout_b, const double* 
         if (err == hipSuccess)                                                                                  \
             err = launch_pack<S_>(b, bsr, bsk, N, K, (int)p.npad, (int)p.kpad, p.eb, p.b_pack, p.b_slice_bytes, \
                                   stream);                                                                      \
+        mark_main(info, true, stream);                                                                          \
         if (err == hipSuccess) err = launch_core<S_>(p, alpha, beta, c, ldc, stream);                           \
+        mark_main(info, false, stream);                                                                         \
         break;
         JLC_SPLIT_CASE(3)
         JLC_SPLIT_CASE(4)

@@ -34,9 +34,10 @@ COL = 1
 MODE_FP64 = 0
 MODE_SPLIT_INT8 = 1
 MODE_SPLIT_BF16 = 2
+MODE_SPLIT_CRT = 3
 
 KERNEL_NAMES = {0: "none", 1: "scale", 2: "mfma_f64", 3: "gemv", 4: "split_int8", 5: "split_bf16", 6: "grouped",
-                7: "qr"}
+                7: "qr", 8: "split_crt"}
 
 
 def mode_slices(mode: int, slices: int) -> int:
@@ -64,6 +65,9 @@ class Stats(C.Structure):
         ("tile_k", C.c_int32),
         ("slices", C.c_int32),
         ("launches", C.c_int32),
+        ("bits_a", C.c_int32),
+        ("bits_b", C.c_int32),
+        ("main_ms", C.c_double),
     ]
 
     @property

@@ -585,3 +585,200 @@ void oracle_split_gemm_digits(int a_layout, int b_layout, int64_t m, int64_t n, 
     free(ea);
     free(eb);
 }
+
+/* ---------------------------------------------------------------------------
+ * CRT split form (modular integer emulation; Ozaki, Uchino, Imamura 2025,
+ * "Ozaki Scheme II: a GEMM-oriented emulation of floating-point matrix
+ * multiplication using an integer modular technique"). No reference
+ * counterpart: the split-precision component of the DGEMM engine.
+ *
+ *   1. Row i of op(A) is scaled to integers a' = rint(a * 2^(pa - ea[i]))
+ *      (|a'| <= 2^pa), column j of op(B) likewise with pb, eb[j].
+ *   2. For moduli m_0..m_{N-1} (pairwise coprime, <= 256) the symmetric
+ *      residues a' mod m_t, b' mod m_t fit in int8; each modular product
+ *      C_t = (A'_t B'_t) mod m_t is one integer GEMM.
+ *   3. With M = prod m_t > 4 * k * 2^(pa+pb), C' = A'B' is the unique integer
+ *      in (-M/2, M/2) with those residues: X = sum_t r_t W_t (W_t = (M/m_t) *
+ *      ((M/m_t)^-1 mod m_t)) is formed exactly in four 40-bit limbs,
+ *      q = rint(X / M), C' = X - qM limb by limb, limbs balanced and summed
+ *      by Horner from the top in double-double, rounded once at the end.
+ *   4. C = alpha * C' * 2^(ea+eb-pa-pb) (+ beta*C through fma).
+ * ------------------------------------------------------------------------- */
+static const int CRT_MODULI[20] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                   223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+#define CRT_LIMBS 4
+#define CRT_LIMB_BITS 40
+
+int oracle_crt_max_moduli(void) { return 20; }
+int oracle_crt_modulus(int t) { return (t >= 0 && t < 20) ? CRT_MODULI[t] : 0; }
+
+double oracle_crt_log2_product(int nmod) {
+    double s = 0.0;
+    for (int t = 0; t < nmod && t < 20; t++) s += log2((double)CRT_MODULI[t]);
+    return s;
+}
+
+/* Little-endian base-2^32 big integers of 8 words (256 bits). */
+static void big_set_small(uint32_t* x, uint32_t v) {
+    memset(x, 0, 8 * sizeof(uint32_t));
+    x[0] = v;
+}
+static void big_mul_small(uint32_t* x, uint32_t v) {
+    uint64_t carry = 0;
+    for (int w = 0; w < 8; w++) {
+        uint64_t t = (uint64_t)x[w] * v + carry;
+        x[w] = (uint32_t)t;
+        carry = t >> 32;
+    }
+}
+static uint32_t big_mod_small(const uint32_t* x, uint32_t m) {
+    uint64_t r = 0;
+    for (int w = 7; w >= 0; w--) r = ((r << 32) | x[w]) % m;
+    return (uint32_t)r;
+}
+/* Bits [40*j, 40*j+40) of x as a double (exact). */
+static double big_limb(const uint32_t* x, int j) {
+    double v = 0.0;
+    for (int b = CRT_LIMB_BITS - 1; b >= 0; b--) {
+        int bit = j * CRT_LIMB_BITS + b;
+        v = v * 2.0 + (double)((x[bit >> 5] >> (bit & 31)) & 1u);
+    }
+    return v;
+}
+static double big_to_double(const uint32_t* x) {
+    double v = 0.0;
+    for (int w = 7; w >= 0; w--) v = v * 4294967296.0 + (double)x[w];
+    return v;
+}
+
+/* Tables of the reconstruction for N moduli: w[t*4 + j] = limb j of W_t,
+ * mlimb[j] = limb j of M, finv[j] = 2^(40 j) / M. */
+void oracle_crt_tables(int nmod, double* w, double* mlimb, double* finv) {
+    uint32_t M[8], Mi[8];
+    big_set_small(M, 1);
+    for (int t = 0; t < nmod; t++) big_mul_small(M, (uint32_t)CRT_MODULI[t]);
+    for (int t = 0; t < nmod; t++) {
+        big_set_small(Mi, 1);
+        for (int u = 0; u < nmod; u++)
+            if (u != t) big_mul_small(Mi, (uint32_t)CRT_MODULI[u]);
+        const uint32_t m = (uint32_t)CRT_MODULI[t];
+        const uint32_t r = big_mod_small(Mi, m);
+        uint32_t y = 0;
+        for (uint32_t c = 1; c < m; c++)
+            if ((r * c) % m == 1) { y = c; break; }
+        big_mul_small(Mi, y);
+        for (int j = 0; j < CRT_LIMBS; j++) w[t * CRT_LIMBS + j] = big_limb(Mi, j);
+    }
+    const double md = big_to_double(M);
+    for (int j = 0; j < CRT_LIMBS; j++) {
+        mlimb[j] = big_limb(M, j);
+        finv[j] = ldexp(1.0, CRT_LIMB_BITS * j) / md;
+    }
+}
+
+/* Symmetric residue of an integer-valued double |x| < 2^71: in [-128, 127] for
+ * m = 256 and [-(m-1)/2, (m-1)/2] for odd m. */
+static int crt_residue(double x, int m) {
+    __int128 v = (__int128)x;
+    int r = (int)(v % m);
+    if (r > (m - 1) / 2) r -= m;
+    if (r < -(m / 2)) r += m;
+    return r;
+}
+
+/* Integer image of x in a row with exponent e scaled to p bits. */
+static double crt_scaled(double x, int e, int p) {
+    double s = ldexp(x, p - e);
+    return fabs(s) < 9007199254740992.0 ? rint(s) : s;
+}
+
+void oracle_crt_residues(const double* x, int64_t stride, int64_t count, int e, int p, int nmod, int8_t* out,
+                         int64_t out_stride) {
+    for (int64_t q = 0; q < count; q++) {
+        const double s = (e == SPLIT_NONFINITE) ? 0.0 : crt_scaled(x[q * stride], e, p);
+        for (int t = 0; t < nmod; t++) out[t * out_stride + q] = (int8_t)crt_residue(s, CRT_MODULI[t]);
+    }
+}
+
+/* Reconstructs the integer with the given residues (|value| < M/4) as a double. */
+double oracle_crt_reconstruct(const int* r, int nmod, const double* w, const double* mlimb, const double* finv) {
+    double X[CRT_LIMBS];
+    for (int j = 0; j < CRT_LIMBS; j++) {
+        double s = 0.0;
+        for (int t = 0; t < nmod; t++) s += (double)r[t] * w[t * CRT_LIMBS + j];
+        X[j] = s;
+    }
+    double qf = 0.0;
+    for (int j = CRT_LIMBS - 1; j >= 0; j--) qf += X[j] * finv[j];
+    const double q = rint(qf);
+    double D[CRT_LIMBS];
+    for (int j = 0; j < CRT_LIMBS; j++) D[j] = fma(-q, mlimb[j], X[j]);
+    const double base = ldexp(1.0, CRT_LIMB_BITS);
+    for (int j = 0; j < CRT_LIMBS - 1; j++) {
+        const double c = rint(ldexp(D[j], -CRT_LIMB_BITS));
+        D[j] = D[j] - c * base;
+        D[j + 1] = D[j + 1] + c;
+    }
+    /* Horner from the top limb in double-double (TwoSum per step). */
+    double h = D[CRT_LIMBS - 1], l = 0.0;
+    for (int j = CRT_LIMBS - 2; j >= 0; j--) {
+        const double hb = h * base;
+        const double s = hb + D[j];
+        const double bb = s - hb;
+        const double e = (hb - (s - bb)) + (D[j] - bb);
+        l = l * base + e;
+        h = s;
+    }
+    return h + l;
+}
+
+void oracle_crt_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                     int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int nmod, int pa,
+                     int pb, int threads) {
+    double w[20 * CRT_LIMBS], mlimb[CRT_LIMBS], finv[CRT_LIMBS];
+    oracle_crt_tables(nmod, w, mlimb, finv);
+    int8_t* ar = (int8_t*)calloc((size_t)(nmod * m * k + 1), 1);
+    int8_t* br = (int8_t*)calloc((size_t)(nmod * n * k + 1), 1);
+    int* ea = (int*)malloc(sizeof(int) * (size_t)(m + 1));
+    int* eb = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+    for (int64_t i = 0; i < m; i++) {
+        const double* row = a_layout == ORACLE_ROW ? a + i * lda : a + i;
+        int64_t st = a_layout == ORACLE_ROW ? 1 : lda;
+        ea[i] = oracle_split_exponent(row, st, k);
+        oracle_crt_residues(row, st, k, ea[i], pa, nmod, ar + i * k, m * k);
+    }
+    for (int64_t j = 0; j < n; j++) {
+        const double* col = b_layout == ORACLE_ROW ? b + j : b + j * ldb;
+        int64_t st = b_layout == ORACLE_ROW ? ldb : 1;
+        eb[j] = oracle_split_exponent(col, st, k);
+        oracle_crt_residues(col, st, k, eb[j], pb, nmod, br + j * k, n * k);
+    }
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int64_t i = 0; i < m; i++) {
+        for (int64_t j = 0; j < n; j++) {
+            int r[20];
+            for (int t = 0; t < nmod; t++) {
+                const int8_t* x = ar + (int64_t)t * m * k + i * k;
+                const int8_t* y = br + (int64_t)t * n * k + j * k;
+                int64_t s = 0;
+                for (int64_t q = 0; q < k; q++) s += (int64_t)x[q] * y[q];
+                const int mt = CRT_MODULI[t];
+                int rr = (int)(s % mt);
+                if (rr > (mt - 1) / 2) rr -= mt;
+                if (rr < -(mt / 2)) rr += mt;
+                r[t] = rr;
+            }
+            const double cp = oracle_crt_reconstruct(r, nmod, w, mlimb, finv);
+            const double v = (ea[i] == SPLIT_NONFINITE || eb[j] == SPLIT_NONFINITE)
+                                 ? NAN
+                                 : ldexp(cp, ea[i] + eb[j] - pa - pb);
+            double* p = &c[i * ldc + j];
+            *p = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * *p);
+        }
+    }
+    free(ar);
+    free(br);
+    free(ea);
+    free(eb);
+}

@@ -16,6 +16,8 @@
  *   - oracle_matvec            SpecializedKernels.matvec      (src/main/java/net/faulj/compute/SpecializedKernels.java:27-71)
  *   - oracle_dd_*              double-double dot products (Knuth TwoSum, FMA TwoProduct; Ogita-Rump-Oishi 2005).
  *                              No reference counterpart: it is the accuracy oracle for the split-precision form.
+ *   - oracle_crt_*             CRT split form (modular INT8 products + Chinese remainder reconstruction).
+ *                              No reference counterpart: the north star's split-precision component.
  *   - oracle_qr_*              HouseholderQR blocked restatement (src/main/java/net/faulj/decomposition/qr/HouseholderQR.java)
  *
  * Parity pins (the checks that make a plausible mistake fail) live in tests/test_oracle.py.
@@ -99,6 +101,21 @@ void oracle_split_gemm_digits(int a_layout, int b_layout, int64_t m, int64_t n, 
                               const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
                               int64_t ldc, int slices, int bits, int64_t chunk_k, int threads);
 int64_t oracle_split_bf16_chunk_k(int slices, int64_t k);
+
+/* CRT split form (modular integer products recombined by the Chinese remainder
+ * theorem; Ozaki-Uchino-Imamura 2025). Moduli: a fixed list of 20 pairwise
+ * coprime integers <= 256. nmod moduli, pa / pb bits for op(A) rows / op(B)
+ * columns; requires log2(prod m) >= pa + pb + ceil(log2 k) + 2. */
+int oracle_crt_max_moduli(void);
+int oracle_crt_modulus(int t);
+double oracle_crt_log2_product(int nmod);
+void oracle_crt_tables(int nmod, double* w, double* mlimb, double* finv);
+void oracle_crt_residues(const double* x, int64_t stride, int64_t count, int e, int p, int nmod, int8_t* out,
+                         int64_t out_stride);
+double oracle_crt_reconstruct(const int* r, int nmod, const double* w, const double* mlimb, const double* finv);
+void oracle_crt_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                     int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int nmod, int pa,
+                     int pb, int threads);
 
 /* Blocked Householder QR restatement (column-major in/out). */
 int oracle_qr_factor(int64_t m, int64_t n, double* a_colmajor, int64_t lda, double* tau, int64_t nb);

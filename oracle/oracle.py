@@ -68,6 +68,15 @@ def _load():
                                              _i64, C.c_int, C.c_int, _i64, C.c_int]
     lib.oracle_split_bf16_chunk_k.argtypes = [C.c_int, _i64]
     lib.oracle_split_bf16_chunk_k.restype = _i64
+    lib.oracle_crt_modulus.argtypes = [C.c_int]
+    lib.oracle_crt_log2_product.argtypes = [C.c_int]
+    lib.oracle_crt_log2_product.restype = _d
+    lib.oracle_crt_tables.argtypes = [C.c_int, _p, _p, _p]
+    lib.oracle_crt_residues.argtypes = [_p, _i64, _i64, C.c_int, C.c_int, C.c_int, _p, _i64]
+    lib.oracle_crt_reconstruct.argtypes = [_p, C.c_int, _p, _p, _p]
+    lib.oracle_crt_reconstruct.restype = _d
+    lib.oracle_crt_gemm.argtypes = [C.c_int, C.c_int, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64,
+                                    C.c_int, C.c_int, C.c_int, C.c_int]
     lib.oracle_qr_factor.argtypes = [_i64, _i64, _p, _i64, _p, _i64]
     lib.oracle_qr_form_q.argtypes = [_i64, _i64, _p, _i64, _p, _p, _i64]
     _lib = lib
@@ -288,6 +297,58 @@ def split_gemm(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: floa
 
 def split_bf16_chunk_k(slices: int, k: int) -> int:
     return _load().oracle_split_bf16_chunk_k(slices, k)
+
+
+CRT_MAX_MODULI = 20
+
+
+def crt_moduli(nmod: int = CRT_MAX_MODULI) -> list[int]:
+    lib = _load()
+    return [lib.oracle_crt_modulus(t) for t in range(nmod)]
+
+
+def crt_log2_product(nmod: int) -> float:
+    return _load().oracle_crt_log2_product(nmod)
+
+
+def crt_tables(nmod: int):
+    """(w[nmod, 4], mlimb[4], finv[4]) of the CRT reconstruction."""
+    w = np.zeros(nmod * 4)
+    ml = np.zeros(4)
+    fi = np.zeros(4)
+    _load().oracle_crt_tables(nmod, _ptr(w), _ptr(ml), _ptr(fi))
+    return w.reshape(nmod, 4), ml, fi
+
+
+def crt_residues(x: np.ndarray, e: int, p: int, nmod: int) -> np.ndarray:
+    """Symmetric residues (nmod x len(x), int8) of the p-bit integer images of x (row exponent e)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((nmod, x.size), dtype=np.int8)
+    _load().oracle_crt_residues(_ptr(x), 1, x.size, e, p, nmod, out.ctypes.data, x.size)
+    return out
+
+
+def crt_reconstruct(residues, nmod: int) -> float:
+    w, ml, fi = crt_tables(nmod)
+    r = np.ascontiguousarray(residues, dtype=np.int32)
+    return _load().oracle_crt_reconstruct(r.ctypes.data, nmod, _ptr(w), _ptr(ml), _ptr(fi))
+
+
+def crt_gemm(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, a, lda: int, b, ldb: int,
+             beta: float, c, ldc: int, nmod: int, pa: int, pb: int, a_off: int = 0, b_off: int = 0, c_off: int = 0,
+             nthreads: int | None = None) -> None:
+    """In-place CRT split-form GEMM restatement (bit-exact reference of the CRT form)."""
+    _load().oracle_crt_gemm(layout_a, layout_b, m, n, k, alpha, _ptr(a, a_off), lda, _ptr(b, b_off), ldb, beta,
+                            _ptr(c, c_off), ldc, nmod, pa, pb, nthreads or threads())
+
+
+def crt_bits_for(nmod: int, k: int) -> tuple[int, int]:
+    """(pa, pb) the CRT form uses for an explicit moduli count: the available
+    bits floor(log2 M) - ceil(log2 k) - 2 split evenly, at most 66 each."""
+    lk = int(np.ceil(np.log2(k))) if k > 1 else 0
+    avail = int(np.floor(crt_log2_product(nmod))) - lk - 2
+    pa = min(avail // 2, 66)
+    return pa, min(avail - pa, 66)
 
 
 def split_gemm_bf16(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, a, lda: int, b, ldb: int,

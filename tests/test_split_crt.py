@@ -1,0 +1,146 @@
+"""GPU parity of the CRT split form (JLC_MODE_SPLIT_CRT).
+
+Residues, integer products and the CRT reconstruction are exact integer
+work and the final conversion follows a fixed FP64 sequence, so the GPU
+result must equal the oracle's restatement (oracle_crt_gemm) bit for bit.
+Accuracy is then stated against the double-double oracle.
+"""
+import numpy as np
+import pytest
+
+from jlc_amd import native
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _operand(rng, r, c, phi):
+    return (rng.random((r, c)) - 0.5) * np.exp(phi * rng.standard_normal((r, c)))
+
+
+def _store(X, layout, ld, off, pad_rows):
+    r, c = X.shape
+    buf = np.zeros(off + ld * (r if layout == 0 else c))
+    if layout == 0:
+        for i in range(r):
+            buf[off + i * ld: off + i * ld + c] = X[i]
+    else:
+        for j in range(c):
+            buf[off + j * ld: off + j * ld + r] = X[:, j]
+    return buf
+
+
+def _case(rng, la, lb, m, n, k, nmod, alpha, beta, phi=1.0, pad=0, off=0):
+    A = _operand(rng, m, k, phi)
+    B = _operand(rng, k, n, phi)
+    lda = (k if la == 0 else m) + pad
+    ldb = (n if lb == 0 else k) + pad
+    ldc = n + pad
+    a = _store(A, la, lda, off, pad)
+    b = _store(B, lb, ldb, off, pad)
+    c = rng.uniform(-1, 1, off + ldc * m)
+    c0 = c.copy()
+    st = native.dgemm(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                      native.mode_slices(native.MODE_SPLIT_CRT, nmod), off, off, off)
+    assert st.kernel_name == "split_crt", st.kernel_name
+    if nmod:
+        assert st.slices == nmod and (st.bits_a, st.bits_b) == orc.crt_bits_for(nmod, k)
+    exp = c0.copy()
+    orc.crt_gemm(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, exp, ldc, st.slices, st.bits_a, st.bits_b,
+                 a_off=off, b_off=off, c_off=off)
+    same = (c == exp) | (np.isnan(c) & np.isnan(exp))
+    assert same.all(), (la, lb, m, n, k, nmod, int((~same).sum()), float(np.nanmax(np.abs(c - exp))))
+    return st
+
+
+@pytest.mark.parametrize("nmod", [4, 8, 12, 16, 20])
+def test_bitwise_equal_to_restatement(nmod):
+    rng = np.random.default_rng(300 + nmod)
+    _case(rng, 0, 0, 256, 256, 64, nmod, 1.0, 0.0)
+    _case(rng, 0, 0, 200, 130, 97, nmod, -1.5, 0.5)
+
+
+@pytest.mark.parametrize("la,lb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_all_layouts_ragged(la, lb):
+    rng = np.random.default_rng(20 * la + lb)
+    for m, n, k in [(1, 1, 1), (33, 17, 5), (257, 65, 129), (300, 250, 200)]:
+        _case(rng, la, lb, m, n, k, 16, 2.5, -0.5, pad=1, off=3)
+
+
+def test_automatic_moduli_count():
+    rng = np.random.default_rng(8)
+    st = _case(rng, 0, 0, 300, 280, 1000, 0, 1.0, 0.0, phi=0.5)
+    assert 2 <= st.slices <= 20 and st.bits_a >= 40 and st.bits_b >= 40
+    need = st.bits_a + st.bits_b + int(np.ceil(np.log2(1000))) + 2
+    assert orc.crt_log2_product(st.slices) >= need > orc.crt_log2_product(st.slices - 1)
+
+
+def test_modular_fold_beyond_65536_k():
+    # INT32 sums are folded modulo m every 65536 k; 70000 crosses one fold.
+    rng = np.random.default_rng(70)
+    _case(rng, 0, 0, 24, 20, 70000, 16, 1.0, 0.0, phi=0.5)
+
+
+def test_nonfinite_rows_and_columns_are_nan():
+    rng = np.random.default_rng(4)
+    m, n, k = 70, 66, 40
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (k, n))
+    A[5, 3] = np.inf
+    B[7, 9] = np.nan
+    c = np.zeros(m * n)
+    native.dgemm(0, 0, m, n, k, 1.0, A.ravel().copy(), k, B.ravel().copy(), n, 0.0, c, n,
+                 native.mode_slices(native.MODE_SPLIT_CRT, 14))
+    c = c.reshape(m, n)
+    assert np.all(np.isnan(c[5])) and np.all(np.isnan(c[:, 9]))
+    ok = np.ones((m, n), bool)
+    ok[5] = False
+    ok[:, 9] = False
+    assert np.all(np.isfinite(c[ok]))
+
+
+def test_too_wide_spread_runs_fp64_kernel():
+    # Entries spanning ~2^200 within a row need more bits than 20 moduli hold.
+    rng = np.random.default_rng(5)
+    m, n, k = 64, 64, 64
+    A = rng.uniform(-1, 1, (m, k)) * np.exp2(rng.integers(-100, 100, (m, k)))
+    B = rng.uniform(-1, 1, (k, n))
+    c = np.zeros(m * n)
+    st = native.dgemm(0, 0, m, n, k, 1.0, A.ravel().copy(), k, B.ravel().copy(), n, 0.0, c, n,
+                      native.MODE_SPLIT_CRT)
+    assert st.kernel_name == "mfma_f64"
+    c2 = np.zeros(m * n)
+    native.dgemm(0, 0, m, n, k, 1.0, A.ravel().copy(), k, B.ravel().copy(), n, 0.0, c2, n, native.MODE_FP64)
+    assert np.array_equal(c, c2)
+
+
+def test_invalid_moduli_count_rejected():
+    a = np.ones(4)
+    c = np.zeros(4)
+    with pytest.raises(native.JlcError):
+        native.dgemm(0, 0, 2, 2, 2, 1.0, a, 2, a, 2, 0.0, c, 2, native.mode_slices(native.MODE_SPLIT_CRT, 21))
+    with pytest.raises(native.JlcError):
+        native.dgemm(0, 0, 2, 2, 2, 1.0, a, 2, a, 2, 0.0, c, 2, native.mode_slices(native.MODE_SPLIT_CRT, 1))
+
+
+def test_accuracy_matches_fp64_at_auto_moduli():
+    """Against double-double sums: the automatic choice keeps the max relative
+    and componentwise errors within those of the FP64 MFMA kernel (factor 2)."""
+    rng = np.random.default_rng(12)
+    m = n = 512
+    k = 2048
+    A = _operand(rng, m, k, 0.5)
+    B = _operand(rng, k, n, 0.5)
+    a, b = A.ravel().copy(), B.ravel().copy()
+    ii = rng.integers(0, m, 400)
+    jj = rng.integers(0, n, 400)
+    hi, lo, ab = orc.dd_entries(0, 0, k, a, k, b, n, ii, jj)
+    errs = {}
+    for name, mode in (("fp64", native.MODE_FP64), ("crt", native.MODE_SPLIT_CRT)):
+        c = np.zeros(m * n)
+        native.dgemm(0, 0, m, n, k, 1.0, a, k, b, n, 0.0, c, n, mode)
+        got = c.reshape(m, n)[ii, jj]
+        err = np.abs(got - hi - lo)
+        errs[name] = (float(np.max(err / np.abs(hi + lo))), float(np.max(err / ab)))
+    assert errs["crt"][0] <= 2 * errs["fp64"][0] + 1e-16, errs
+    assert errs["crt"][1] <= 2 * errs["fp64"][1] + 2 ** -53, errs
