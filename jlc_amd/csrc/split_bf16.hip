@@ -106,27 +106,24 @@ __global__ __launch_bounds__(256) void split_gemm_bf16(int M, int N, int ksteps,
     int tm, tn;
     tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
     const int mb0 = tm * 2, nb0 = tn * 2;
-    const v4i* src[PER];
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-        int b = w + 4 * i;
-        if (b >= NBLK) b = 0;
-        if (b < NA) src[i] = Ap + (b / 2) * a_slice + (int64_t)(mb0 + (b % 2)) * ksteps * 64;
-        else {
-            b -= NA;
-            src[i] = Bp + (b / 2) * b_slice + (int64_t)(nb0 + (b % 2)) * ksteps * 64;
-        }
-    }
+    // Unconditional block moves (a share past NBLK repeats block NBLK-1, same
+    // bytes to the same slot) with addresses rebuilt from wave-uniform scalars.
+    auto blk_of = [&](int i) {
+        const int b = w + 4 * i;
+        return b < NBLK ? b : NBLK - 1;
+    };
+    auto src_of = [&](int b) {
+        return b < NA ? Ap + (b / 2) * a_slice + (int64_t)(mb0 + (b % 2)) * ksteps * 64
+                      : Bp + ((b - NA) / 2) * b_slice + (int64_t)(nb0 + ((b - NA) % 2)) * ksteps * 64;
+    };
     v4i pre[PER];
     auto load = [&](int kc) {
 #pragma unroll
-        for (int i = 0; i < PER; i++)
-            if (w + 4 * i < NBLK) pre[i] = src[i][(int64_t)kc * 64 + lane];
+        for (int i = 0; i < PER; i++) pre[i] = src_of(blk_of(i))[(int64_t)kc * 64 + lane];
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < PER; i++)
-            if (w + 4 * i < NBLK) lds[(buf * NBLK + w + 4 * i) * 64 + lane] = pre[i];
+        for (int i = 0; i < PER; i++) lds[(buf * NBLK + blk_of(i)) * 64 + lane] = pre[i];
     };
     v16f accf[S];
     v16i acci[S];
@@ -157,24 +154,31 @@ __global__ __launch_bounds__(256) void split_gemm_bf16(int M, int N, int ksteps,
     load(0);
     store(0);
     __syncthreads();
-    for (int kc = 0; kc < ksteps; kc++) {
-        const int cur = kc & 1;
-        const bool more = kc + 1 < ksteps;
-        if (more) load(kc + 1);
-        const v4i* st = lds + cur * NBLK * 64 + lane;
-        bf8 bf[S];
+    // Branch-free K steps inside each exact-FP32 run (clamped prefetch,
+    // unconditional LDS store of the next step), so the wait counts keep the
+    // next step's loads in flight; chunk is a multiple of fsteps.
+    const int last = ksteps - 1;
+    for (int c0 = 0; c0 < ksteps; c0 += chunk) {
+        const int c1 = min(ksteps, c0 + chunk);
+        for (int f0 = c0; f0 < c1; f0 += fsteps) {
+            const int f1 = min(c1, f0 + fsteps);
+            for (int kc = f0; kc < f1; kc++) {
+                const int cur = kc & 1;
+                load(min(kc + 1, last));
+                const v4i* st = lds + cur * NBLK * 64 + lane;
+                bf8 bf[S];
 #pragma unroll
-        for (int u = 0; u < S; u++) bf[u] = __builtin_bit_cast(bf8, st[(NA + u * 2 + wn) * 64]);
+                for (int u = 0; u < S; u++) bf[u] = __builtin_bit_cast(bf8, st[(NA + u * 2 + wn) * 64]);
 #pragma unroll
-        for (int t = 0; t < S; t++) {
-            const bf8 af = __builtin_bit_cast(bf8, st[(t * 2 + wm) * 64]);
+                for (int t = 0; t < S; t++) {
+                    const bf8 af = __builtin_bit_cast(bf8, st[(t * 2 + wm) * 64]);
 #pragma unroll
-            for (int u = S - 1 - t; u >= 0; u--)
-                accf[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, bf[u], accf[t + u], 0, 0, 0);
-        }
-        if (more) store(cur ^ 1);
-        __syncthreads();
-        if ((kc + 1) % fsteps == 0 || !more) {
+                    for (int u = S - 1 - t; u >= 0; u--)
+                        accf[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, bf[u], accf[t + u], 0, 0, 0);
+                }
+                store(cur ^ 1);
+                __syncthreads();
+            }
             // FP32 diagonal sums are exact integers here: move them into INT32.
 #pragma unroll
             for (int d = 0; d < S; d++) {
@@ -183,11 +187,9 @@ __global__ __launch_bounds__(256) void split_gemm_bf16(int M, int N, int ksteps,
                 accf[d] = v16f{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
             }
         }
-        if ((kc + 1) % chunk == 0 || !more) {
-            flush(kc + 1 <= chunk);
+        flush(c0 == 0);
 #pragma unroll
-            for (int d = 0; d < S; d++) acci[d] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        }
+        for (int d = 0; d < S; d++) acci[d] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     }
 }
 

@@ -45,6 +45,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 constexpr int NONFINITE = -100000;
 constexpr int CT = 256;        // C tile rows and columns
 constexpr int KSTAGE = 64;     // K per LDS stage (two 32-deep MFMA steps)
+constexpr int KPAD = 128;      // K padding: an even number of stages
 constexpr int FOLD_STEPS = 2048;  // 32-deep steps between modular folds (65536 k)
 
 // Symmetric residue of an integer-valued double |y| < 2^53 modulo m.
@@ -141,10 +142,11 @@ __device__ __forceinline__ v16i reduce_acc(v16i acc, double m, double inv_m, dou
 
 // One INT8 GEMM per modulus (blockIdx.y); C residues out in accumulator order.
 // Wavefront w: rows (w>>2)*128 + [0,128) as 4 blocks, columns (w&3)*64 + [0,64) as 2 blocks.
-// K runs in 64-deep stages through a ring of three 32 KiB LDS buffers fed from
-// two register sets: stage s+2 is fetched from global memory while stage s is
-// multiplied and stage s+1 (fetched one stage earlier) is written to LDS, so a
-// global load has two stages of MFMA work to arrive. One barrier per stage.
+// K runs in 64-deep stages (an even number of them: K is padded to 128) through
+// a ring of three 32 KiB LDS buffers fed from two register sets: stage s+2 is
+// fetched from global memory while stage s is multiplied and stage s+1 (fetched
+// one stage earlier) is written to LDS, so a global load has two stages of MFMA
+// work to arrive. One barrier per stage.
 constexpr int NBUF_LDS = 3;
 constexpr int STAGE_V4 = 32 * 64;  // v4i per stage (32 blocks of 1 KiB)
 
@@ -213,33 +215,32 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
 
+    // Iteration s: fetch stage s+2, multiply stage s, write stage s+1 to LDS.
+    // The loop is branch-free (the stage count is even and the prefetch index is
+    // clamped; the clamped repeats land in a buffer nobody reads), so the
+    // compiler's wait counts let each register set's loads stay in flight for
+    // two stages instead of draining at every join.
     load(0, r0);
     store(0, r0);
-    if (stages > 1) load(1, r1);
+    load(1, r1);
     __syncthreads();
-    // Iteration s: fetch stage s+2, multiply stage s, write stage s+1 to LDS.
-    // Unrolled by two so the register sets are named statically: even s uses
-    // r0 for the fetch and r1 for the LDS write, odd s the reverse.
+    const int last = stages - 1;
     int buf = 0;
     for (int s = 0; s < stages; s += 2) {
-        {
-            if (s + 2 < stages) load(s + 2, r0);
-            compute(buf);
-            const int nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
-            if (s + 1 < stages) store(nb, r1);
-            __syncthreads();
-            fold(s);
-            buf = nb;
-        }
-        if (s + 1 < stages) {
-            if (s + 3 < stages) load(s + 3, r1);
-            compute(buf);
-            const int nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
-            if (s + 2 < stages) store(nb, r0);
-            __syncthreads();
-            fold(s + 1);
-            buf = nb;
-        }
+        load(min(s + 2, last), r0);
+        compute(buf);
+        int nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
+        store(nb, r1);
+        __syncthreads();
+        fold(s);
+        buf = nb;
+        load(min(s + 3, last), r1);
+        compute(buf);
+        nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
+        store(nb, r0);
+        __syncthreads();
+        fold(s + 1);
+        buf = nb;
     }
     // Residues of C' in accumulator order: block (row block, col block) of the plane.
 #pragma unroll
@@ -418,7 +419,7 @@ void crt_plan_sizes(int64_t m, int64_t n, int64_t k, int nmod, CrtPlan* p) {
     p->nmod = nmod;
     p->mpad = (m + CT - 1) / CT * CT;
     p->npad = (n + CT - 1) / CT * CT;
-    p->kpad = (k + KSTAGE - 1) / KSTAGE * KSTAGE;
+    p->kpad = (k + KPAD - 1) / KPAD * KPAD;
     p->a_plane_bytes = p->mpad * p->kpad;
     p->b_plane_bytes = p->npad * p->kpad;
     p->c_plane_bytes = p->mpad * p->npad;

@@ -245,28 +245,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
     const int mb0 = tm * (TM / 32), nb0 = tn * (TN / 32);
 
-    const v4i* src[PER];
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-        int b = w + 8 * i;
-        if (b >= NBLK) b = 0;
-        if (b < NA) {
-            src[i] = Ap + (b / 4) * a_slice + (int64_t)(mb0 + (b % 4)) * ksteps * 64;
-        } else {
-            b -= NA;
-            src[i] = Bp + (b / 2) * b_slice + (int64_t)(nb0 + (b % 2)) * ksteps * 64;
-        }
-    }
+    // Block moves are unconditional: a wavefront whose share runs past NBLK
+    // repeats block NBLK-1, writing the same bytes to the same LDS slot as the
+    // wavefront that owns it. Addresses are rebuilt from wave-uniform scalars at
+    // each use, so no per-lane pointer array stays live across the K loop.
+    auto blk_of = [&](int i) {
+        const int b = w + 8 * i;
+        return b < NBLK ? b : NBLK - 1;
+    };
+    auto src_of = [&](int b) {
+        return b < NA ? Ap + (b / 4) * a_slice + (int64_t)(mb0 + (b % 4)) * ksteps * 64
+                      : Bp + ((b - NA) / 2) * b_slice + (int64_t)(nb0 + ((b - NA) % 2)) * ksteps * 64;
+    };
     v4i pre[PER];
     auto load = [&](int kc) {
 #pragma unroll
-        for (int i = 0; i < PER; i++)
-            if (w + 8 * i < NBLK) pre[i] = src[i][(int64_t)kc * 64 + lane];
+        for (int i = 0; i < PER; i++) pre[i] = src_of(blk_of(i))[(int64_t)kc * 64 + lane];
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < PER; i++)
-            if (w + 8 * i < NBLK) lds[(buf * NBLK + w + 8 * i) * 64 + lane] = pre[i];
+        for (int i = 0; i < PER; i++) lds[(buf * NBLK + blk_of(i)) * 64 + lane] = pre[i];
     };
 
     v16i acc[S];
@@ -297,30 +295,35 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     load(0);
     store(0);
     __syncthreads();
-    for (int kc = 0; kc < ksteps; kc++) {
-        const int cur = kc & 1;
-        const bool more = kc + 1 < ksteps;
-        if (more) load(kc + 1);
-        const v4i* st = lds + cur * NBLK * 64 + lane;
-        v4i bf[S];
+    // The K loop is branch-free inside a chunk (the prefetch index is clamped and
+    // the LDS store is unconditional; the repeat of the last step lands in the
+    // buffer nobody reads), so the compiler's wait counts keep the next step's
+    // loads in flight across the MFMAs instead of draining them at a join.
+    const int last = ksteps - 1;
+    for (int c0 = 0; c0 < ksteps; c0 += chunk) {
+        const int c1 = min(ksteps, c0 + chunk);
+        for (int kc = c0; kc < c1; kc++) {
+            const int cur = kc & 1;
+            load(min(kc + 1, last));
+            const v4i* st = lds + cur * NBLK * 64 + lane;
+            v4i bf[S];
 #pragma unroll
-        for (int u = 0; u < S; u++) bf[u] = st[(NA + u * 2 + wn) * 64];
-        // t outer, u descending: consecutive MFMAs always target different diagonals.
-        // The next stage's LDS store sits after the first two slice rows of MFMAs.
+            for (int u = 0; u < S; u++) bf[u] = st[(NA + u * 2 + wn) * 64];
+            // t outer, u descending: consecutive MFMAs always target different diagonals.
+            // The next stage's LDS store sits after the first two slice rows of MFMAs.
 #pragma unroll
-        for (int t = 0; t < S; t++) {
-            if (t == 2 && more) store(cur ^ 1);
-            const v4i af = st[(t * 4 + wm) * 64];
+            for (int t = 0; t < S; t++) {
+                if (t == 2) store(cur ^ 1);
+                const v4i af = st[(t * 4 + wm) * 64];
 #pragma unroll
-            for (int u = S - 1 - t; u >= 0; u--)
-                acc[t + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[u], acc[t + u], 0, 0, 0);
+                for (int u = S - 1 - t; u >= 0; u--)
+                    acc[t + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[u], acc[t + u], 0, 0, 0);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if ((kc + 1) % chunk == 0 || !more) {
-            flush(kc + 1 <= chunk);
+        flush(c0 == 0);
 #pragma unroll
-            for (int d = 0; d < S; d++) acc[d] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        }
+        for (int d = 0; d < S; d++) acc[d] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     }
 }
 
