@@ -398,6 +398,9 @@ def main() -> int:
     ap.add_argument("--no-split", action="store_true", help="time only the FP64 MFMA form")
     ap.add_argument("--slices", type=int, default=0, help="INT8 slice count for the split form (0 = auto)")
     ap.add_argument("--moduli", type=int, default=0, help="moduli count for the CRT form (0 = auto)")
+    ap.add_argument("--forms", default="", help="split forms to time, comma list of crt,int8,bf16 (default all)")
+    ap.add_argument("--host-path", action="store_true",
+                    help="also time the host-pointer entry point (PCIe staging included) on the value form")
     ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched"], default="dgemm",
                     help="dgemm: the 16384^3 metric workload; batched: the grouped-GEMM configuration")
     ap.add_argument("--count", type=int, default=100000, help="problems in the batched workload")
@@ -479,10 +482,14 @@ def main() -> int:
 
     fp64 = run_form("fp64_mfma", native.MODE_FP64)
     splits = []
+    wanted = set(args.forms.split(",")) if args.forms else {"crt", "int8", "bf16"}
     if not args.no_split:
-        splits.append(run_form("split_crt", native.mode_slices(native.MODE_SPLIT_CRT, args.moduli)))
-        splits.append(run_form("split_int8", native.mode_slices(native.MODE_SPLIT_INT8, args.slices)))
-        splits.append(run_form("split_bf16", native.mode_slices(native.MODE_SPLIT_BF16, 0)))
+        if "crt" in wanted:
+            splits.append(run_form("split_crt", native.mode_slices(native.MODE_SPLIT_CRT, args.moduli)))
+        if "int8" in wanted:
+            splits.append(run_form("split_int8", native.mode_slices(native.MODE_SPLIT_INT8, args.slices)))
+        if "bf16" in wanted:
+            splits.append(run_form("split_bf16", native.mode_slices(native.MODE_SPLIT_BF16, 0)))
     # value: best throughput among forms whose max relative error is at most the FP64 MFMA form's.
     best = fp64
     for sf in splits:
@@ -523,6 +530,20 @@ def main() -> int:
         "roofline": roofline,
         "speedup_vs_fp64": {f["form"]: round(f["tflops"] / fp64["tflops"], 3) for f in splits},
     }
+    if args.host_path:
+        mode = {"fp64_mfma": native.MODE_FP64, "split_crt": native.MODE_SPLIT_CRT,
+                "split_int8": native.MODE_SPLIT_INT8, "split_bf16": native.MODE_SPLIT_BF16}[best["form"]]
+        c = np.zeros(m * n)
+        native.dgemm(native.ROW, native.ROW, m, n, k, 1.0, a, k, b, n, 0.0, c, n, mode)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            native.dgemm(native.ROW, native.ROW, m, n, k, 1.0, a, k, b, n, 0.0, c, n, mode)
+        hms = (time.perf_counter() - t0) * 1e3 / reps
+        out["host_path"] = {"form": best["form"], "ms_per_call": round(hms, 2),
+                            "tflops": round(flops / (hms * 1e-3) / 1e12, 3),
+                            "note": "jlc_dgemm on host arrays: staging of A, B (and C) over PCIe included"}
+        log(f"host-pointer path: {hms:.1f} ms, {out['host_path']['tflops']} TFLOP/s")
     if not args.no_cpu_baseline:
         log("timing CPU baseline")
         out["cpu_baseline"] = cpu_baseline()
