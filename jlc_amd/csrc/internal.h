@@ -117,8 +117,9 @@ struct QrBatchDesc {
 };
 int qr_panel_chunk();
 // Factors panel columns [k0, k1) of the column-major m-row matrix at (in place, tau[k0..k1)).
-// part: 2*G+1 doubles, dots: G*128 doubles, G = ceil((m-k0)/qr_panel_chunk()); k1 - k0 <= 128.
+// part: 2*G+1 doubles, dots: G*128 doubles, vtmp: m doubles, G = ceil((m-k0)/qr_panel_chunk()); k1 - k0 <= 128.
 hipError_t launch_qr_panel(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part, double* dots,
+                           double* vtmp,
                            hipStream_t stream);
 hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb, double* vc, hipStream_t stream);
 hipError_t launch_build_t(const double* g, int nb, const double* tau, double* T, hipStream_t stream);

@@ -13,18 +13,21 @@
 // W = V^T C, W = T^T W, C -= V W (updateTrailingMatrix, :167-209) — the
 // three GEMMs of that update run on this library's GEMM entry points.
 //
-// Panel kernels. Rows k0..m-1 of the panel are cut into chunks of PCHUNK rows,
-// one workgroup each. Per column k two launches:
-//   panel_update_norm: apply reflector k-1 to columns k..k1-1 (w_j reduced
-//     from the dot partials, every workgroup in the same order) and produce
-//     the per-chunk (max, scaled sum of squares) of column k below the pivot;
-//     the owner of row k also saves the pivot x0.
-//   panel_reflect_dots: every workgroup combines the norm partials in the same
-//     order (so all agree on beta and tau), scales its part of v, and emits
-//     per-chunk dot products v . a_j for j = k+1..k1-1; the owner of row k
-//     writes R(k,k) = beta and tau[k].
-// No workgroup reads data another one writes in the same launch, so no grid
-// synchronisation is needed.
+// Panel kernels. Rows k0..m-1 of the panel are cut into chunks of PCHUNK rows
+// and the panel columns into groups of CW; a launch has one workgroup per
+// (row chunk, column group), so every column step runs on thousands of
+// workgroups with eight independent columns per thread. Per column k:
+//   panel_update: apply reflector k-1 to columns k..k1-1 (w_j reduced from the
+//     dot partials, every workgroup in the same order), store v_{k-1}, and
+//     produce the per-chunk (max, scaled sum of squares) of column k below the
+//     pivot; the owner of row k also saves the pivot x0.
+//   panel_reflect: every workgroup combines the norm partials in the same
+//     order (so all agree on beta and tau), forms its part of v into a side
+//     buffer, and emits per-chunk dot products v . a_j for j = k+1..k1-1; the
+//     owner of row k writes R(k,k) = beta and tau[k].
+// No workgroup reads data another one writes in the same launch (v travels
+// through the side buffer and reaches the matrix one launch later), so no
+// grid synchronisation is needed.
 #include <math.h>
 
 #include "internal.h"
@@ -68,105 +71,138 @@ __device__ double block_max(double v, double* red) {
 
 // Norm partial layout: part[b*2 + 0] = chunk max |x|, part[b*2 + 1] = sum (x/max)^2;
 // part[2*G] = pivot x0 of the current column.
-__global__ __launch_bounds__(PTHREADS) void panel_update_norm(double* __restrict__ at, int64_t lda, int m, int k0,
-                                                              int k1, int k, const double* __restrict__ tau,
-                                                              const double* __restrict__ dots, int G,
-                                                              double* __restrict__ part) {
-    __shared__ double red[4];
-    __shared__ double w_s[128];
-    const int b = blockIdx.x;
-    const int row = k0 + b * PCHUNK + threadIdx.x;
-    // 1. apply reflector k-1 (if any) to columns k..k1-1 on rows >= k-1
-    if (k > k0) {
-        const int kp = k - 1;
-        const double t = tau[kp];
-        if (t != 0.0) {
-            for (int j = k + threadIdx.x; j < k1; j += PTHREADS) {
-                double s = 0.0;
-                for (int q = 0; q < G; q++) s += dots[(int64_t)q * 128 + (j - k0)];
-                w_s[j - k0 - (k - k0)] = s;
-            }
-            __syncthreads();
-            if (row >= kp && row < m) {
-                const double v = row == kp ? 1.0 : at[(int64_t)kp * lda + row];
-                for (int j = k; j < k1; j++) at[(int64_t)j * lda + row] -= t * v * w_s[j - k];
-            }
-        }
-    }
-    // 2. norm partial of column k below the pivot
-    const bool live = row > k && row < m;
-    const double x = live ? at[(int64_t)k * lda + row] : 0.0;
-    const double mx = block_max(fabs(x), red);
-    double sq = 0.0;
-    if (mx > 0.0) {
-        const double y = x / mx;
-        sq = y * y;
-    }
-    sq = block_sum(sq, red);
-    if (threadIdx.x == 0) {
-        part[2 * b] = mx;
-        part[2 * b + 1] = sq;
-    }
-    if (row == k) part[2 * G] = at[(int64_t)k * lda + k];
-}
+// Grid of both panel kernels: x = row chunk b (PCHUNK rows from k0), y = group of CW
+// consecutive columns starting at the first column the launch works on.
+constexpr int CW = 8;
 
-__global__ __launch_bounds__(PTHREADS) void panel_reflect_dots(double* __restrict__ at, int64_t lda, int m, int k0,
-                                                               int k1, int k, double* __restrict__ tau, int G,
-                                                               const double* __restrict__ part,
-                                                               double* __restrict__ dots) {
-    __shared__ double red[4];
-    __shared__ double v_s[PCHUNK];
-    const int b = blockIdx.x;
-    const int row = k0 + b * PCHUNK + threadIdx.x;
-    // Combine the norm partials in a fixed order: every workgroup gets the same bits.
+// Reflector scalars of column k from the norm partials, combined in a fixed
+// order so every workgroup gets the same bits (computeHouseholder, :219-291).
+struct Reflector {
+    double tau, beta, v0inv;
+};
+// The partials are combined by the whole workgroup (each thread takes every
+// 256th chunk, then the fixed block reduction tree), so every workgroup of the
+// launch computes the same bits.
+__device__ Reflector reflector_from(const double* __restrict__ part, int G, int k, int m, double* red) {
     const double x0 = part[2 * G];
-    double M = fabs(x0);
-    for (int q = 0; q < G; q++) M = fmax(M, part[2 * q]);
-    double t = 0.0, beta = x0, v0inv = 0.0;
-    bool negate = false;
+    double mloc = 0.0;
+    for (int q = threadIdx.x; q < G; q += PTHREADS) mloc = fmax(mloc, part[2 * q]);
+    const double M = fmax(fabs(x0), block_max(mloc, red));
+    Reflector r{0.0, x0, 0.0};
     if (k < m - 1 && M >= 2.2250738585072014e-308) {
-        double sigma = 0.0;
-        for (int q = 0; q < G; q++) {
-            const double r = part[2 * q] / M;
-            sigma += part[2 * q + 1] * r * r;
+        double sloc = 0.0;
+        for (int q = threadIdx.x; q < G; q += PTHREADS) {
+            const double s = part[2 * q] / M;
+            sloc += part[2 * q + 1] * s * s;
         }
+        const double sigma = block_sum(sloc, red);
         const double xnorm = sqrt(sigma);
         const double x0n = x0 / M;
         if (xnorm == 0.0) {
             if (x0n < 0.0) {
-                t = 2.0;
-                beta = -x0;
-                negate = true;
+                r.tau = 2.0;
+                r.beta = -x0;
             }
         } else {
             const double nbeta = -copysign(hypot(x0n, xnorm), x0n);
-            beta = nbeta * M;
-            t = (nbeta - x0n) / nbeta;
-            v0inv = 1.0 / ((x0n - nbeta) * M);
+            r.beta = nbeta * M;
+            r.tau = (nbeta - x0n) / nbeta;
+            r.v0inv = 1.0 / ((x0n - nbeta) * M);
         }
     }
-    (void)negate;
-    // scale v on this chunk (rows > k) and stage it
+    return r;
+}
+
+// Column k's step, first half. With k > k0: apply reflector k-1 (v from vtmp,
+// tau[k-1], w_j = sum over row chunks of dots in a fixed order) to columns
+// k + CW*y .. ; the y == 0 groups also store v_{k-1} into the matrix (nobody
+// reads column k-1 in this launch) and, for every k, emit the norm partials of
+// column k (its first column) after the update.
+__global__ __launch_bounds__(PTHREADS) void panel_update(double* __restrict__ at, int64_t lda, int m, int k0, int k1,
+                                                         int k, const double* __restrict__ tau,
+                                                         const double* __restrict__ vtmp,
+                                                         const double* __restrict__ dots, int G,
+                                                         double* __restrict__ part) {
+    __shared__ double red[4];
+    __shared__ double w_s[CW];
+    const int b = blockIdx.x;
+    const int row = k0 + b * PCHUNK + threadIdx.x;
+    const int j0 = k + CW * blockIdx.y;
+    const int jn = min(CW, k1 - j0);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (k > k0) {
+        const int kp = k - 1;
+        const double t = tau[kp];
+        // w_j: wave wv reduces columns wv and wv + 4 of the group, lanes over chunks, fixed tree
+        for (int c = wv; c < CW; c += 4) {
+            double s = 0.0;
+            if (c < jn)
+                for (int q = lane; q < G; q += 64) s += dots[(int64_t)q * 128 + (j0 + c - k0)];
+            s = wave_sum(s);
+            if (lane == 0) w_s[c] = s;
+        }
+        __syncthreads();
+        if (t != 0.0 && row >= kp && row < m) {
+            const double v = row == kp ? 1.0 : vtmp[row];
+            const double tv = t * v;
+#pragma unroll
+            for (int c = 0; c < CW; c++)
+                if (c < jn) at[(int64_t)(j0 + c) * lda + row] -= tv * w_s[c];
+        }
+        if (blockIdx.y == 0 && row > kp && row < m) at[(int64_t)kp * lda + row] = vtmp[row];
+    }
+    if (blockIdx.y == 0) {
+        const bool live = row > k && row < m;
+        const double x = live ? at[(int64_t)k * lda + row] : 0.0;
+        const double mx = block_max(fabs(x), red);
+        double sq = 0.0;
+        if (mx > 0.0) {
+            const double y = x / mx;
+            sq = y * y;
+        }
+        sq = block_sum(sq, red);
+        if (threadIdx.x == 0) {
+            part[2 * b] = mx;
+            part[2 * b + 1] = sq;
+        }
+        if (row == k) part[2 * G] = at[(int64_t)k * lda + k];
+    }
+}
+
+// Column k's step, second half: every workgroup derives the same reflector from
+// the partials; v = x * v0inv below the pivot (1 at the pivot) is computed from
+// the unscaled column, which nobody writes in this launch. The y == 0 groups
+// store v into vtmp; the owner of row k writes R(k,k) = beta and tau[k]; every
+// group emits the per-chunk dot products v . a_j for its columns j > k.
+__global__ __launch_bounds__(PTHREADS) void panel_reflect(double* __restrict__ at, int64_t lda, int m, int k0,
+                                                          int k1, int k, double* __restrict__ tau, int G,
+                                                          const double* __restrict__ part,
+                                                          double* __restrict__ vtmp, double* __restrict__ dots) {
+    __shared__ double v_s[PCHUNK];
+    __shared__ double red[4];
+    const int b = blockIdx.x;
+    const int row = k0 + b * PCHUNK + threadIdx.x;
+    const Reflector rf = reflector_from(part, G, k, m, red);
     double v = 0.0;
     if (row > k && row < m) {
         v = at[(int64_t)k * lda + row];
-        if (v0inv != 0.0) {
-            v *= v0inv;
-            at[(int64_t)k * lda + row] = v;
-        }
+        if (rf.v0inv != 0.0) v *= rf.v0inv;
+        if (blockIdx.y == 0) vtmp[row] = v;
     } else if (row == k) {
         v = 1.0;
     }
     v_s[threadIdx.x] = (row >= k && row < m) ? v : 0.0;
-    if (row == k) {
-        at[(int64_t)k * lda + k] = beta;
-        tau[k] = t;
+    if (row == k && blockIdx.y == 0) {
+        at[(int64_t)k * lda + k] = rf.beta;
+        tau[k] = rf.tau;
     }
     __syncthreads();
-    // dot partials for columns k+1..k1-1: each wavefront takes every 4th column
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j0 = k + 1 + CW * blockIdx.y;
     const int rbase = k0 + b * PCHUNK;
-    for (int j = k + 1 + w; j < k1; j += 4) {
+    for (int c = wv; c < CW; c += 4) {
+        const int j = j0 + c;
+        if (j >= k1) break;
         double s = 0.0;
         for (int i = lane; i < PCHUNK; i += 64) {
             const int r = rbase + i;
@@ -175,6 +211,13 @@ __global__ __launch_bounds__(PTHREADS) void panel_reflect_dots(double* __restric
         s = wave_sum(s);
         if (lane == 0) dots[(int64_t)b * 128 + (j - k0)] = s;
     }
+}
+
+// After the panel's last column: store its v (rows below the pivot) from vtmp.
+__global__ __launch_bounds__(256) void panel_store_v(double* __restrict__ at, int64_t lda, int m, int k,
+                                                     const double* __restrict__ vtmp) {
+    const int row = k + 1 + blockIdx.x * 256 + threadIdx.x;
+    if (row < m) at[(int64_t)k * lda + row] = vtmp[row];
 }
 
 // Apply the last reflector of a panel (column k1-1) has no panel columns left; nothing to do.
@@ -195,22 +238,37 @@ __global__ __launch_bounds__(256) void build_vc(const double* __restrict__ at, i
 
 // T (nb x nb row-major, upper triangular) from G = V^T V and tau:
 // T(i,i) = tau_i, T(0:i, i) = T(0:i, 0:i) * (-tau_i * G(0:i, i)) (buildT, HouseholderQR.java:296-320).
+// T is built in one nb x nb LDS array (up to 128 KiB): its upper triangle is
+// T, its diagonal tau, and its strict lower triangle holds y_i = -tau_i G(0:i, i)
+// transposed (row i, columns c < i), all loaded up front. Column i's entries
+// T(r, i), r < i, are row-times-vector products over c in [r, i): two threads
+// per row take alternate c from LDS and one lane swap adds the halves; one
+// barrier per column.
 __global__ __launch_bounds__(256) void build_t(const double* __restrict__ g, int nb, const double* __restrict__ tau,
                                                double* __restrict__ T) {
-    extern __shared__ double tmp[];
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) T[idx] = 0.0;
+    extern __shared__ double S[];  // nb * nb
+    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+        const int r = idx / nb, c = idx % nb;  // g row r, column c: y_c[r] for r < c
+        if (r < c) S[c * nb + r] = -tau[c] * g[idx];
+        else if (r == c) S[idx] = tau[r];
+    }
     __syncthreads();
-    for (int i = 0; i < nb; i++) {
-        const double ti = tau[i];
-        for (int r = threadIdx.x; r < i; r += blockDim.x) tmp[r] = -ti * g[r * nb + i];
-        __syncthreads();
-        for (int r = threadIdx.x; r < i; r += blockDim.x) {
-            double s = 0.0;
-            for (int c = r; c < i; c++) s += T[r * nb + c] * tmp[c];
-            T[r * nb + i] = s;
+    const int r = threadIdx.x >> 1, h = threadIdx.x & 1;  // nb <= 128: one thread pair per row
+    for (int i = 1; i < nb; i++) {
+        double s = 0.0;
+        if (r < i) {
+            const double* tr = S + r * nb;
+            const double* yi = S + i * nb;
+#pragma unroll 4
+            for (int c = r + h; c < i; c += 2) s += tr[c] * yi[c];
         }
-        if (threadIdx.x == 0) T[i * nb + i] = ti;
+        s += __shfl_xor(s, 1, 64);
+        if (r < i && h == 0) S[r * nb + i] = s;
         __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+        const int r = idx / nb, c = idx % nb;
+        T[idx] = r <= c ? S[idx] : 0.0;
     }
 }
 
@@ -332,14 +390,20 @@ __global__ __launch_bounds__(256) void qr_batched_small(const QrBatchDesc* __res
 int qr_panel_chunk() { return PCHUNK; }
 
 hipError_t launch_qr_panel(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part, double* dots,
-                           hipStream_t stream) {
+                           double* vtmp, hipStream_t stream) {
     const int G = (m - k0 + PCHUNK - 1) / PCHUNK;
     for (int k = k0; k < k1; k++) {
-        hipLaunchKernelGGL(panel_update_norm, dim3(G), dim3(PTHREADS), 0, stream, at, lda, m, k0, k1, k, tau, dots, G,
-                           part);
-        hipLaunchKernelGGL(panel_reflect_dots, dim3(G), dim3(PTHREADS), 0, stream, at, lda, m, k0, k1, k, tau, G,
-                           part, dots);
+        const int gu = k > k0 ? (k1 - k + CW - 1) / CW : 1;
+        hipLaunchKernelGGL(panel_update, dim3(G, gu), dim3(PTHREADS), 0, stream, at, lda, m, k0, k1, k, tau, vtmp,
+                           dots, G, part);
+        const int gr = k + 1 < k1 ? (k1 - k - 1 + CW - 1) / CW : 1;
+        hipLaunchKernelGGL(panel_reflect, dim3(G, gr), dim3(PTHREADS), 0, stream, at, lda, m, k0, k1, k, tau, G,
+                           part, vtmp, dots);
     }
+    const int last = k1 - 1;
+    if (last + 1 < m)
+        hipLaunchKernelGGL(panel_store_v, dim3((m - last - 1 + 255) / 256), dim3(256), 0, stream, at, lda, m, last,
+                           vtmp);
     return hipGetLastError();
 }
 
@@ -350,7 +414,9 @@ hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb,
 }
 
 hipError_t launch_build_t(const double* g, int nb, const double* tau, double* T, hipStream_t stream) {
-    hipLaunchKernelGGL(build_t, dim3(1), dim3(256), nb * sizeof(double), stream, g, nb, tau, T);
+    const size_t lds = (size_t)nb * nb * sizeof(double);
+    (void)hipFuncSetAttribute((const void*)build_t, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(build_t, dim3(1), dim3(256), lds, stream, g, nb, tau, T);
     return hipGetLastError();
 }
 

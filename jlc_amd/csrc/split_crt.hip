@@ -70,9 +70,10 @@ __device__ __forceinline__ v4i residues16(const int (&c2)[16], const int (&c1)[1
     unsigned char q[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-        const int x = c2[j] * k44 + c1[j] * k22 + c0[j];
+        // Every factor fits 24 signed bits (|c| < 2^22, |k| <= 128, |qq| < 2^23, m <= 256): full-rate multiplies.
+        const int x = __mul24(c2[j], k44) + __mul24(c1[j], k22) + c0[j];
         const int qq = (int)rintf((float)x * invf);
-        int r = x - qq * m;
+        int r = x - __mul24(qq, m);
         r = r > hl ? r - m : r;
         r = r < ll ? r + m : r;
         r = r > hl ? r - m : r;
@@ -90,6 +91,9 @@ __device__ __forceinline__ v4i residues16(const int (&c2)[16], const int (&c1)[1
 // Residue planes of op(X) (element (r, k) at X[r*sr + k*sk], R x Kd, zero
 // padded): one wavefront per 1 KiB block; lane l handles row l%32,
 // k in [16*(l/32), +16) and writes 16 bytes per modulus.
+// VEC: k-contiguous rows (sk == 1), 16-byte aligned, Kd a multiple of 16:
+// each lane reads its 16 doubles as eight 16-byte loads.
+template <bool VEC>
 __global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, int64_t sr, int64_t sk, int R, int Kd,
                                                 int kchunks, int nblocks, const int* __restrict__ e, int p,
                                                 CrtTables T, v4i* __restrict__ out, int64_t plane_stride) {
@@ -101,15 +105,32 @@ __global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, in
     const int k0 = kc * 32 + 16 * (lane >> 5);
     const int ex = r < R ? e[r] : 0;
     const bool live = r < R && ex != NONFINITE;
+    // Loads use clamped, always-valid indices and the padding is selected to
+    // zero afterwards, so the element loop has no branches.
+    const int64_t rowoff = (int64_t)(r < R ? r : R - 1) * sr;
+    double xv[16];
+    if (VEC) {
+        const int kb = k0 < Kd ? k0 : Kd - 16;
+#pragma unroll
+        for (int j = 0; j < 16; j += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(X + rowoff + kb + j);
+            xv[j] = v.x;
+            xv[j + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int k = k0 + j;
+            xv[j] = X[rowoff + (int64_t)(k < Kd ? k : Kd - 1) * sk];
+        }
+    }
     int c2[16], c1[16], c0[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int k = k0 + j;
-        double s = 0.0;
-        if (live && k < Kd) {
-            s = ldexp(X[(int64_t)r * sr + (int64_t)k * sk], p - ex);
-            s = fabs(s) < 9007199254740992.0 ? rint(s) : s;
-        }
+        double s = ldexp(xv[j], p - ex);
+        s = fabs(s) < 9007199254740992.0 ? rint(s) : s;
+        s = (live && k < Kd) ? s : 0.0;
         // s = c2 * 2^44 + c1 * 2^22 + c0 exactly; every chunk has the sign of s and |chunk| < 2^22.
         const double h = trunc(s * 0x1p-44);
         const double rest = fma(-h, 0x1p44, s);
@@ -264,15 +285,18 @@ __device__ __forceinline__ double crt_value(const int (&r)[NM], const CrtTables&
     double X[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) X[j] = 0.0;
+    // Every product and partial sum here is an integer below 2^53, so the fused
+    // form is exact and equal to the separate one.
 #pragma unroll
     for (int t = 0; t < NM; t++) {
         const double rt = (double)r[t];
 #pragma unroll
-        for (int j = 0; j < 4; j++) X[j] = X[j] + rt * T.w[t][j];
+        for (int j = 0; j < 4; j++) X[j] = fma(rt, T.w[t][j], X[j]);
     }
+    // X/M lies within 1/4 of the integer q, so the rounding of this estimate cannot change q.
     double qf = 0.0;
 #pragma unroll
-    for (int j = 3; j >= 0; j--) qf = qf + X[j] * T.finv[j];
+    for (int j = 3; j >= 0; j--) qf = fma(X[j], T.finv[j], qf);
     const double q = rint(qf);
     double D[4];
 #pragma unroll
@@ -438,13 +462,20 @@ hipError_t launch_crt(int layout_a, int layout_b, const double* a, int64_t lda, 
     v4i* ap = static_cast<v4i*>(p.a_pack);
     v4i* bp = static_cast<v4i*>(p.b_pack);
     v4i* cr = static_cast<v4i*>(p.c_res);
-    hipLaunchKernelGGL(crt_pack, dim3((ablocks + 3) / 4), dim3(256), 0, stream, a, asr, ask, M, K, kchunks, ablocks,
-                       p.ea, p.pa, T, ap, p.a_plane_bytes / 16);
-    hipError_t err = hipGetLastError();
+    auto pack = [&](const double* x, int64_t sr, int64_t sk, int R, int blocks, const int* e, int bits, v4i* o,
+                    int64_t plane) {
+        const bool vec = sk == 1 && K % 16 == 0 && sr % 2 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0;
+        if (vec)
+            hipLaunchKernelGGL(crt_pack<true>, dim3((blocks + 3) / 4), dim3(256), 0, stream, x, sr, sk, R, K,
+                               kchunks, blocks, e, bits, T, o, plane);
+        else
+            hipLaunchKernelGGL(crt_pack<false>, dim3((blocks + 3) / 4), dim3(256), 0, stream, x, sr, sk, R, K,
+                               kchunks, blocks, e, bits, T, o, plane);
+        return hipGetLastError();
+    };
+    hipError_t err = pack(a, asr, ask, M, ablocks, p.ea, p.pa, ap, p.a_plane_bytes / 16);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(crt_pack, dim3((bblocks + 3) / 4), dim3(256), 0, stream, b, bsr, bsk, N, K, kchunks, bblocks,
-                       p.eb, p.pb, T, bp, p.b_plane_bytes / 16);
-    if ((err = hipGetLastError()) != hipSuccess) return err;
+    if ((err = pack(b, bsr, bsk, N, bblocks, p.eb, p.pb, bp, p.b_plane_bytes / 16)) != hipSuccess) return err;
     const int tiles_m = (int)(p.mpad / CT), tiles_n = (int)(p.npad / CT);
     const int cblocks_n = (int)(p.npad / 32);
     const size_t lds = (size_t)NBUF_LDS * STAGE_V4 * sizeof(v4i);
