@@ -284,7 +284,8 @@ def bench_qr(args) -> int:
     work = native.DeviceArray(m * n)
     tau = native.DeviceArray(n)
     forms = {}
-    for name, mode in (("fp64", native.MODE_FP64), ("split_int8", native.MODE_SPLIT_INT8)):
+    for name, mode in (("fp64", native.MODE_FP64), ("split_int8", native.MODE_SPLIT_INT8),
+                       ("split_crt", native.MODE_SPLIT_CRT)):
         times = []
         for it in range(args.warmup + max(1, args.steps)):
             native.dev_copy(work.ptr, A.ptr, m * n * 8, stream)

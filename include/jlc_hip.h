@@ -157,7 +157,8 @@ int jlc_dgemm_grouped_dev(int64_t count, const jlc_gemm_desc* descs, int mode, v
  * (v0 = 1 implicit) below it, tau[min(m,n)] their scalars, as computeHouseholder
  * (:219-291) stores them. Panels of nb <= 128 columns are factored in FP64 on
  * the GPU; the compact-WY trailing updates (:167-209) run on the GEMM entry
- * points in `mode` (JLC_MODE_FP64 or JLC_MODE_SPLIT_INT8[, slices]); T is FP64. */
+ * points in `mode` (JLC_MODE_FP64, JLC_MODE_SPLIT_INT8[, slices] or
+ * JLC_MODE_SPLIT_CRT[, moduli]); T is FP64. */
 int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, int64_t nb, int mode, void* stream,
                    jlc_stats* stats);
 /* Thin Q (column-major m x n, ldq) from the factors (buildQ, HouseholderQR.java:390-444), n <= m. */

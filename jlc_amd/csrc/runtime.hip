@@ -1064,8 +1064,8 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     if (m < 0 || n < 0 || m > INT32_MAX || n > INT32_MAX) return fail(JLC_ERR_INVALID_ARG, "bad dimensions");
     if (nb < 1 || nb > 128) return fail(JLC_ERR_INVALID_ARG, "panel width nb must be in [1, 128]");
     const int form = mode & 0xff;
-    if (form != JLC_MODE_FP64 && form != JLC_MODE_SPLIT_INT8)
-        return fail(JLC_ERR_INVALID_ARG, "trailing-update mode must be FP64 or SPLIT_INT8");
+    if (form != JLC_MODE_FP64 && form != JLC_MODE_SPLIT_INT8 && form != JLC_MODE_SPLIT_CRT)
+        return fail(JLC_ERR_INVALID_ARG, "trailing-update mode must be FP64, SPLIT_INT8 or SPLIT_CRT");
     const int64_t kmax = m < n ? m : n;
     if (kmax == 0) return JLC_OK;
     if (!a || !tau || lda < m) return fail(JLC_ERR_INVALID_ARG, "a/tau must not be null and lda >= m");

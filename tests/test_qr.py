@@ -65,7 +65,7 @@ def test_graded_spectrum_split_trailing_updates():
     s = np.geomspace(1.0, 1e-8, n)
     A = (U * s) @ V.T
     res = {}
-    for name, mode in (("fp64", native.MODE_FP64), ("split", native.MODE_SPLIT_INT8)):
+    for name, mode in (("fp64", native.MODE_FP64), ("split", native.MODE_SPLIT_INT8), ("crt", native.MODE_SPLIT_CRT)):
         da, d0, dt, _ = _factor(A, nb=128, mode=mode)
         res[name] = _errors(A, da, d0, dt, 128)
     print(res, flush=True)
