@@ -361,13 +361,21 @@ __global__ __launch_bounds__(256) void qr_batched_small(const QrBatchDesc* __res
             d.tau[k] = t;
         }
         if (t != 0.0) {
-            // w_j = a(k,j) + sum_{i>k} v_i a(i,j); a(:,j) -= t * v * w_j
-            for (int j = k + 1 + w; j < n; j += 4) {
-                const double* cj = a_s + (int64_t)j * m;
+            // w_j = a(k,j) + sum_{i>k} v_i a(i,j); a(:,j) -= t * v * w_j.
+            // Eight threads per column (32 columns at a time) take interleaved rows
+            // and three lane swaps add their partial sums.
+            const int sub = threadIdx.x & 7, slot = threadIdx.x >> 3;
+            for (int jb = k + 1; jb < n; jb += 32) {
+                const int j = jb + slot;
                 double s = 0.0;
-                for (int i = k + 1 + lane; i < m; i += 64) s += col[i] * cj[i];
-                s = wave_sum(s);
-                if (lane == 0) wcol[j - k - 1] = s + cj[k];
+                if (j < n) {
+                    const double* cj = a_s + (int64_t)j * m;
+                    for (int i = k + 1 + sub; i < m; i += 8) s += col[i] * cj[i];
+                }
+                s += __shfl_xor(s, 1, 64);
+                s += __shfl_xor(s, 2, 64);
+                s += __shfl_xor(s, 4, 64);
+                if (j < n && sub == 0) wcol[j - k - 1] = s + a_s[(int64_t)j * m + k];
             }
             __syncthreads();
             for (int j = k + 1 + w; j < n; j += 4) {

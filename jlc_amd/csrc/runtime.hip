@@ -1214,13 +1214,32 @@ int jlc_dqr_errors_dev(int64_t m, int64_t n, const double* a_orig, int64_t lda_o
 
 static_assert(sizeof(jlc_qr_desc) == sizeof(jlc::QrBatchDesc), "QR descriptor layouts must match");
 
+// Matrices are grouped into size classes by the LDS their tile needs, one
+// launch per class, so small matrices run several workgroups per CU instead of
+// being held to the occupancy of the largest matrix of the batch.
 struct jlc_qr_batched_plan {
+    static constexpr int NCLASS = 4;
     int count = 0;
     int max_mn = 0;
+    int cls_off[NCLASS] = {}, cls_cnt[NCLASS] = {}, cls_mn[NCLASS] = {};
     jlc::QrBatchDesc* d_descs = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+
+namespace {
+// LDS entries (doubles) of each class: 8, 4, 2 and 1 workgroups per CU.
+constexpr int QR_CLASS_MN[jlc_qr_batched_plan::NCLASS] = {2304, 4864, 9984, 20480};
+
+hipError_t launch_qr_plan(const jlc_qr_batched_plan* p, hipStream_t s) {
+    for (int c = 0; c < jlc_qr_batched_plan::NCLASS; c++) {
+        if (!p->cls_cnt[c]) continue;
+        hipError_t e = jlc::launch_qr_batched_small(p->d_descs + p->cls_off[c], p->cls_cnt[c], p->cls_mn[c], s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+}  // namespace
 
 int jlc_qr_batched_plan_create(int64_t count, const jlc_qr_desc* descs, jlc_qr_batched_plan** plan) {
     if (!plan) return fail(JLC_ERR_INVALID_ARG, "plan must not be null");
@@ -1241,9 +1260,27 @@ int jlc_qr_batched_plan_create(int64_t count, const jlc_qr_desc* descs, jlc_qr_b
     jlc_qr_batched_plan* p = new jlc_qr_batched_plan;
     p->count = (int)count;
     p->max_mn = max_mn;
+    // Stable order by size class; each class launches with its own largest tile.
+    std::vector<jlc_qr_desc> sorted;
+    sorted.reserve((size_t)count);
+    for (int c = 0, off = 0; c < jlc_qr_batched_plan::NCLASS; c++) {
+        const int lo = c ? QR_CLASS_MN[c - 1] : 0, hi = QR_CLASS_MN[c];
+        int mx = 0;
+        for (int64_t i = 0; i < count; i++) {
+            const int mn = descs[i].m * descs[i].n;
+            if (mn > lo && mn <= hi) {
+                sorted.push_back(descs[i]);
+                mx = std::max(mx, mn);
+            }
+        }
+        p->cls_off[c] = off;
+        p->cls_cnt[c] = (int)sorted.size() - off;
+        p->cls_mn[c] = mx;
+        off = (int)sorted.size();
+    }
     hipError_t e = hipMalloc(&p->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::QrBatchDesc));
     if (e == hipSuccess && count)
-        e = hipMemcpy(p->d_descs, descs, (size_t)count * sizeof(jlc::QrBatchDesc), hipMemcpyHostToDevice);
+        e = hipMemcpy(p->d_descs, sorted.data(), (size_t)count * sizeof(jlc::QrBatchDesc), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         jlc_qr_batched_plan_destroy(p);
         return fail(JLC_ERR_ALLOC, "QR plan upload failed: %s", hipGetErrorString(e));
@@ -1263,7 +1300,7 @@ int jlc_qr_batched_plan_replay(jlc_qr_batched_plan* plan, void* stream, jlc_stat
         hipStream_t cap;
         JLC_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) e = jlc::launch_qr_batched_small(plan->d_descs, plan->count, plan->max_mn, cap);
+        if (e == hipSuccess) e = launch_qr_plan(plan, cap);
         hipGraph_t graph = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &graph);
         if (e == hipSuccess) e = e2;
@@ -1312,7 +1349,7 @@ int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream
         if (e != hipSuccess) rc = hip_fail(e, "hipEventRecord");
     }
     if (rc == JLC_OK && count) {
-        hipError_t e = jlc::launch_qr_batched_small(plan->d_descs, plan->count, plan->max_mn, s);
+        hipError_t e = launch_qr_plan(plan, s);
         if (e != hipSuccess) rc = hip_fail(e, "batched QR launch");
     }
     if (rc == JLC_OK) {
