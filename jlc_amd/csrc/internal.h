@@ -33,7 +33,8 @@ struct TuneTable {
     int splitk_tiles_per_cu = 2;      // split K until slices * tiles >= this * CUs (0 = never split)
     int splitk_min_kslice = 1024;     // smallest K per slice
     int grouped_waves_per_simd = 0;   // persistent grouped grid (0 = occupancy limit)
-    double crt_bits_offset = 48.0;    // CRT form: bits of a row = ceil(offset + row spread)
+    double crt_bits_offset = 52.0;    // CRT form: bits of a row = ceil(offset + slope * row spread)
+    double crt_bits_slope = 0.35;
 };
 TuneTable tune_snapshot();
 

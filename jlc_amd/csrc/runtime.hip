@@ -246,15 +246,17 @@ int choose_slices(const float* spread_a, int64_t m, const float* spread_b, int64
     double bits = 53.0 + sa + sb - 4.0;
     int s = (int)std::ceil(bits / digit_bits);
     if (s < jlc::SPLIT_MIN_SLICES) s = jlc::SPLIT_MIN_SLICES;
-    if (s > max_slices) s = max_slices;
-    return s;
+    return s > max_slices ? 0 : s;  // 0: more slices than the form has
 }
 
 int ensure_buf(int slot, size_t bytes);
 
+// Runs a slice form; in automatic mode *fallback is set (nothing launched
+// beyond the statistics) when the spread needs more slices than the form has.
 int run_split(bool bf16, int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
               int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int slices,
-              hipStream_t stream, jlc::TileInfo* info) {
+              hipStream_t stream, jlc::TileInfo* info, bool* fallback) {
+    *fallback = false;
     ThreadCtx& ctx = t_ctx;
     jlc::SplitPlan p;
     const int max_slices = bf16 ? jlc::SPLIT_BF16_MAX_SLICES : jlc::SPLIT_MAX_SLICES;
@@ -280,6 +282,10 @@ int run_split(bool bf16, int la, int lb, int64_t m, int64_t n, int64_t k, double
                                      stream));
         JLC_HIP_CHECK(hipStreamSynchronize(stream));
         slices = choose_slices(host.data(), m, host.data() + p.mpad, n, bf16 ? 8 : 7, max_slices);
+        if (slices == 0) {
+            *fallback = true;
+            return JLC_OK;
+        }
         p.slices = slices;
     }
     if ((rc = ensure_buf(4, (size_t)(p.slices * p.a_slice_bytes))) != JLC_OK) return rc;
@@ -296,12 +302,15 @@ int run_split(bool bf16, int la, int lb, int64_t m, int64_t n, int64_t k, double
 
 // Integer bits per operand and moduli count of the CRT form. A row with
 // spread s (bits between its scale 2^e and its geometric-mean |x|) gets
-// ceil(offset + s) bits, so its entries near the mean keep about offset bits;
-// the moduli must cover pa + pb + ceil(log2 k) + 2 bits (|C'| < M/4).
+// ceil(offset + slope*s) bits: wider rows lose more bits to the shared scale,
+// but the FP64 product the form is matched against also errs more on them
+// (its componentwise error grows from 3.4 u at spread 5 to 114 u at spread 15
+// on the benchmark distribution), so the slope is well below one. The moduli
+// must cover pa + pb + ceil(log2 k) + 2 bits (|C'| < M/4).
 int crt_choose(double spread_a, double spread_b, int64_t k, int* pa, int* pb) {
-    const double off = jlc::tune_snapshot().crt_bits_offset;
-    *pa = std::max(2, (int)std::ceil(off + spread_a));
-    *pb = std::max(2, (int)std::ceil(off + spread_b));
+    const jlc::TuneTable tt = jlc::tune_snapshot();
+    *pa = std::max(2, (int)std::ceil(tt.crt_bits_offset + tt.crt_bits_slope * spread_a));
+    *pb = std::max(2, (int)std::ceil(tt.crt_bits_offset + tt.crt_bits_slope * spread_b));
     const int lk = k > 1 ? (int)std::ceil(std::log2((double)k)) : 0;
     if (*pa > jlc::CRT_MAX_BITS || *pb > jlc::CRT_MAX_BITS) return 0;
     const double need = *pa + *pb + lk + 2;
@@ -359,6 +368,11 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
         pb = std::min(avail - pa, jlc::CRT_MAX_BITS);
     }
     jlc::crt_plan_sizes(m, n, k, nmod, &p);
+    // Launch geometry is 32-bit: 1 KiB fragment blocks per plane of either operand and of C.
+    if ((p.mpad / 32) * (p.kpad / 32) > INT32_MAX || (p.npad / 32) * (p.kpad / 32) > INT32_MAX ||
+        (p.mpad / 32) * (p.npad / 32) * 64 > INT32_MAX)
+        return fail(JLC_ERR_UNSUPPORTED, "CRT form: %lld x %lld x %lld exceeds the 32-bit block count",
+                    (long long)m, (long long)n, (long long)k);
     p.pa = pa;
     p.pb = pb;
     p.ea = sp.ea;
@@ -395,9 +409,12 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
     }
     if (form == JLC_MODE_SPLIT_INT8 || form == JLC_MODE_SPLIT_BF16) {
         const bool bf16 = form == JLC_MODE_SPLIT_BF16;
+        bool fallback = false;
         *kernel = bf16 ? JLC_KERNEL_SPLIT_BF16 : JLC_KERNEL_SPLIT_INT8;
-        return run_split(bf16, la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, (mode >> 8) & 0xff, stream,
-                         info);
+        int rc = run_split(bf16, la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, (mode >> 8) & 0xff, stream,
+                           info, &fallback);
+        if (rc != JLC_OK || !fallback) return rc;
+        // spread too wide for the form's slices: the FP64 kernel serves the call
     }
     if (n == 1 && la == JLC_ROW) {
         // The n == 1 regime is the matvec (GemmDispatch.selectKernel, GemmDispatch.java:80-82):
@@ -515,6 +532,7 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"splitk_min_kslice", nullptr, &jlc::TuneTable::splitk_min_kslice},
         {"grouped_waves_per_simd", nullptr, &jlc::TuneTable::grouped_waves_per_simd},
         {"crt_bits_offset", &jlc::TuneTable::crt_bits_offset, nullptr},
+        {"crt_bits_slope", &jlc::TuneTable::crt_bits_slope, nullptr},
     };
     for (const F& f : fields) {
         if (std::strcmp(f.name, key) != 0) continue;
@@ -568,6 +586,7 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "splitk_min_kslice %d\n", t.splitk_min_kslice);
     std::fprintf(f, "grouped_waves_per_simd %d\n", t.grouped_waves_per_simd);
     std::fprintf(f, "crt_bits_offset %.17g\n", t.crt_bits_offset);
+    std::fprintf(f, "crt_bits_slope %.17g\n", t.crt_bits_slope);
     std::fclose(f);
     return JLC_OK;
 }

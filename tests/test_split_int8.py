@@ -119,3 +119,25 @@ def test_invalid_slice_counts_rejected():
             native.dgemm(0, 0, 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a.copy(), 2,
                          native.mode_slices(native.MODE_SPLIT_INT8, s))
         assert ei.value.status == native.JLC_ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("form,name", [(native.MODE_SPLIT_INT8, "split_int8"), (native.MODE_SPLIT_BF16, "split_bf16")])
+def test_automatic_slices_beyond_maximum_run_fp64_kernel(form, name):
+    """Rows spanning ~2^60 need more slices than either form has: the automatic
+    mode hands the call to the FP64 kernel, whose result it then equals."""
+    rng = np.random.default_rng(31)
+    m, n, k = 96, 80, 72
+    A = rng.uniform(-1, 1, (m, k)) * np.exp2(rng.integers(-30, 30, (m, k)))
+    B = rng.uniform(-1, 1, (k, n)) * np.exp2(rng.integers(-30, 30, (k, n)))
+    C0 = rng.uniform(-1, 1, m * n)
+    c = C0.copy()
+    st = native.dgemm(0, 0, m, n, k, 1.5, A.ravel().copy(), k, B.ravel().copy(), n, 0.25, c, n, form)
+    assert st.kernel_name == "mfma_f64"
+    c2 = C0.copy()
+    native.dgemm(0, 0, m, n, k, 1.5, A.ravel().copy(), k, B.ravel().copy(), n, 0.25, c2, n, native.MODE_FP64)
+    assert np.array_equal(c, c2)
+    # a narrow spread stays on the slice form
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (k, n))
+    st = native.dgemm(0, 0, m, n, k, 1.0, A.ravel().copy(), k, B.ravel().copy(), n, 0.0, np.zeros(m * n), n, form)
+    assert st.kernel_name == name
