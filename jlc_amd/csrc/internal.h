@@ -33,8 +33,11 @@ struct TuneTable {
     int splitk_tiles_per_cu = 2;      // split K until slices * tiles >= this * CUs (0 = never split)
     int splitk_min_kslice = 1024;     // smallest K per slice
     int grouped_waves_per_simd = 0;   // persistent grouped grid (0 = occupancy limit)
-    double crt_bits_offset = 52.0;    // CRT form: bits of a row = ceil(offset + slope * row spread)
-    double crt_bits_slope = 0.35;
+    double crt_bits_offset = 49.4;    // CRT form: norm bits = ceil(offset + log2(k)/2 + slope * max sig)
+    double crt_bits_slope = 0.43;
+    int crt_waves = 8;                // CRT product kernel: 4 (128 x 128 per wavefront) or 8 (128 x 64)
+    int crt_group = 16;               // CRT product kernel: tile rows per column strip of the tile order
+    int crt_xcd = 0;                  // CRT product kernel: 1 = keep neighbouring tiles on one XCD
 };
 TuneTable tune_snapshot();
 
@@ -134,7 +137,7 @@ hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_
 // CRT split form: N pairwise coprime moduli <= 256, residues in int8.
 constexpr int CRT_MAX_MODULI = 20;
 constexpr int CRT_MIN_MODULI = 2;
-constexpr int CRT_MAX_BITS = 66;  // integer bits per operand: three signed 22-bit chunks
+constexpr int CRT_MAX_BITS = 65;  // norm bits per operand (|a'| < 2^66: three signed 22-bit chunks)
 extern const int CRT_MODULI[CRT_MAX_MODULI];
 
 // Per-modulus constants and the reconstruction tables (passed by value to kernels).
@@ -162,11 +165,16 @@ struct CrtPlan {
     void* a_pack = nullptr;   // nmod planes of op(A) residues (fragment order)
     void* b_pack = nullptr;   // nmod planes of op(B) residues
     void* c_res = nullptr;    // nmod planes of C' residues (accumulator order)
-    int* ea = nullptr;        // mpad row exponents of op(A)
-    int* eb = nullptr;        // npad column exponents of op(B)
+    int* ea = nullptr;        // mpad row norm exponents of op(A)
+    int* eb = nullptr;        // npad column norm exponents of op(B)
 };
 
 double crt_log2_product(int nmod);
+// Norm exponents ta/tb (2^t at or above each row / column 2-norm, from the
+// split statistics' max exponents) and the chooser statistic sig per row /
+// column; uses sp.stats_part as scratch.
+hipError_t launch_crt_norms(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
+                            const SplitPlan& sp, int* ta, int* tb, float* sig_a, float* sig_b, hipStream_t stream);
 void crt_tables(int nmod, CrtTables* T);
 void crt_plan_sizes(int64_t m, int64_t n, int64_t k, int nmod, CrtPlan* p);
 hipError_t launch_crt(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,

@@ -300,22 +300,45 @@ int run_split(bool bf16, int la, int lb, int64_t m, int64_t n, int64_t k, double
     return JLC_OK;
 }
 
-// Integer bits per operand and moduli count of the CRT form. A row with
-// spread s (bits between its scale 2^e and its geometric-mean |x|) gets
-// ceil(offset + slope*s) bits: wider rows lose more bits to the shared scale,
-// but the FP64 product the form is matched against also errs more on them
-// (its componentwise error grows from 3.4 u at spread 5 to 114 u at spread 15
-// on the benchmark distribution), so the slope is well below one. The moduli
-// must cover pa + pb + ceil(log2 k) + 2 bits (|C'| < M/4).
-int crt_choose(double spread_a, double spread_b, int64_t k, int* pa, int* pb) {
+// Whether nmod moduli hold every C' of the CRT form with norm bits (na, nb):
+// |C'_ij| <= ||a'_i|| ||b'_j|| <= (2^na + sqrt(k)/2)(2^nb + sqrt(k)/2) must
+// stay below M/4 (the reconstruction's quotient estimate needs the margin).
+bool crt_fits(int na, int nb, int64_t k, int nmod) {
+    const double h = 0.5 * std::sqrt((double)k);
+    const double need = std::log2(std::ldexp(1.0, na) + h) + std::log2(std::ldexp(1.0, nb) + h) + 2.0;
+    return need <= jlc::crt_log2_product(nmod) - 1e-6;
+}
+
+// Explicit moduli count: the largest even split of the norm bits that fits.
+bool crt_explicit_bits(int nmod, int64_t k, int* na, int* nb) {
+    for (int total = 2 * jlc::CRT_MAX_BITS; total >= 4; total--) {
+        *na = total / 2;
+        *nb = total - *na;
+        if (crt_fits(*na, *nb, k, nmod)) return true;
+    }
+    return false;
+}
+
+// Norm bits per operand and moduli count of the CRT form. Row i is scaled so
+// that ||a'_i|| ~ 2^na; its rounding error relative to the row norm is then
+// ~2^-na and sqrt(k) * 2^-na relative to an rms entry. The FP64 product the
+// form is matched against errs more on wide (heavy-tailed) rows too, so the
+// bits grow slowly with sig = log2(rms / geometric mean):
+//   na = ceil(offset + log2(k)/2 + slope * max_i sig_i),
+// with offset 49.4 and slope 0.43 set from the benchmark distributions
+// (a = (u - 0.5) exp(phi g), k = 16384; max sig ~1.1, ~10 and ~23 at phi =
+// 0.5, 2 and 4): the max relative and componentwise errors stay below the
+// FP64 kernel's at phi = 0.5 (57 bits, 15 moduli) and phi = 1 and 2 (16
+// moduli), and phi = 4 exceeds the 65-bit limit and runs on the FP64 kernel
+// (62 bits had given 30x its componentwise error there).
+int crt_choose(double sig_a, double sig_b, int64_t k, int* na, int* nb) {
     const jlc::TuneTable tt = jlc::tune_snapshot();
-    *pa = std::max(2, (int)std::ceil(tt.crt_bits_offset + tt.crt_bits_slope * spread_a));
-    *pb = std::max(2, (int)std::ceil(tt.crt_bits_offset + tt.crt_bits_slope * spread_b));
-    const int lk = k > 1 ? (int)std::ceil(std::log2((double)k)) : 0;
-    if (*pa > jlc::CRT_MAX_BITS || *pb > jlc::CRT_MAX_BITS) return 0;
-    const double need = *pa + *pb + lk + 2;
+    const double base = tt.crt_bits_offset + 0.5 * std::log2((double)std::max<int64_t>(k, 1));
+    *na = std::max(2, (int)std::ceil(base + tt.crt_bits_slope * sig_a));
+    *nb = std::max(2, (int)std::ceil(base + tt.crt_bits_slope * sig_b));
+    if (*na > jlc::CRT_MAX_BITS || *nb > jlc::CRT_MAX_BITS) return 0;
     for (int nm = jlc::CRT_MIN_MODULI; nm <= jlc::CRT_MAX_MODULI; nm++)
-        if (jlc::crt_log2_product(nm) >= need) return nm;
+        if (crt_fits(*na, *nb, k, nm)) return nm;
     return 0;
 }
 
@@ -326,6 +349,11 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
             jlc::TileInfo* info, bool* fallback) {
     ThreadCtx& ctx = t_ctx;
     *fallback = false;
+    if (k >= (int64_t)1 << 22) {  // the norm sums hold k < 2^22 terms of at most 2^40 + 1 units
+        if (nmod != 0) return fail(JLC_ERR_UNSUPPORTED, "CRT form: k = %lld exceeds 2^22 - 1", (long long)k);
+        *fallback = true;
+        return JLC_OK;
+    }
     jlc::CrtPlan p;
     jlc::crt_plan_sizes(m, n, k, jlc::CRT_MAX_MODULI, &p);
     jlc::SplitPlan sp;
@@ -334,7 +362,8 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
     sp.k = k;
     sp.mpad = p.mpad;
     sp.npad = p.npad;
-    const size_t stat_bytes = ((size_t)(p.mpad + p.npad) * (sizeof(int) + sizeof(float)) + 255) / 256 * 256;
+    // ea, eb, spread_a, spread_b, then the norm exponents ta, tb and chooser statistics sig_a, sig_b
+    const size_t stat_bytes = ((size_t)(p.mpad + p.npad) * 2 * (sizeof(int) + sizeof(float)) + 255) / 256 * 256;
     const size_t part_bytes = (size_t)jlc::split_stats_part_elems(m, n, k) * 4 * sizeof(double);
     int rc = ensure_buf(3, stat_bytes + part_bytes);
     if (rc != JLC_OK) return rc;
@@ -343,13 +372,17 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
     sp.eb = sp.ea + p.mpad;
     sp.spread_a = reinterpret_cast<float*>(sp.eb + p.npad);
     sp.spread_b = sp.spread_a + p.mpad;
+    int* ta = reinterpret_cast<int*>(sp.spread_b + p.npad);
+    int* tb = ta + p.mpad;
+    float* sig_a = reinterpret_cast<float*>(tb + p.npad);
+    float* sig_b = sig_a + p.mpad;
     sp.stats_part = reinterpret_cast<double*>(sb + stat_bytes);
     JLC_HIP_CHECK(jlc::launch_split_stats(la, lb, a, lda, b, ldb, &sp, stream));
+    JLC_HIP_CHECK(jlc::launch_crt_norms(la, lb, a, lda, b, ldb, sp, ta, tb, sig_a, sig_b, stream));
     int pa, pb;
-    const int lk = k > 1 ? (int)std::ceil(std::log2((double)k)) : 0;
     if (nmod == 0) {
         std::vector<float> host((size_t)(p.mpad + p.npad));
-        JLC_HIP_CHECK(hipMemcpyAsync(host.data(), sp.spread_a, host.size() * sizeof(float), hipMemcpyDeviceToHost,
+        JLC_HIP_CHECK(hipMemcpyAsync(host.data(), sig_a, host.size() * sizeof(float), hipMemcpyDeviceToHost,
                                      stream));
         JLC_HIP_CHECK(hipStreamSynchronize(stream));
         float sa = 0.0f, sbm = 0.0f;
@@ -360,12 +393,8 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
             *fallback = true;
             return JLC_OK;
         }
-    } else {
-        const int avail = (int)std::floor(jlc::crt_log2_product(nmod)) - lk - 2;
-        if (avail < 2) return fail(JLC_ERR_INVALID_ARG, "%d moduli leave no integer bits for k=%lld", nmod,
-                                   (long long)k);
-        pa = std::min(avail / 2, jlc::CRT_MAX_BITS);
-        pb = std::min(avail - pa, jlc::CRT_MAX_BITS);
+    } else if (!crt_explicit_bits(nmod, k, &pa, &pb)) {
+        return fail(JLC_ERR_INVALID_ARG, "%d moduli leave no integer bits for k=%lld", nmod, (long long)k);
     }
     jlc::crt_plan_sizes(m, n, k, nmod, &p);
     // Launch geometry is 32-bit: 1 KiB fragment blocks per plane of either operand and of C.
@@ -375,8 +404,8 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
                     (long long)m, (long long)n, (long long)k);
     p.pa = pa;
     p.pb = pb;
-    p.ea = sp.ea;
-    p.eb = sp.eb;
+    p.ea = ta;
+    p.eb = tb;
     if ((rc = ensure_buf(4, (size_t)(nmod * p.a_plane_bytes))) != JLC_OK) return rc;
     if ((rc = ensure_buf(5, (size_t)(nmod * p.b_plane_bytes))) != JLC_OK) return rc;
     if ((rc = ensure_buf(7, (size_t)(nmod * p.c_plane_bytes))) != JLC_OK) return rc;
@@ -533,6 +562,9 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"grouped_waves_per_simd", nullptr, &jlc::TuneTable::grouped_waves_per_simd},
         {"crt_bits_offset", &jlc::TuneTable::crt_bits_offset, nullptr},
         {"crt_bits_slope", &jlc::TuneTable::crt_bits_slope, nullptr},
+        {"crt_waves", nullptr, &jlc::TuneTable::crt_waves},
+        {"crt_group", nullptr, &jlc::TuneTable::crt_group},
+        {"crt_xcd", nullptr, &jlc::TuneTable::crt_xcd},
     };
     for (const F& f : fields) {
         if (std::strcmp(f.name, key) != 0) continue;
@@ -587,6 +619,9 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "grouped_waves_per_simd %d\n", t.grouped_waves_per_simd);
     std::fprintf(f, "crt_bits_offset %.17g\n", t.crt_bits_offset);
     std::fprintf(f, "crt_bits_slope %.17g\n", t.crt_bits_slope);
+    std::fprintf(f, "crt_waves %d\n", t.crt_waves);
+    std::fprintf(f, "crt_group %d\n", t.crt_group);
+    std::fprintf(f, "crt_xcd %d\n", t.crt_xcd);
     std::fclose(f);
     return JLC_OK;
 }

@@ -3,11 +3,14 @@
 // "Ozaki Scheme II: a GEMM-oriented emulation of floating-point matrix
 // multiplication using an integer modular technique").
 //
-//  1. Scaling. Row i of op(A) is mapped to integers a' = rint(a * 2^(pa-ea[i]))
-//     with 2^ea[i] just above the row max |a| (frexp exponent), so |a'| <= 2^pa;
-//     column j of op(B) likewise with pb, eb[j]. Power-of-two scaling is exact,
-//     and the only rounding of the whole method is this one per input entry
-//     (plus the final conversion of the exact integer result to FP64).
+//  1. Scaling. Row i of op(A) is mapped to integers a' = rint(a * 2^(pa-ta[i]))
+//     with 2^ta[i] a power of two at or above the row's 2-norm (crt_norm_*),
+//     so ||a'||_2 <= 2^pa + sqrt(k)/2; column j of op(B) likewise with pb,
+//     tb[j]. Power-of-two scaling is exact, and the only rounding of the whole
+//     method is this one per input entry (plus the final conversion of the
+//     exact integer result to FP64). Scaling by the norm rather than the max
+//     lets the moduli count follow Cauchy-Schwarz, |C'_ij| <= ||a'_i|| ||b'_j||,
+//     instead of the k * 2^(pa+pb) worst case.
 //  2. Residues. For N pairwise coprime moduli m_t <= 256 the symmetric
 //     residues a' mod m_t and b' mod m_t fit in int8. They are written in the
 //     MFMA fragment order of the slice form: one 1 KiB block per (modulus,
@@ -20,13 +23,13 @@
 //     Unlike the slice form there is no operand reuse between moduli, so the
 //     GEMM is a plain large-tile INT8 GEMM: 256 x 256 C tile, 8 wavefronts of
 //     128 x 64, K staged 64 deep through a three-buffer LDS ring (96 KiB).
-//  4. Reconstruction. With M = prod m_t > 4 k 2^(pa+pb), C' is the unique
+//  4. Reconstruction. With M = prod m_t > 4 max ||a'_i|| ||b'_j||, C' is the unique
 //     integer in (-M/2, M/2) with those residues. X = sum_t r_t W_t
 //     (W_t = (M/m_t) * ((M/m_t)^-1 mod m_t)) is formed exactly in four 40-bit
 //     limbs (every partial sum is an integer below 2^53), q = rint(X/M),
 //     C' = X - qM limb by limb (exact fma), the limbs are balanced and summed
 //     from the top in double-double and rounded once; then
-//     C = alpha * C' * 2^(ea+eb-pa-pb) (+ beta*C through fma).
+//     C = alpha * C' * 2^(ta+tb-pa-pb) (+ beta*C through fma).
 //
 // Rows / columns holding Inf or NaN give NaN outputs, as in the slice form.
 #include <math.h>
@@ -55,6 +58,114 @@ __device__ __forceinline__ double red(double y, double m, double inv_m, double h
     r = r > hi_lim ? r - m : r;
     r = r < lo_lim ? r + m : r;
     return r;
+}
+
+// Norm exponents. For row r with max exponent e (frexp; NONFINITE for Inf/NaN
+// rows) each entry contributes u(x) = ceil(fl(v*v) * 2^40) + 1 units of 2^-40,
+// v = |x| * 2^-e in [0, 1) (0 for x = 0): an upper bound of v^2 * 2^40, since
+// fl(v*v) is at most 2^-53 relatively below v*v. The integer sum S bounds
+// ||x||^2 <= S * 2^(2e-40) whatever the summation order, so
+//   t = e + ceil((L - 40) / 2),  L = bit length of S - 1  (2^L >= S)
+// is the same on every device and ||x|| <= 2^t. S < 2^63 while k < 2^22.
+// sig = log2(||x|| / sqrt(nnz)) - mean log2|x| (rms over geometric mean of the
+// non-zero entries, from S and the spread e - mean log2|x|) feeds the bit
+// chooser.
+struct NormPart {
+    unsigned long long s;
+    unsigned int cnt, pad;
+};
+
+__device__ __forceinline__ unsigned long long norm_units(double x, int e) {
+    const double v = ldexp(fabs(x), -e);
+    return v > 0.0 ? (unsigned long long)ceil(v * v * 0x1p40) + 1ull : 0ull;
+}
+
+__device__ __forceinline__ void norm_finish(unsigned long long S, unsigned int cnt, int e, float spread, int* t,
+                                            float* sig) {
+    if (e == NONFINITE || S == 0ull) {
+        *t = e == NONFINITE ? NONFINITE : 0;
+        *sig = 0.0f;
+        return;
+    }
+    const int L = S == 1ull ? 0 : 64 - __clzll((long long)(S - 1ull));
+    *t = e + (L >= 40 ? (L - 39) >> 1 : -((40 - L) >> 1));
+    *sig = 0.5f * (log2f((float)S) - 40.0f) + spread - 0.5f * log2f((float)cnt);
+}
+
+// k-contiguous rows (element (r, k) at X[r*sr + k]); one wavefront per row.
+__global__ __launch_bounds__(256) void crt_norm_rows(const double* __restrict__ X, int64_t sr, int R, int Kd,
+                                                     const int* __restrict__ e, const float* __restrict__ spread,
+                                                     int* __restrict__ t, float* __restrict__ sig) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= R) return;
+    const int ex = e[r];
+    unsigned long long S = 0ull;
+    unsigned int cnt = 0u;
+    if (ex != NONFINITE) {
+        const double* row = X + (int64_t)r * sr;
+        for (int k = lane; k < Kd; k += 64) {
+            const unsigned long long u = norm_units(row[k], ex);
+            S += u;
+            cnt += u != 0ull;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        S += __shfl_xor(S, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
+    }
+    if (lane == 0) norm_finish(S, cnt, ex, spread[r], &t[r], &sig[r]);
+}
+
+// Rows r contiguous across r (element (r, k) at X[k*sk + r]): pass 1 sums one
+// of kparts K ranges for 64 rows per block (wave w takes every fourth k), pass
+// 2 adds the ranges. Integer sums: the result does not depend on the split.
+__global__ __launch_bounds__(256) void crt_norm_cols_partial(const double* __restrict__ X, int64_t sk, int R, int Kd,
+                                                             int krange, const int* __restrict__ e,
+                                                             NormPart* __restrict__ part) {
+    __shared__ NormPart red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = blockIdx.x * 64 + lane;
+    const int k0 = blockIdx.y * krange;
+    const int k1 = min(Kd, k0 + krange);
+    unsigned long long S = 0ull;
+    unsigned int cnt = 0u;
+    const int ex = r < R ? e[r] : NONFINITE;
+    if (ex != NONFINITE) {
+        const double* col = X + r;
+        for (int k = k0 + w; k < k1; k += 4) {
+            const unsigned long long u = norm_units(col[(int64_t)k * sk], ex);
+            S += u;
+            cnt += u != 0ull;
+        }
+    }
+    red[w][lane] = NormPart{S, cnt, 0u};
+    __syncthreads();
+    if (w == 0 && r < R) {
+        NormPart acc = red[0][lane];
+#pragma unroll
+        for (int i = 1; i < 4; i++) {
+            acc.s += red[i][lane].s;
+            acc.cnt += red[i][lane].cnt;
+        }
+        part[(int64_t)blockIdx.y * R + r] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void crt_norm_cols_finish(const NormPart* __restrict__ part, int R, int kparts,
+                                                            const int* __restrict__ e,
+                                                            const float* __restrict__ spread, int* __restrict__ t,
+                                                            float* __restrict__ sig) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    unsigned long long S = 0ull;
+    unsigned int cnt = 0u;
+    for (int kp = 0; kp < kparts; kp++) {
+        S += part[(int64_t)kp * R + r].s;
+        cnt += part[(int64_t)kp * R + r].cnt;
+    }
+    norm_finish(S, cnt, e[r], spread[r], &t[r], &sig[r]);
 }
 
 // Residues of 16 integer images s = c2*2^44 + c1*2^22 + c0 (22-bit signed
@@ -143,9 +254,16 @@ __global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, in
     for (int t = 0; t < T.nmod; t++) out[t * plane_stride + off] = residues16(c2, c1, c0, T, t);
 }
 
-// Column strips of 16 tile rows (256 x 256 tiles), planes in blockIdx.y.
-__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
-    constexpr int GROUP = 16;
+// Column strips of `group` tile rows (256 x 256 tiles), planes in blockIdx.y.
+// With xcd set (and a tile count divisible by 8), workgroup b, which the
+// dispatcher places on XCD b % 8, takes tile (b % 8) * (count / 8) + b / 8 of
+// that order, so the workgroups resident on one XCD (and sharing its L2) work
+// on neighbouring tiles.
+__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int& tm, int& tn, int group = 16,
+                                            bool xcd = false) {
+    const int count = tiles_m * tiles_n;
+    if (xcd && count % 8 == 0) bid = (bid & 7) * (count >> 3) + (bid >> 3);
+    const int GROUP = group;
     const int per_group = GROUP * tiles_n;
     const int g = bid / per_group;
     const int first_m = g * GROUP;
@@ -173,14 +291,14 @@ constexpr int STAGE_V4 = 32 * 64;  // v4i per stage (32 blocks of 1 KiB)
 
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void crt_gemm_i8(
     int kchunks, const v4i* __restrict__ Ap, int64_t a_plane, const v4i* __restrict__ Bp, int64_t b_plane,
-    v4i* __restrict__ Cr, int64_t c_plane, int cblocks_n, int tiles_m, int tiles_n, CrtTables T) {
+    v4i* __restrict__ Cr, int64_t c_plane, int cblocks_n, int tiles_m, int tiles_n, int group, int xcd, CrtTables T) {
     extern __shared__ v4i lds[];  // [NBUF_LDS][32 blocks: A kc*8+rb (0..15), B 16 + kc*8+cb][64 lanes]
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = w >> 2, wn = w & 3;
     const int t = blockIdx.y;
     int tm, tn;
-    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn, group, xcd != 0);
     const v4i* A = Ap + t * a_plane + (int64_t)tm * 8 * kchunks * 64;
     const v4i* B = Bp + t * b_plane + (int64_t)tn * 8 * kchunks * 64;
     // Loads of a stage: 32 blocks of 1 KiB; wavefront w moves blocks w, w+8, w+16, w+24.
@@ -275,6 +393,125 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             o.z = (r[8] & 255) | ((r[9] & 255) << 8) | ((r[10] & 255) << 16) | ((r[11] & 255) << 24);
             o.w = (r[12] & 255) | ((r[13] & 255) << 8) | ((r[14] & 255) << 16) | ((r[15] & 255) << 24);
             const int64_t rbc = tm * 8 + wm * 4 + i, cbc = tn * 8 + wn * 2 + j;
+            Cr[t * c_plane + (rbc * cblocks_n + cbc) * 64 + lane] = o;
+        }
+}
+
+// The same product with four wavefronts of 128 x 128 (2 x 2 over the 256 x 256
+// tile, one wavefront per SIMD): each fragment read from LDS feeds four MFMAs
+// instead of two or four, halving the LDS read traffic per operation (the
+// 256 accumulator registers live in the accumulation register file). Stage
+// loads: 32 blocks of 1 KiB, wavefront w moves blocks w + 4i (i < 4: A, else B).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void crt_gemm_i8_w4(
+    int kchunks, const v4i* __restrict__ Ap, int64_t a_plane, const v4i* __restrict__ Bp, int64_t b_plane,
+    v4i* __restrict__ Cr, int64_t c_plane, int cblocks_n, int tiles_m, int tiles_n, CrtTables T) {
+    extern __shared__ v4i lds[];  // [NBUF_LDS][32 blocks: A kc*8+rb (0..15), B 16 + kc*8+cb][64 lanes]
+    // An (empty) use of the accumulation registers: without one the compiler
+    // assumes the kernel needs none and keeps the 256 accumulators in the
+    // 256 ordinary vector registers, spilling everything else.
+    asm volatile("" ::: "a0");
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w >> 1, wn = w & 1;
+    const int t = blockIdx.y;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    // block w + 4i of a stage: operand i >> 2, row/col block (w >> 1) + 2 (i & 3), k chunk w & 1
+    const int64_t blk_stride = (int64_t)2 * kchunks * 64;  // two row blocks apart
+    const v4i* srcA = Ap + t * a_plane + ((int64_t)tm * 8 * kchunks + (int64_t)(w >> 1) * kchunks + (w & 1)) * 64 + lane;
+    const v4i* srcB = Bp + t * b_plane + ((int64_t)tn * 8 * kchunks + (int64_t)(w >> 1) * kchunks + (w & 1)) * 64 + lane;
+    const int dst0 = ((w & 1) * 8 + (w >> 1)) * 64 + lane;  // + 2i blocks; B at + 16 blocks
+    const int stages = kchunks / 2;
+    v4i r0[8], r1[8];
+    auto load = [&](int s, v4i (&r)[8]) {
+        const int64_t o = (int64_t)s * 128;
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = srcA[o + i * blk_stride];
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[4 + i] = srcB[o + i * blk_stride];
+    };
+    auto store = [&](int buf, const v4i (&r)[8]) {
+        v4i* base = lds + buf * STAGE_V4 + dst0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) base[i * 128] = r[i];
+#pragma unroll
+        for (int i = 0; i < 4; i++) base[(16 + 2 * i) * 64] = r[4 + i];
+    };
+    v16i acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double m = T.mod[t], inv_m = T.inv[t], hl = T.hi_lim[t], ll = T.lo_lim[t];
+
+    auto compute = [&](int buf) {
+        const v4i* base = lds + buf * STAGE_V4 + lane;
+#pragma unroll
+        for (int kc = 0; kc < 2; kc++) {
+            v4i af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) af[i] = base[(kc * 8 + wm * 4 + i) * 64];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bf[j] = base[(16 + kc * 8 + wn * 4 + j) * 64];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    };
+    // Fold between 65536-deep K ranges: acc -= rint(acc / m) * m in integer
+    // arithmetic (the FP32 quotient may be off by a few units, leaving
+    // |acc| < 4m, far inside the INT32 headroom the next range needs).
+    const int mi = (int)m;
+    const float invf = T.invf[t];
+    auto fold = [&]() {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[i][j][r] -= (int)rintf((float)acc[i][j][r] * invf) * mi;
+    };
+
+    // Same branch-free three-buffer pipeline as crt_gemm_i8, with the folds
+    // outside the inner loop.
+    load(0, r0);
+    store(0, r0);
+    load(1, r1);
+    __syncthreads();
+    const int last = stages - 1;
+    constexpr int FOLD_STAGES = FOLD_STEPS / 2;
+    int buf = 0;
+    for (int s0 = 0; s0 < stages; s0 += FOLD_STAGES) {
+        const int s1 = min(s0 + FOLD_STAGES, stages);
+        for (int s = s0; s < s1; s += 2) {
+            load(min(s + 2, last), r0);
+            compute(buf);
+            int nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
+            store(nb, r1);
+            __syncthreads();
+            buf = nb;
+            load(min(s + 3, last), r1);
+            compute(buf);
+            nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
+            store(nb, r0);
+            __syncthreads();
+            buf = nb;
+        }
+        if (s1 < stages) fold();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const v16i r = reduce_acc(acc[i][j], m, inv_m, hl, ll);
+            v4i o;
+            o.x = (r[0] & 255) | ((r[1] & 255) << 8) | ((r[2] & 255) << 16) | ((r[3] & 255) << 24);
+            o.y = (r[4] & 255) | ((r[5] & 255) << 8) | ((r[6] & 255) << 16) | ((r[7] & 255) << 24);
+            o.z = (r[8] & 255) | ((r[9] & 255) << 8) | ((r[10] & 255) << 16) | ((r[11] & 255) << 24);
+            o.w = (r[12] & 255) | ((r[13] & 255) << 8) | ((r[14] & 255) << 16) | ((r[15] & 255) << 24);
+            const int64_t rbc = tm * 8 + wm * 4 + i, cbc = tn * 8 + wn * 4 + j;
             Cr[t * c_plane + (rbc * cblocks_n + cbc) * 64 + lane] = o;
         }
 }
@@ -389,6 +626,34 @@ struct Big {
 const int CRT_MODULI[CRT_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
                                         223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
 
+hipError_t launch_crt_norms(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
+                            const SplitPlan& sp, int* ta, int* tb, float* sig_a, float* sig_b, hipStream_t stream) {
+    const int M = (int)sp.m, N = (int)sp.n, K = (int)sp.k;
+    NormPart* part = reinterpret_cast<NormPart*>(sp.stats_part);
+    auto rows = [&](const double* x, int64_t sr, int R, const int* e, const float* spread, int* t, float* sig) {
+        hipLaunchKernelGGL(crt_norm_rows, dim3((R + 3) / 4), dim3(256), 0, stream, x, sr, R, K, e, spread, t, sig);
+        return hipGetLastError();
+    };
+    auto cols = [&](const double* x, int64_t sk, int R, const int* e, const float* spread, int* t, float* sig) {
+        const int kparts = split_stats_kparts(R, K);
+        const int krange = (K + kparts - 1) / kparts;
+        hipLaunchKernelGGL(crt_norm_cols_partial, dim3((R + 63) / 64, kparts), dim3(256), 0, stream, x, sk, R, K,
+                           krange, e, part);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(crt_norm_cols_finish, dim3((R + 255) / 256), dim3(256), 0, stream, part, R, kparts, e,
+                           spread, t, sig);
+        return hipGetLastError();
+    };
+    // Rows of op(A): ROW -> element (i,k) at a[i*lda + k]; COL -> a[k*lda + i].
+    hipError_t err = layout_a == JLC_LAYOUT_ROW ? rows(a, lda, M, sp.ea, sp.spread_a, ta, sig_a)
+                                                : cols(a, lda, M, sp.ea, sp.spread_a, ta, sig_a);
+    if (err != hipSuccess) return err;
+    // Columns of op(B): COL -> element (k,j) at b[j*ldb + k]; ROW -> b[k*ldb + j].
+    return layout_b == JLC_LAYOUT_COL ? rows(b, ldb, N, sp.eb, sp.spread_b, tb, sig_b)
+                                      : cols(b, ldb, N, sp.eb, sp.spread_b, tb, sig_b);
+}
+
 double crt_log2_product(int nmod) {
     double s = 0.0;
     for (int t = 0; t < nmod && t < CRT_MAX_MODULI; t++) s += std::log2((double)CRT_MODULI[t]);
@@ -479,11 +744,20 @@ hipError_t launch_crt(int layout_a, int layout_b, const double* a, int64_t lda, 
     const int tiles_m = (int)(p.mpad / CT), tiles_n = (int)(p.npad / CT);
     const int cblocks_n = (int)(p.npad / 32);
     const size_t lds = (size_t)NBUF_LDS * STAGE_V4 * sizeof(v4i);
-    (void)hipFuncSetAttribute((const void*)crt_gemm_i8, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const TuneTable tt = tune_snapshot();
+    const bool w4 = tt.crt_waves == 4;
+    const int group = tt.crt_group > 0 ? tt.crt_group : 16, xcd = tt.crt_xcd;
+    const void* kfn = w4 ? (const void*)crt_gemm_i8_w4 : (const void*)crt_gemm_i8;
+    (void)hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     mark_main(info, true, stream);
-    hipLaunchKernelGGL(crt_gemm_i8, dim3(tiles_m * tiles_n, p.nmod), dim3(512), lds, stream, kchunks, ap,
-                       p.a_plane_bytes / 16, bp, p.b_plane_bytes / 16, cr, p.c_plane_bytes / 16, cblocks_n, tiles_m,
-                       tiles_n, T);
+    if (w4)
+        hipLaunchKernelGGL(crt_gemm_i8_w4, dim3(tiles_m * tiles_n, p.nmod), dim3(256), lds, stream, kchunks, ap,
+                           p.a_plane_bytes / 16, bp, p.b_plane_bytes / 16, cr, p.c_plane_bytes / 16, cblocks_n,
+                           tiles_m, tiles_n, T);
+    else
+        hipLaunchKernelGGL(crt_gemm_i8, dim3(tiles_m * tiles_n, p.nmod), dim3(512), lds, stream, kchunks, ap,
+                           p.a_plane_bytes / 16, bp, p.b_plane_bytes / 16, cr, p.c_plane_bytes / 16, cblocks_n,
+                           tiles_m, tiles_n, group, xcd, T);
     mark_main(info, false, stream);
     if ((err = hipGetLastError()) != hipSuccess) return err;
     const int cblocks = (int)(p.mpad / 32) * cblocks_n;

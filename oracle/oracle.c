@@ -692,6 +692,25 @@ static double crt_scaled(double x, int e, int p) {
     return fabs(s) < 9007199254740992.0 ? rint(s) : s;
 }
 
+/* Norm exponent of the CRT form's scaling: with e the row's max exponent,
+ * S = sum over entries of ceil(fl(v*v) * 2^40) + 1 (v = |x| 2^-e, 0 entries
+ * add 0) is an integer bound of ||x||^2 in units of 2^(2e-40); the result is
+ * t = e + ceil((L - 40) / 2) with 2^L >= S, so ||x||_2 <= 2^t. Integer
+ * sums: any summation order gives the same t. */
+int oracle_crt_norm_exponent(const double* x, int64_t stride, int64_t count, int e) {
+    if (e == SPLIT_NONFINITE) return SPLIT_NONFINITE;
+    unsigned long long S = 0;
+    for (int64_t i = 0; i < count; i++) {
+        const double v = ldexp(fabs(x[i * stride]), -e);
+        if (v > 0.0) S += (unsigned long long)ceil(v * v * 0x1p40) + 1ull;
+    }
+    if (S == 0) return 0;
+    int L = 0;
+    while (L < 64 && (S - 1) >> L) L++; /* bit length of S - 1 */
+    const int twice = L - 40;           /* ceil(twice / 2) */
+    return e + (twice >= 0 ? (twice + 1) / 2 : -((-twice) / 2));
+}
+
 void oracle_crt_residues(const double* x, int64_t stride, int64_t count, int e, int p, int nmod, int8_t* out,
                          int64_t out_stride) {
     for (int64_t q = 0; q < count; q++) {
@@ -744,13 +763,13 @@ void oracle_crt_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k
     for (int64_t i = 0; i < m; i++) {
         const double* row = a_layout == ORACLE_ROW ? a + i * lda : a + i;
         int64_t st = a_layout == ORACLE_ROW ? 1 : lda;
-        ea[i] = oracle_split_exponent(row, st, k);
+        ea[i] = oracle_crt_norm_exponent(row, st, k, oracle_split_exponent(row, st, k));
         oracle_crt_residues(row, st, k, ea[i], pa, nmod, ar + i * k, m * k);
     }
     for (int64_t j = 0; j < n; j++) {
         const double* col = b_layout == ORACLE_ROW ? b + j : b + j * ldb;
         int64_t st = b_layout == ORACLE_ROW ? ldb : 1;
-        eb[j] = oracle_split_exponent(col, st, k);
+        eb[j] = oracle_crt_norm_exponent(col, st, k, oracle_split_exponent(col, st, k));
         oracle_crt_residues(col, st, k, eb[j], pb, nmod, br + j * k, n * k);
     }
     if (threads < 1) threads = 1;

@@ -104,8 +104,11 @@ int64_t oracle_split_bf16_chunk_k(int slices, int64_t k);
 
 /* CRT split form (modular integer products recombined by the Chinese remainder
  * theorem; Ozaki-Uchino-Imamura 2025). Moduli: a fixed list of 20 pairwise
- * coprime integers <= 256. nmod moduli, pa / pb bits for op(A) rows / op(B)
- * columns; requires log2(prod m) >= pa + pb + ceil(log2 k) + 2. */
+ * coprime integers <= 256. nmod moduli; row i of op(A) is scaled by
+ * 2^(pa - t_i) with 2^t_i >= ||row||_2 (oracle_crt_norm_exponent), column j of
+ * op(B) by 2^(pb - t_j); requires
+ * log2(prod m) >= log2((2^pa + sqrt(k)/2)(2^pb + sqrt(k)/2)) + 2. */
+int oracle_crt_norm_exponent(const double* x, int64_t stride, int64_t count, int e);
 int oracle_crt_max_moduli(void);
 int oracle_crt_modulus(int t);
 double oracle_crt_log2_product(int nmod);

@@ -72,6 +72,8 @@ def _load():
     lib.oracle_crt_log2_product.argtypes = [C.c_int]
     lib.oracle_crt_log2_product.restype = _d
     lib.oracle_crt_tables.argtypes = [C.c_int, _p, _p, _p]
+    lib.oracle_crt_norm_exponent.argtypes = [_p, _i64, _i64, C.c_int]
+    lib.oracle_crt_norm_exponent.restype = C.c_int
     lib.oracle_crt_residues.argtypes = [_p, _i64, _i64, C.c_int, C.c_int, C.c_int, _p, _i64]
     lib.oracle_crt_reconstruct.argtypes = [_p, C.c_int, _p, _p, _p]
     lib.oracle_crt_reconstruct.restype = _d
@@ -342,13 +344,33 @@ def crt_gemm(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float,
                             _ptr(c, c_off), ldc, nmod, pa, pb, nthreads or threads())
 
 
+CRT_MAX_BITS = 65
+
+
+def crt_norm_exponent(x: np.ndarray, e: int | None = None) -> int:
+    """t with ||x||_2 <= 2^t from the integer norm bound (e: max exponent; computed when None)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib = _load()
+    if e is None:
+        e = lib.oracle_split_exponent(_ptr(x), 1, x.size)
+    return lib.oracle_crt_norm_exponent(_ptr(x), 1, x.size, e)
+
+
+def crt_fits(pa: int, pb: int, k: int, nmod: int) -> bool:
+    """Whether nmod moduli hold C' for norm bits (pa, pb): (2^pa + sqrt(k)/2)(2^pb + sqrt(k)/2) * 4 <= M."""
+    h = 0.5 * np.sqrt(float(k))
+    need = np.log2(np.ldexp(1.0, pa) + h) + np.log2(np.ldexp(1.0, pb) + h) + 2.0
+    return bool(need <= crt_log2_product(nmod) - 1e-6)
+
+
 def crt_bits_for(nmod: int, k: int) -> tuple[int, int]:
-    """(pa, pb) the CRT form uses for an explicit moduli count: the available
-    bits floor(log2 M) - ceil(log2 k) - 2 split evenly, at most 66 each."""
-    lk = int(np.ceil(np.log2(k))) if k > 1 else 0
-    avail = int(np.floor(crt_log2_product(nmod))) - lk - 2
-    pa = min(avail // 2, 66)
-    return pa, min(avail - pa, 66)
+    """(pa, pb) the CRT form uses for an explicit moduli count: the largest even
+    split of the norm bits that the moduli hold, at most 65 each."""
+    for total in range(2 * CRT_MAX_BITS, 3, -1):
+        pa = total // 2
+        if crt_fits(pa, total - pa, k, nmod):
+            return pa, total - pa
+    raise ValueError(f"{nmod} moduli leave no bits for k={k}")
 
 
 def split_gemm_bf16(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, a, lda: int, b, ldb: int,

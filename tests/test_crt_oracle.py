@@ -77,27 +77,77 @@ def test_reconstruction_is_correctly_rounded():
     assert orc.crt_reconstruct([0] * 16, 16) == 0.0
 
 
+def _norm_sq(x) -> Fraction:
+    return sum((Fraction(float(v)) ** 2 for v in x), Fraction(0))
+
+
+def test_norm_exponent_known_answers():
+    assert orc.crt_norm_exponent(np.ones(4)) == 2          # ||x|| = 2; the +1 units make exact powers round up
+    assert orc.crt_norm_exponent(np.array([0.75])) == 0    # 0.5 < 0.75 <= 1
+    assert orc.crt_norm_exponent(np.array([-3.0, 4.0])) == 3  # ||x|| = 5
+    assert orc.crt_norm_exponent(np.zeros(7)) == 0
+    assert orc.crt_norm_exponent(np.array([1.0, np.nan])) == orc.SPLIT_NONFINITE
+    assert orc.crt_norm_exponent(np.array([np.inf, 0.0])) == orc.SPLIT_NONFINITE
+    # tiny and huge scales: no overflow or underflow in the bound
+    assert orc.crt_norm_exponent(np.array([3e-300, 4e-300])) == math.frexp(5e-300)[1]
+    assert orc.crt_norm_exponent(np.array([3e300, 4e300])) == math.frexp(5e300)[1]
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.5, 2.0, 8.0])
+def test_norm_exponent_bounds_the_norm_tightly(phi):
+    """||x|| <= 2^t exactly, and 2^(t-1) lies below the norm plus the bound's
+    per-entry slack (2 units of 2^(2e-40) per non-zero entry)."""
+    rng = np.random.default_rng(int(phi * 10) + 1)
+    for size in (1, 2, 31, 500):
+        for _ in range(20):
+            x = (rng.random(size) - 0.5) * np.exp(phi * rng.standard_normal(size))
+            x[rng.random(size) < 0.1] = 0.0
+            t = orc.crt_norm_exponent(x)
+            nsq = _norm_sq(x)
+            if nsq == 0:
+                assert t == 0
+                continue
+            e = math.frexp(float(np.max(np.abs(x))))[1]
+            assert nsq <= Fraction(2) ** (2 * t), (size, t)
+            slack = Fraction(2) * Fraction(2) ** (2 * e - 40) * int(np.count_nonzero(x))
+            assert Fraction(2) ** (2 * (t - 1)) < nsq + slack, (size, t)
+            # integer sums: order and scale do not matter
+            assert orc.crt_norm_exponent(rng.permutation(x)) == t
+            assert orc.crt_norm_exponent(np.ldexp(x, 7)) == t + 7
+
+
 def _exact(A, B, pa, pb, alpha=1.0):
     m, k = A.shape
     n = B.shape[1]
-    ea = [math.frexp(float(np.max(np.abs(A[i]))))[1] for i in range(m)]
-    eb = [math.frexp(float(np.max(np.abs(B[:, j]))))[1] for j in range(n)]
+    ea = [orc.crt_norm_exponent(A[i]) for i in range(m)]
+    eb = [orc.crt_norm_exponent(B[:, j]) for j in range(n)]
 
     def img(x, e, p):
         s = math.ldexp(float(x), p - e)
         return int(round(s)) if abs(s) < 2 ** 53 else int(s)
     Ai = [[img(A[i, q], ea[i], pa) for q in range(k)] for i in range(m)]
     Bi = [[img(B[q, j], eb[j], pb) for j in range(n)] for q in range(k)]
+    # norm scaling: ||a'|| <= 2^pa + sqrt(k)/2 (squared: (2^pa + sqrt(k)/2)^2 <= 4^pa + 2^pa sqrt(k) + k)
+    for i in range(m):
+        assert sum(v * v for v in Ai[i]) <= 4 ** pa + 2 ** pa * (math.isqrt(k) + 1) + k
+    for j in range(n):
+        assert sum(Bi[q][j] ** 2 for q in range(k)) <= 4 ** pb + 2 ** pb * (math.isqrt(k) + 1) + k
     C = np.zeros((m, n))
     for i in range(m):
         for j in range(n):
-            C[i, j] = alpha * math.ldexp(float(sum(Ai[i][q] * Bi[q][j] for q in range(k))), ea[i] + eb[j] - pa - pb)
+            cp = sum(Ai[i][q] * Bi[q][j] for q in range(k))
+            C[i, j] = alpha * math.ldexp(float(cp), ea[i] + eb[j] - pa - pb)
     return C
 
 
 def _nmod_for(pa, pb, k):
-    need = pa + pb + (math.ceil(math.log2(k)) if k > 1 else 0) + 2
-    return next(t for t in range(2, 21) if orc.crt_log2_product(t) >= need)
+    """Fewest moduli whose product exceeds 4 (2^pa + sqrt(k)/2)(2^pb + sqrt(k)/2):
+    by Cauchy-Schwarz every integer product of norm-scaled images is below a quarter of it."""
+    h = Fraction(math.isqrt(4 * k) + 1, 4)  # >= sqrt(k)/2
+    bound = 4 * (2 ** pa + h) * (2 ** pb + h)
+    nmod = next(t for t in range(2, 21) if math.prod(orc.crt_moduli(t)) > bound)
+    assert orc.crt_fits(pa, pb, k, nmod)
+    return nmod
 
 
 @pytest.mark.parametrize("m,n,k,pa,pb,phi", [(7, 5, 33, 20, 20, 0.5), (9, 6, 70, 53, 53, 0.5),

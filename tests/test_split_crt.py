@@ -71,8 +71,9 @@ def test_automatic_moduli_count():
     rng = np.random.default_rng(8)
     st = _case(rng, 0, 0, 300, 280, 1000, 0, 1.0, 0.0, phi=0.5)
     assert 2 <= st.slices <= 20 and st.bits_a >= 40 and st.bits_b >= 40
-    need = st.bits_a + st.bits_b + int(np.ceil(np.log2(1000))) + 2
-    assert orc.crt_log2_product(st.slices) >= need > orc.crt_log2_product(st.slices - 1)
+    # the fewest moduli that hold (2^pa + sqrt(k)/2)(2^pb + sqrt(k)/2) with the M/4 margin
+    assert orc.crt_fits(st.bits_a, st.bits_b, 1000, st.slices)
+    assert not orc.crt_fits(st.bits_a, st.bits_b, 1000, st.slices - 1)
 
 
 def test_modular_fold_beyond_65536_k():
@@ -144,3 +145,32 @@ def test_accuracy_matches_fp64_at_auto_moduli():
         errs[name] = (float(np.max(err / np.abs(hi + lo))), float(np.max(err / ab)))
     assert errs["crt"][0] <= 2 * errs["fp64"][0] + 1e-16, errs
     assert errs["crt"][1] <= 2 * errs["fp64"][1] + 2 ** -53, errs
+
+
+def test_four_wave_product_kernel():
+    """The 4-wavefront (128 x 128 per wavefront) product kernel, selected by
+    the crt_waves tuning key, gives the same bits, across a modular fold too."""
+    old = native.tune_get("crt_waves")
+    native.tune_set("crt_waves", 4)
+    try:
+        rng = np.random.default_rng(44)
+        _case(rng, 0, 0, 300, 270, 130, 16, 1.0, 0.0)
+        _case(rng, 1, 1, 257, 65, 129, 9, -1.5, 0.5, pad=1, off=3)
+        _case(rng, 0, 0, 24, 20, 70000, 16, 1.0, 0.0, phi=0.5)
+    finally:
+        native.tune_set("crt_waves", old)
+
+
+def test_depth_beyond_norm_sum_limit():
+    """k >= 2^22 is beyond the integer norm sums: automatic mode hands the call
+    to the FP64 kernel, an explicit moduli count is rejected."""
+    m = n = 2
+    k = 1 << 22
+    rng = np.random.default_rng(3)
+    a = rng.uniform(-1, 1, m * k)
+    b = rng.uniform(-1, 1, k * n)
+    c = np.zeros(m * n)
+    st = native.dgemm(0, 0, m, n, k, 1.0, a, k, b, n, 0.0, c, n, native.MODE_SPLIT_CRT)
+    assert st.kernel_name == "mfma_f64"
+    with pytest.raises(native.JlcError):
+        native.dgemm(0, 0, m, n, k, 1.0, a, k, b, n, 0.0, c, n, native.mode_slices(native.MODE_SPLIT_CRT, 16))
