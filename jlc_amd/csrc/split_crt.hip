@@ -48,7 +48,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 constexpr int NONFINITE = -100000;
 constexpr int CT = 256;        // C tile rows and columns
 constexpr int KSTAGE = 64;     // K per LDS stage (two 32-deep MFMA steps)
-constexpr int KPAD = 128;      // K padding: an even number of stages
+constexpr int KPAD = 256;      // K padding: a multiple of four stages
 constexpr int FOLD_STEPS = 2048;  // 32-deep steps between modular folds (65536 k)
 
 // Symmetric residue of an integer-valued double |y| < 2^53 modulo m.
@@ -281,13 +281,25 @@ __device__ __forceinline__ v16i reduce_acc(v16i acc, double m, double inv_m, dou
 
 // One INT8 GEMM per modulus (blockIdx.y); C residues out in accumulator order.
 // Wavefront w: rows (w>>2)*128 + [0,128) as 4 blocks, columns (w&3)*64 + [0,64) as 2 blocks.
-// K runs in 64-deep stages (an even number of them: K is padded to 128) through
-// a ring of three 32 KiB LDS buffers fed from two register sets: stage s+2 is
-// fetched from global memory while stage s is multiplied and stage s+1 (fetched
-// one stage earlier) is written to LDS, so a global load has two stages of MFMA
-// work to arrive. One barrier per stage.
-constexpr int NBUF_LDS = 3;
+// K runs in 64-deep stages (a multiple of four of them: K is padded to 256).
+// Global loads run four stages ahead in four register sets, so a load has
+// four stages of MFMA work to arrive (two stages left the MFMA pipe idle on
+// load latency: MFMA busy 58% of cycles against 64% with four); stage s is
+// multiplied from one of two 32 KiB LDS buffers while stage s+1 is written to
+// the other, one barrier per stage. The loop is unrolled by four so that the
+// register sets have static names, and the modular folds sit between
+// 65536-deep K ranges, outside it.
+constexpr int NBUF_LDS = 2;
 constexpr int STAGE_V4 = 32 * 64;  // v4i per stage (32 blocks of 1 KiB)
+constexpr int PREFETCH = 4;        // register sets (stages in flight)
+
+__device__ __forceinline__ void fold_acc(v16i& acc, int m, float invf) {
+    // acc -= rint(acc / m) * m with an FP32 quotient (off by a few units at
+    // most): |acc| < 4m afterwards, far inside the INT32 headroom the next
+    // 65536-deep range needs (k * 2^14 = 2^30).
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] -= (int)rintf((float)acc[r] * invf) * m;
+}
 
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void crt_gemm_i8(
     int kchunks, const v4i* __restrict__ Ap, int64_t a_plane, const v4i* __restrict__ Bp, int64_t b_plane,
@@ -312,15 +324,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         dst[i] = ((b & 16) + kc * 8 + rb) * 64 + lane;
     }
     const int stages = kchunks / 2;
-    v4i r0[4], r1[4];
-    auto load = [&](int s, v4i (&r)[4]) {
+    v4i r[PREFETCH][4];
+    auto load = [&](int s, v4i (&x)[4]) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) r[i] = src[i][(int64_t)s * 128];
+        for (int i = 0; i < 4; i++) x[i] = src[i][(int64_t)s * 128];
     };
-    auto store = [&](int buf, const v4i (&r)[4]) {
+    auto store = [&](int buf, const v4i (&x)[4]) {
         v4i* base = lds + buf * STAGE_V4;
 #pragma unroll
-        for (int i = 0; i < 4; i++) base[dst[i]] = r[i];
+        for (int i = 0; i < 4; i++) base[dst[i]] = x[i];
     };
     v16i acc[4][2];
 #pragma unroll
@@ -345,41 +357,38 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                     acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
     };
-    auto fold = [&](int s) {
-        if (((s + 1) * 2) % FOLD_STEPS == 0) {
+
+    // Stage q waits in register set q % 4 and is multiplied from LDS buffer
+    // q % 2. Iteration s: multiply stage s, write stage s+1 to LDS, refill
+    // its register set with stage s+5 (clamped: the repeats of the last stage
+    // land in a set nobody stores), barrier. Branch-free, so the compiler's
+    // wait counts keep the later sets' loads in flight.
+    const int last = stages - 1;
+#pragma unroll
+    for (int q = 0; q < PREFETCH; q++) load(min(q, last), r[q]);
+    store(0, r[0]);
+    load(min(PREFETCH, last), r[0]);
+    __syncthreads();
+    constexpr int FOLD_STAGES = FOLD_STEPS / 2;  // a multiple of 4
+    const int mi = (int)m;
+    const float invf = T.invf[t];
+    for (int s0 = 0; s0 < stages; s0 += FOLD_STAGES) {
+        const int s1 = min(s0 + FOLD_STAGES, stages);
+        for (int s = s0; s < s1; s += PREFETCH) {
+#pragma unroll
+            for (int u = 0; u < PREFETCH; u++) {
+                compute(u & 1);
+                store((u + 1) & 1, r[(u + 1) & 3]);
+                load(min(s + u + PREFETCH + 1, last), r[(u + 1) & 3]);
+                __syncthreads();
+            }
+        }
+        if (s1 < stages) {
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc[i][j] = reduce_acc(acc[i][j], m, inv_m, hl, ll);
+                for (int j = 0; j < 2; j++) fold_acc(acc[i][j], mi, invf);
         }
-    };
-
-    // Iteration s: fetch stage s+2, multiply stage s, write stage s+1 to LDS.
-    // The loop is branch-free (the stage count is even and the prefetch index is
-    // clamped; the clamped repeats land in a buffer nobody reads), so the
-    // compiler's wait counts let each register set's loads stay in flight for
-    // two stages instead of draining at every join.
-    load(0, r0);
-    store(0, r0);
-    load(1, r1);
-    __syncthreads();
-    const int last = stages - 1;
-    int buf = 0;
-    for (int s = 0; s < stages; s += 2) {
-        load(min(s + 2, last), r0);
-        compute(buf);
-        int nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
-        store(nb, r1);
-        __syncthreads();
-        fold(s);
-        buf = nb;
-        load(min(s + 3, last), r1);
-        compute(buf);
-        nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
-        store(nb, r0);
-        __syncthreads();
-        fold(s + 1);
-        buf = nb;
     }
     // Residues of C' in accumulator order: block (row block, col block) of the plane.
 #pragma unroll
@@ -397,6 +406,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
 }
 
+constexpr int W4_NBUF = 3;  // LDS ring of the four-wavefront kernel
+
 // The same product with four wavefronts of 128 x 128 (2 x 2 over the 256 x 256
 // tile, one wavefront per SIMD): each fragment read from LDS feeds four MFMAs
 // instead of two or four, halving the LDS read traffic per operation (the
@@ -405,7 +416,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void crt_gemm_i8_w4(
     int kchunks, const v4i* __restrict__ Ap, int64_t a_plane, const v4i* __restrict__ Bp, int64_t b_plane,
     v4i* __restrict__ Cr, int64_t c_plane, int cblocks_n, int tiles_m, int tiles_n, CrtTables T) {
-    extern __shared__ v4i lds[];  // [NBUF_LDS][32 blocks: A kc*8+rb (0..15), B 16 + kc*8+cb][64 lanes]
+    extern __shared__ v4i lds[];  // [W4_NBUF][32 blocks: A kc*8+rb (0..15), B 16 + kc*8+cb][64 lanes]
     // An (empty) use of the accumulation registers: without one the compiler
     // assumes the kernel needs none and keeps the 256 accumulators in the
     // 256 ordinary vector registers, spilling everything else.
@@ -488,13 +499,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         for (int s = s0; s < s1; s += 2) {
             load(min(s + 2, last), r0);
             compute(buf);
-            int nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
+            int nb = buf == W4_NBUF - 1 ? 0 : buf + 1;
             store(nb, r1);
             __syncthreads();
             buf = nb;
             load(min(s + 3, last), r1);
             compute(buf);
-            nb = buf == NBUF_LDS - 1 ? 0 : buf + 1;
+            nb = buf == W4_NBUF - 1 ? 0 : buf + 1;
             store(nb, r0);
             __syncthreads();
             buf = nb;
@@ -743,9 +754,9 @@ hipError_t launch_crt(int layout_a, int layout_b, const double* a, int64_t lda, 
     if ((err = pack(b, bsr, bsk, N, bblocks, p.eb, p.pb, bp, p.b_plane_bytes / 16)) != hipSuccess) return err;
     const int tiles_m = (int)(p.mpad / CT), tiles_n = (int)(p.npad / CT);
     const int cblocks_n = (int)(p.npad / 32);
-    const size_t lds = (size_t)NBUF_LDS * STAGE_V4 * sizeof(v4i);
     const TuneTable tt = tune_snapshot();
     const bool w4 = tt.crt_waves == 4;
+    const size_t lds = (size_t)(w4 ? W4_NBUF : NBUF_LDS) * STAGE_V4 * sizeof(v4i);
     const int group = tt.crt_group > 0 ? tt.crt_group : 16, xcd = tt.crt_xcd;
     const void* kfn = w4 ? (const void*)crt_gemm_i8_w4 : (const void*)crt_gemm_i8;
     (void)hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
