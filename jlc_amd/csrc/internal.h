@@ -146,8 +146,7 @@ struct CrtTables {
     int pad_ = 0;
     double mod[CRT_MAX_MODULI];
     double inv[CRT_MAX_MODULI];     // 1/m rounded
-    int k22[CRT_MAX_MODULI];        // 2^22 mod m, symmetric
-    int k44[CRT_MAX_MODULI];        // 2^44 mod m, symmetric
+    int kdig[CRT_MAX_MODULI][3];    // 2^(22c+8i) mod m (c, i < 3), symmetric, as packed int8 (see crt_pack)
     float invf[CRT_MAX_MODULI];     // 1/m in FP32
     int pad2_ = 0;
     double hi_lim[CRT_MAX_MODULI];  // symmetric residue range [lo_lim, hi_lim]

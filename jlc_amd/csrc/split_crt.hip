@@ -168,34 +168,58 @@ __global__ __launch_bounds__(256) void crt_norm_cols_finish(const NormPart* __re
     norm_finish(S, cnt, e[r], spread[r], &t[r], &sig[r]);
 }
 
-// Residues of 16 integer images s = c2*2^44 + c1*2^22 + c0 (22-bit signed
-// chunks, |s| < 2^66) packed as bytes. t = c2*(2^44 mod m) + c1*(2^22 mod m) + c0
-// is exact in INT32 (|t| < 2^31) and congruent to s; it is reduced with an
-// FP32 quotient estimate (|t/m| < 2^23, three FP32 roundings: the estimate is
-// within 1.62 of t/m) and two correction rounds.
-__device__ __forceinline__ v4i residues16(const int (&c2)[16], const int (&c1)[16], const int (&c0)[16],
+// Residues through signed byte digits. An integer image |s| < 2^66 is split
+// into three signed 22-bit chunks (s = c2 2^44 + c1 2^22 + c0, each with the
+// sign of s) and each chunk into balanced base-256 digits (c = d0 + d1 2^8 +
+// d2 2^16, |d| <= 128), nine digits in all, packed four to a register. Then
+// x = sum_d digit * (2^weight mod m) is three v_dot4 int8 dot products, exact
+// with |x| <= 9 * 128 * 128 < 2^18, and congruent to s mod m. x is exact in
+// FP32 and |x / m| < 2^11, so rint(x * fl(1/m)) is rint(x / m) (the quotient
+// error is below 2^-12, under the 1/(2m) distance of x/m from any half
+// integer for odd m; m = 256 divides exactly), and r = x - q m is the
+// symmetric residue; for m = 256 r = +128 stores the byte of -128.
+__device__ __forceinline__ void balanced_bytes(int c, int& d0, int& d1, int& d2) {
+    d0 = ((c + 128) & 255) - 128;
+    c = (c - d0) >> 8;
+    d1 = ((c + 128) & 255) - 128;
+    d2 = (c - d1) >> 8;
+}
+
+__device__ __forceinline__ int byte4(int a, int b, int c, int d) {
+    return (int)((unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) |
+                 ((unsigned)(d & 255) << 24));
+}
+
+// Digits of one image, packed: D0 = (c0: d0 d1 d2, c1: d0), D1 = (c1: d1 d2, c2: d0 d1), D2 = (c2: d2).
+__device__ __forceinline__ void image_digits(int c2, int c1, int c0, int& D0, int& D1, int& D2) {
+    int a0, a1, a2, b0, b1, b2, e0, e1, e2;
+    balanced_bytes(c0, a0, a1, a2);
+    balanced_bytes(c1, b0, b1, b2);
+    balanced_bytes(c2, e0, e1, e2);
+    D0 = byte4(a0, a1, a2, b0);
+    D1 = byte4(b1, b2, e0, e1);
+    D2 = e2 & 255;
+}
+
+__device__ __forceinline__ v4i residues16(const int (&D0)[16], const int (&D1)[16], const int (&D2)[16],
                                           const CrtTables& T, int t) {
-    const int m = (int)T.mod[t], k44 = T.k44[t], k22 = T.k22[t];
-    const int hl = (int)T.hi_lim[t], ll = (int)T.lo_lim[t];
+    const int K0 = T.kdig[t][0], K1 = T.kdig[t][1], K2 = T.kdig[t][2];
+    const float mf = (float)T.mod[t];
     const float invf = T.invf[t];
-    unsigned char q[16];
+    int q[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-        // Every factor fits 24 signed bits (|c| < 2^22, |k| <= 128, |qq| < 2^23, m <= 256): full-rate multiplies.
-        const int x = __mul24(c2[j], k44) + __mul24(c1[j], k22) + c0[j];
-        const int qq = (int)rintf((float)x * invf);
-        int r = x - __mul24(qq, m);
-        r = r > hl ? r - m : r;
-        r = r < ll ? r + m : r;
-        r = r > hl ? r - m : r;
-        r = r < ll ? r + m : r;
-        q[j] = (unsigned char)(signed char)r;
+        const int x = __builtin_amdgcn_sdot4(D2[j], K2, __builtin_amdgcn_sdot4(D1[j], K1,
+                                             __builtin_amdgcn_sdot4(D0[j], K0, 0, false), false), false);
+        const float xf = (float)x;
+        // x - q m is an integer below 2^18: exact in the FMA
+        q[j] = (int)fmaf(-rintf(xf * invf), mf, xf);
     }
     v4i w;
-    w.x = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
-    w.y = q[4] | (q[5] << 8) | (q[6] << 16) | (q[7] << 24);
-    w.z = q[8] | (q[9] << 8) | (q[10] << 16) | (q[11] << 24);
-    w.w = q[12] | (q[13] << 8) | (q[14] << 16) | (q[15] << 24);
+    w.x = byte4(q[0], q[1], q[2], q[3]);
+    w.y = byte4(q[4], q[5], q[6], q[7]);
+    w.z = byte4(q[8], q[9], q[10], q[11]);
+    w.w = byte4(q[12], q[13], q[14], q[15]);
     return w;
 }
 
@@ -235,7 +259,7 @@ __global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, in
             xv[j] = X[rowoff + (int64_t)(k < Kd ? k : Kd - 1) * sk];
         }
     }
-    int c2[16], c1[16], c0[16];
+    int D0[16], D1[16], D2[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int k = k0 + j;
@@ -246,12 +270,10 @@ __global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, in
         const double h = trunc(s * 0x1p-44);
         const double rest = fma(-h, 0x1p44, s);
         const double mid = trunc(rest * 0x1p-22);
-        c2[j] = (int)h;
-        c1[j] = (int)mid;
-        c0[j] = (int)fma(-mid, 0x1p22, rest);
+        image_digits((int)h, (int)mid, (int)fma(-mid, 0x1p22, rest), D0[j], D1[j], D2[j]);
     }
     const int64_t off = (int64_t)blk * 64 + lane;
-    for (int t = 0; t < T.nmod; t++) out[t * plane_stride + off] = residues16(c2, c1, c0, T, t);
+    for (int t = 0; t < T.nmod; t++) out[t * plane_stride + off] = residues16(D0, D1, D2, T, t);
 }
 
 // Column strips of `group` tile rows (256 x 256 tiles), planes in blockIdx.y.
@@ -688,8 +710,17 @@ void crt_tables(int nmod, CrtTables* T) {
             int c = (int)(v % m);
             return c > (m - 1) / 2 ? c - m : c;
         };
-        T->k22[t] = sym(1LL << 22);
-        T->k44[t] = sym(1LL << 44);
+        // digit weights 2^(22c + 8i), packed in the order of image_digits
+        int kw[9];
+        for (int c = 0; c < 3; c++)
+            for (int i = 0; i < 3; i++) kw[3 * c + i] = sym(1LL << (22 * c + 8 * i));
+        auto pk = [](int a, int b, int c, int d) {
+            return (int)((uint32_t)(a & 255) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16) |
+                         ((uint32_t)(d & 255) << 24));
+        };
+        T->kdig[t][0] = pk(kw[0], kw[1], kw[2], kw[3]);
+        T->kdig[t][1] = pk(kw[4], kw[5], kw[6], kw[7]);
+        T->kdig[t][2] = pk(kw[8], 0, 0, 0);
         T->invf[t] = 1.0f / (float)m;
         Big Mi;
         Mi.w[0] = 1;
