@@ -549,38 +549,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         }
 }
 
+// 40-bit limbs that hold M = prod of the first NM moduli (M < 2^(40 L):
+// log2 M is 39.9, 79.2, 117.8 and 155.4 for 5, 10, 15 and 20 moduli); the
+// limbs of M and of every W_t at and above L are zero, so the reconstruction
+// over L limbs gives the same bits as over four.
+constexpr int crt_limbs(int nm) { return nm <= 5 ? 1 : nm <= 10 ? 2 : nm <= 15 ? 3 : 4; }
+
 // CRT reconstruction of one residue vector (exact up to the final rounding).
 template <int NM>
 __device__ __forceinline__ double crt_value(const int (&r)[NM], const CrtTables& T) {
-    double X[4];
+    constexpr int L = crt_limbs(NM);
+    double X[L];
 #pragma unroll
-    for (int j = 0; j < 4; j++) X[j] = 0.0;
+    for (int j = 0; j < L; j++) X[j] = 0.0;
     // Every product and partial sum here is an integer below 2^53, so the fused
     // form is exact and equal to the separate one.
 #pragma unroll
     for (int t = 0; t < NM; t++) {
         const double rt = (double)r[t];
 #pragma unroll
-        for (int j = 0; j < 4; j++) X[j] = fma(rt, T.w[t][j], X[j]);
+        for (int j = 0; j < L; j++) X[j] = fma(rt, T.w[t][j], X[j]);
     }
     // X/M lies within 1/4 of the integer q, so the rounding of this estimate cannot change q.
     double qf = 0.0;
 #pragma unroll
-    for (int j = 3; j >= 0; j--) qf = fma(X[j], T.finv[j], qf);
+    for (int j = L - 1; j >= 0; j--) qf = fma(X[j], T.finv[j], qf);
     const double q = rint(qf);
-    double D[4];
+    double D[L];
 #pragma unroll
-    for (int j = 0; j < 4; j++) D[j] = fma(-q, T.mlimb[j], X[j]);
+    for (int j = 0; j < L; j++) D[j] = fma(-q, T.mlimb[j], X[j]);
     const double base = 0x1p40;
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
+    for (int j = 0; j < L - 1; j++) {
         const double c = rint(D[j] * 0x1p-40);
         D[j] = D[j] - c * base;
         D[j + 1] = D[j + 1] + c;
     }
-    double h = D[3], l = 0.0;
+    double h = D[L - 1], l = 0.0;
 #pragma unroll
-    for (int j = 2; j >= 0; j--) {
+    for (int j = L - 2; j >= 0; j--) {
         const double hb = h * base;
         const double s = hb + D[j];
         const double bb = s - hb;

@@ -53,7 +53,7 @@ def _case(rng, la, lb, m, n, k, nmod, alpha, beta, phi=1.0, pad=0, off=0):
     return st
 
 
-@pytest.mark.parametrize("nmod", [4, 8, 12, 16, 20])
+@pytest.mark.parametrize("nmod", [4, 5, 6, 8, 10, 11, 12, 15, 16, 20])
 def test_bitwise_equal_to_restatement(nmod):
     rng = np.random.default_rng(300 + nmod)
     _case(rng, 0, 0, 256, 256, 64, nmod, 1.0, 0.0)
