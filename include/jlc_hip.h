@@ -48,10 +48,12 @@ extern "C" {
 
 /* Arithmetic modes. The low byte selects the form; for the split forms bits
  * 8..15 carry the slice count, or for the CRT form the moduli count (0 = choose
- * from the inputs' exponent spread). JLC_MODE_SPLIT_CRT: modular INT8 products
- * recombined by the Chinese remainder theorem (2..20 moduli); in automatic
- * mode a spread too wide for 20 moduli runs the FP64 MFMA kernel instead
- * (jlc_stats.kernel says which ran). */
+ * from the inputs' exponent spread). JLC_MODE_SPLIT_CRT: rows of op(A) and
+ * columns of op(B) scaled by powers of two at their 2-norms, modular INT8
+ * products recombined by the Chinese remainder theorem (2..20 moduli, k below
+ * 2^22). In automatic mode a spread that needs more bits than 20 moduli hold
+ * (CRT), or more slices than the form has (9 INT8, 7 BF16), runs the FP64 MFMA
+ * kernel instead (jlc_stats.kernel says which ran). */
 #define JLC_MODE_FP64 0
 #define JLC_MODE_SPLIT_INT8 1
 #define JLC_MODE_SPLIT_BF16 2
@@ -78,8 +80,8 @@ typedef struct jlc_stats {
     int32_t tile_k;
     int32_t slices;     /* split-precision slice count / CRT moduli count (0 for FP64) */
     int32_t launches;   /* kernel launches issued */
-    int32_t bits_a;     /* CRT form: integer bits of op(A) rows (0 otherwise) */
-    int32_t bits_b;     /* CRT form: integer bits of op(B) columns */
+    int32_t bits_a;     /* CRT form: norm bits of op(A) rows, ||a'|| <= 2^bits_a + sqrt(k)/2 (0 otherwise) */
+    int32_t bits_b;     /* CRT form: norm bits of op(B) columns */
     double main_ms;     /* device time of the dominant kernel alone (the GEMM / GEMV launch) */
 } jlc_stats;
 
