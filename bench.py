@@ -156,6 +156,22 @@ def bench_batched(args) -> int:
     per_ms = (time.perf_counter() - t0) * 1e3
     log(f"per-call launches: {per_ms:.1f} ms")
     gbs = w.bytes / (dev_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the reference's CPU path per problem: MicroKernel restatement on one thread, one call per problem
+        from oracle import oracle as orc
+        ncpu = min(2000, w.count)
+        t0 = time.perf_counter()
+        for i in range(ncpu):
+            m, n, k = (int(x) for x in w.dims[i])
+            A, B, C0, _ = w.problem(i)
+            c = C0.copy()
+            orc.parallel_micro(m, n, k, float(w.alpha[i]), A, k, B, n, float(w.beta[i]), c, n, 1)
+        sec = time.perf_counter() - t0
+        cpu = {"value": round(ncpu / sec, 1), "unit": "problems/s", "cores": 1, "kind": "port",
+               "sample": f"first {ncpu} problems, PARALLEL_MICRO restatement (oracle/oracle.c) on 1 thread, "
+                         f"one call per problem from Python ({sec:.2f} s)"}
+        log(f"CPU restatement: {cpu['value']} problems/s")
     print(json.dumps({
         "metric": "batched uneven FP64 GEMM, grouped persistent launch (graph replay)",
         "value": round(w.count / (wall_ms * 1e-3), 1), "unit": "problems/s", "n_gpus": 1, "steps": args.steps,
@@ -168,7 +184,7 @@ def bench_batched(args) -> int:
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "grouped_f64",
                      "kernel_ms": round(dev_ms, 4)},
         "per_call_ms": round(per_ms, 2), "speedup_vs_per_call": round(per_ms / wall_ms, 1),
-        "plan_build_ms": round(plan_ms, 1),
+        "plan_build_ms": round(plan_ms, 1), "cpu_baseline": cpu,
     }), flush=True)
     return 0
 
@@ -236,6 +252,18 @@ def bench_layouts(args) -> int:
                                             "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                                             "traffic": read_traffic("dgemv_32768")}}
     log(f"matvec_32768: {dev_ms:.3f} ms device, {gbs:.0f} GB/s, componentwise err {comp:.2f} u")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        y = np.zeros(mv)
+        t0 = time.perf_counter()
+        orc.matvec(mv, mv, 1.0, A, mv, x, 0.0, y)
+        sec = time.perf_counter() - t0
+        results["matvec_32768"]["cpu_baseline"] = {
+            "value": round(algo_bytes / sec / 1e9, 2), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"the same 32768^2 matvec, SpecializedKernels.matvec restatement, 1 thread ({sec:.2f} s)"}
+        cpu = cpu_baseline()
+        log(f"CPU: matvec {results['matvec_32768']['cpu_baseline']['value']} GB/s, GEMM {cpu['value']} TF")
     head = results["rowmajor_8192"]
     print(json.dumps({
         "metric": "FP64 MFMA DGEMM TFLOP/s across layouts (8192^3 row-major headline) + matvec GB/s",
@@ -246,7 +274,7 @@ def bench_layouts(args) -> int:
                                "matvec 32768^2; uniform[-1,1], alpha=1, beta=0"},
         "roofline": {"bound": "mfma", "achieved": head["tflops"], "peak": FP64_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": head["frac_of_fp64_peak"], "traffic": None},
-        "cases": results}), flush=True)
+        "cpu_baseline": cpu, "cases": results}), flush=True)
     return 0
 
 
