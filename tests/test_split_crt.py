@@ -174,3 +174,35 @@ def test_depth_beyond_norm_sum_limit():
     assert st.kernel_name == "mfma_f64"
     with pytest.raises(native.JlcError):
         native.dgemm(0, 0, m, n, k, 1.0, a, k, b, n, 0.0, c, n, native.mode_slices(native.MODE_SPLIT_CRT, 16))
+
+
+def test_extreme_magnitudes_norm_scaling():
+    """Rows and columns near the ends of the FP64 range, mixed within a row, subnormal
+    entries, all-zero rows and a lone non-zero: the norm exponents (integer sums of
+    scaled squares) and the whole product stay bit-identical to the restatement."""
+    rng = np.random.default_rng(77)
+    m, n, k = 70, 66, 45
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (k, n))
+    A[0] *= 2.0 ** 900
+    A[1] *= 2.0 ** -900
+    A[2, :] = 0.0
+    A[3, :] = 0.0
+    A[3, 7] = -3.5
+    A[4, ::2] *= 2.0 ** 600      # one row spanning ~2^1200: its small half underflows in the scaled squares
+    A[5, :5] = 5e-324            # subnormals
+    A[5, 5:] = 0.0
+    B[:, 0] *= 2.0 ** -1000
+    B[:, 1] *= 2.0 ** 1000
+    B[:, 2] = 0.0
+    a, b = A.ravel().copy(), B.ravel().copy()
+    for nmod in (8, 16, 20):
+        c = np.zeros(m * n)
+        st = native.dgemm(0, 0, m, n, k, 1.0, a, k, b, n, 0.0, c, n, native.mode_slices(native.MODE_SPLIT_CRT, nmod))
+        assert st.kernel_name == "split_crt"
+        exp = np.zeros(m * n)
+        orc.crt_gemm(0, 0, m, n, k, 1.0, a, k, b, n, 0.0, exp, n, st.slices, st.bits_a, st.bits_b)
+        same = (c == exp) | (np.isnan(c) & np.isnan(exp))
+        assert same.all(), (nmod, int((~same).sum()))
+        cm = c.reshape(m, n)
+        assert np.all(cm[2] == 0.0) and np.all(cm[:, 2] == 0.0)
