@@ -49,12 +49,28 @@ __global__ __launch_bounds__(64 * WAVES_PER_GROUP) void grouped_f64(const Groupe
     const int nwaves = gridDim.x * WAVES_PER_GROUP;
     const int r16 = lane & 15;
     const int g = lane >> 4;
+    // The next tile's list entry is fetched while the current tile runs, and C (when beta != 0) is read
+    // before the K loop instead of after it: per tile, only the descriptor load stays serial.
+    uint64_t e_next = wave < ntiles ? tiles[wave] : 0;
     for (int t = wave; t < ntiles; t += nwaves) {
-        const uint64_t e = tiles[t];
+        const uint64_t e = e_next;
+        if (t + nwaves < ntiles) e_next = tiles[t + nwaves];
         const int p = __builtin_amdgcn_readfirstlane((int)(e >> 32));
         const int tm = (int)((e >> 16) & 0xffff), tn = (int)(e & 0xffff);
         const GroupedDesc d = descs[p];
         const int i0 = tm * 32, j0 = tn * 32;
+        double cold[2][2][4];
+        if (d.beta != 0.0) {
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = i0 + 16 * i + g + 4 * r, col = j0 + 16 * j + r16;
+                        cold[i][j][r] = (row < d.m && col < d.n) ? d.c[(int64_t)row * d.ldc + col] : 0.0;
+                    }
+        }
         d4 acc[2][2];
 #pragma unroll
         for (int i = 0; i < 2; i++)
@@ -104,9 +120,8 @@ __global__ __launch_bounds__(64 * WAVES_PER_GROUP) void grouped_f64(const Groupe
                 for (int r = 0; r < 4; r++) {
                     const int row = i0 + 16 * i + g + 4 * r;
                     if (row < d.m) {
-                        double* c = d.c + (int64_t)row * d.ldc + col;
                         const double v = d.alpha * acc[i][j][r];
-                        *c = d.beta == 0.0 ? v : v + d.beta * *c;
+                        d.c[(int64_t)row * d.ldc + col] = d.beta == 0.0 ? v : v + d.beta * cold[i][j][r];
                     }
                 }
             }
