@@ -22,11 +22,12 @@
 //     C' = A'B' in the accumulator's own lane order (1 KiB per 32 x 32 block).
 //     Unlike the slice form there is no operand reuse between moduli, so the
 //     GEMM is a plain large-tile INT8 GEMM: 256 x 256 C tile, 8 wavefronts of
-//     128 x 64, K staged 64 deep through a three-buffer LDS ring (96 KiB).
+//     128 x 64, K staged 64 deep through two LDS buffers, global loads four
+//     stages ahead (crt_gemm_i8).
 //  4. Reconstruction. With M = prod m_t > 4 max ||a'_i|| ||b'_j||, C' is the unique
 //     integer in (-M/2, M/2) with those residues. X = sum_t r_t W_t
-//     (W_t = (M/m_t) * ((M/m_t)^-1 mod m_t)) is formed exactly in four 40-bit
-//     limbs (every partial sum is an integer below 2^53), q = rint(X/M),
+//     (W_t = (M/m_t) * ((M/m_t)^-1 mod m_t)) is formed exactly in the 40-bit
+//     limbs M needs (every partial sum is an integer below 2^53), q = rint(X/M),
 //     C' = X - qM limb by limb (exact fma), the limbs are balanced and summed
 //     from the top in double-double and rounded once; then
 //     C = alpha * C' * 2^(ta+tb-pa-pb) (+ beta*C through fma).
