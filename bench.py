@@ -148,13 +148,15 @@ def bench_batched(args) -> int:
         for d in descs:
             native.dgemm_dev(d.layout_a, d.layout_b, d.m, d.n, d.k, d.alpha, d.a, d.lda, d.b, d.ldb, d.beta, d.c,
                              d.ldc, native.MODE_FP64, stream, stats=False)
-    per_call()
-    native.synchronize()
-    t0 = time.perf_counter()
-    per_call()
-    native.synchronize()
-    per_ms = (time.perf_counter() - t0) * 1e3
-    log(f"per-call launches: {per_ms:.1f} ms")
+    per_ms = None
+    if not args.no_per_call:
+        per_call()
+        native.synchronize()
+        t0 = time.perf_counter()
+        per_call()
+        native.synchronize()
+        per_ms = (time.perf_counter() - t0) * 1e3
+        log(f"per-call launches: {per_ms:.1f} ms")
     gbs = w.bytes / (dev_ms * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu_baseline:
@@ -186,7 +188,8 @@ def bench_batched(args) -> int:
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "grouped_f64",
                      "kernel_ms": round(dev_ms, 4)},
-        "per_call_ms": round(per_ms, 2), "speedup_vs_per_call": round(per_ms / wall_ms, 1),
+        "per_call_ms": None if per_ms is None else round(per_ms, 2),
+        "speedup_vs_per_call": None if per_ms is None else round(per_ms / wall_ms, 1),
         "plan_build_ms": round(plan_ms, 1), "cpu_baseline": cpu,
     }), flush=True)
     return 0
@@ -440,6 +443,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched"], default="dgemm",
                     help="dgemm: the 16384^3 metric workload; batched: the grouped-GEMM configuration")
     ap.add_argument("--count", type=int, default=100000, help="problems in the batched workload")
+    ap.add_argument("--no-per-call", action="store_true",
+                    help="batched workload: skip the one-launch-per-problem comparison (counter runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel over MIB MiB three times (byte-counter calibration)")
     args = ap.parse_args()
