@@ -400,8 +400,21 @@ int crt_choose(double sig_a, double sig_b, int64_t k, int* na, int* nb) {
     *na = std::max(2, (int)std::ceil(base + tt.crt_bits_slope * sig_a));
     *nb = std::max(2, (int)std::ceil(base + tt.crt_bits_slope * sig_b));
     if (*na > jlc::CRT_MAX_BITS || *nb > jlc::CRT_MAX_BITS) return 0;
-    for (int nm = jlc::CRT_MIN_MODULI; nm <= jlc::CRT_MAX_MODULI; nm++)
-        if (crt_fits(*na, *nb, k, nm)) return nm;
+    for (int nm = jlc::CRT_MIN_MODULI; nm <= jlc::CRT_MAX_MODULI; nm++) {
+        if (!crt_fits(*na, *nb, k, nm)) continue;
+        // The products cost the same whatever bits the moduli hold: spend what is left of their
+        // capacity on more bits, the narrower operand first.
+        for (;;) {
+            int* lo = *na <= *nb ? na : nb;
+            if (*lo >= jlc::CRT_MAX_BITS) break;
+            ++*lo;
+            if (!crt_fits(*na, *nb, k, nm)) {
+                --*lo;
+                break;
+            }
+        }
+        return nm;
+    }
     return 0;
 }
 

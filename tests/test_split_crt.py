@@ -74,6 +74,9 @@ def test_automatic_moduli_count():
     # the fewest moduli that hold (2^pa + sqrt(k)/2)(2^pb + sqrt(k)/2) with the M/4 margin
     assert orc.crt_fits(st.bits_a, st.bits_b, 1000, st.slices)
     assert not orc.crt_fits(st.bits_a, st.bits_b, 1000, st.slices - 1)
+    # and the moduli's leftover capacity is spent on bits: one more for the narrower operand no longer fits
+    more = (st.bits_a + 1, st.bits_b) if st.bits_a <= st.bits_b else (st.bits_a, st.bits_b + 1)
+    assert not orc.crt_fits(*more, 1000, st.slices)
 
 
 def test_modular_fold_beyond_65536_k():
