@@ -209,3 +209,19 @@ def test_extreme_magnitudes_norm_scaling():
         assert same.all(), (nmod, int((~same).sum()))
         cm = c.reshape(m, n)
         assert np.all(cm[2] == 0.0) and np.all(cm[:, 2] == 0.0)
+
+
+@pytest.mark.parametrize("mode", [native.MODE_SPLIT_CRT, native.MODE_SPLIT_INT8, native.MODE_SPLIT_BF16])
+def test_beta_zero_never_reads_c(mode):
+    """BLAS semantics kept by the split forms (GemmNumericEdgeTest's beta = 0 case): with beta = 0 the
+    old C is not read, so NaN or Inf in it cannot reach the result."""
+    rng = np.random.default_rng(12)
+    m, n, k = 65, 47, 33
+    A = rng.uniform(-1, 1, m * k)
+    B = rng.uniform(-1, 1, k * n)
+    c = np.full(m * n, np.nan)
+    c[::7] = np.inf
+    native.dgemm(0, 0, m, n, k, 1.25, A, k, B, n, 0.0, c, n, mode)
+    assert np.all(np.isfinite(c))
+    ref = 1.25 * (A.reshape(m, k) @ B.reshape(k, n))
+    assert np.max(np.abs(c.reshape(m, n) - ref)) <= 1e-12 * np.max(np.abs(ref))
