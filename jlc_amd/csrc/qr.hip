@@ -236,34 +236,73 @@ __global__ __launch_bounds__(256) void build_vc(const double* __restrict__ at, i
     vc[idx] = v;
 }
 
-// T (nb x nb row-major, upper triangular) from G = V^T V and tau:
-// T(i,i) = tau_i, T(0:i, i) = T(0:i, 0:i) * (-tau_i * G(0:i, i)) (buildT, HouseholderQR.java:296-320).
-// T is built in one nb x nb LDS array (up to 128 KiB): its upper triangle is
-// T, its diagonal tau, and its strict lower triangle holds y_i = -tau_i G(0:i, i)
-// transposed (row i, columns c < i), all loaded up front. Column i's entries
-// T(r, i), r < i, are row-times-vector products over c in [r, i): two threads
-// per row take alternate c from LDS and one lane swap adds the halves; one
-// barrier per column.
+// T (nb x nb row-major, upper triangular) from G = V^T V and tau, with T(i,i) = tau_i and
+// I - V T V^T = H_1 ... H_nb (buildT, HouseholderQR.java:296-320, builds the same T column by column:
+// T(0:i, i) = T(0:i, 0:i) * (-tau_i * G(0:i, i))). Here by recursive doubling: for adjacent blocks
+// [s, s+b) and [s+b, s+b+b2) whose own T11, T22 are done, T12 = -T11 * G12 * T22 (Schreiber and
+// Van Loan's compact-WY merge), all block pairs of one level at once; log2(nb) levels instead of nb
+// dependent column steps. The nb x nb LDS array holds T in its upper triangle (diagonal tau) and G in
+// its strict lower triangle; each level computes X = G12 T22 into registers, stores it in T12's place,
+// then T12 = -T11 X the same way (at most nb^2/4 entries a level: 16 per thread).
 __global__ __launch_bounds__(256) void build_t(const double* __restrict__ g, int nb, const double* __restrict__ tau,
                                                double* __restrict__ T) {
     extern __shared__ double S[];  // nb * nb
     for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int r = idx / nb, c = idx % nb;  // g row r, column c: y_c[r] for r < c
-        if (r < c) S[c * nb + r] = -tau[c] * g[idx];
+        const int r = idx / nb, c = idx % nb;
+        if (r < c) S[c * nb + r] = g[idx];  // G(r, c) in the lower triangle
         else if (r == c) S[idx] = tau[r];
     }
     __syncthreads();
-    const int r = threadIdx.x >> 1, h = threadIdx.x & 1;  // nb <= 128: one thread pair per row
-    for (int i = 1; i < nb; i++) {
-        double s = 0.0;
-        if (r < i) {
-            const double* tr = S + r * nb;
-            const double* yi = S + i * nb;
-#pragma unroll 4
-            for (int c = r + h; c < i; c += 2) s += tr[c] * yi[c];
+    for (int b = 1; b < nb; b *= 2) {
+        const int total = ((nb + 2 * b - 1) / (2 * b)) * b * b;  // block pairs x b x b entries
+        double val[16];
+        // X = G12 * T22: X(i, j) = sum_{l <= j} G(s+i, s+b+l) T(s+b+l, s+b+j)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int e = threadIdx.x + 256 * q;
+            double x = 0.0;
+            if (e < total) {
+                const int pr = e / (b * b), rem = e % (b * b), i = rem / b, j = rem % b;
+                const int s0 = 2 * b * pr, b2 = min(b, nb - s0 - b);
+                if (j < b2)
+                    for (int l = 0; l <= j; l++) x += S[(s0 + b + l) * nb + s0 + i] * S[(s0 + b + l) * nb + s0 + b + j];
+            }
+            val[q] = x;
         }
-        s += __shfl_xor(s, 1, 64);
-        if (r < i && h == 0) S[r * nb + i] = s;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e < total) {
+                const int pr = e / (b * b), rem = e % (b * b), i = rem / b, j = rem % b;
+                const int s0 = 2 * b * pr, b2 = min(b, nb - s0 - b);
+                if (j < b2) S[(s0 + i) * nb + s0 + b + j] = val[q];
+            }
+        }
+        __syncthreads();
+        // T12 = -T11 * X: T12(i, j) = -sum_{l >= i} T(s+i, s+l) X(l, j)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int e = threadIdx.x + 256 * q;
+            double x = 0.0;
+            if (e < total) {
+                const int pr = e / (b * b), rem = e % (b * b), i = rem / b, j = rem % b;
+                const int s0 = 2 * b * pr, b2 = min(b, nb - s0 - b);
+                if (j < b2)
+                    for (int l = i; l < b; l++) x += S[(s0 + i) * nb + s0 + l] * S[(s0 + l) * nb + s0 + b + j];
+            }
+            val[q] = -x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e < total) {
+                const int pr = e / (b * b), rem = e % (b * b), i = rem / b, j = rem % b;
+                const int s0 = 2 * b * pr, b2 = min(b, nb - s0 - b);
+                if (j < b2) S[(s0 + i) * nb + s0 + b + j] = val[q];
+            }
+        }
         __syncthreads();
     }
     for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
@@ -422,6 +461,8 @@ hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb,
 }
 
 hipError_t launch_build_t(const double* g, int nb, const double* tau, double* T, hipStream_t stream) {
+    // Each doubling level holds at most 16 entries per thread: nb <= 128 keeps every level within 4096.
+    if (nb < 1 || nb > 128) return hipErrorInvalidValue;
     const size_t lds = (size_t)nb * nb * sizeof(double);
     (void)hipFuncSetAttribute((const void*)build_t, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(build_t, dim3(1), dim3(256), lds, stream, g, nb, tau, T);
