@@ -87,6 +87,9 @@ constexpr int SPLIT_MAX_SLICES = 9;
 
 void split_plan_sizes(int64_t m, int64_t n, int64_t k, int slices, SplitPlan* p);
 // Row exponents / spreads of op(A) and column exponents / spreads of op(B).
+// Statistics of one operand's rows (kcontig: element (r, k) at x[r*stride + k], else x[k*stride + r]).
+hipError_t launch_operand_stats(bool kcontig, const double* x, int64_t stride, int R, int K, double* part, int* e,
+                                float* spread, hipStream_t stream);
 hipError_t launch_split_stats(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
                               SplitPlan* p, hipStream_t stream);
 // Slices both operands and runs the slice-product GEMM with FP64 recombination into C.
@@ -169,11 +172,12 @@ struct CrtPlan {
 };
 
 double crt_log2_product(int nmod);
-// Norm exponents ta/tb (2^t at or above each row / column 2-norm, from the
-// split statistics' max exponents) and the chooser statistic sig per row /
+// Split statistics (sp.ea/eb, sp.spread_a/b), norm exponents ta/tb (2^t at or
+// above each row / column 2-norm) and the chooser statistic sig per row /
 // column; uses sp.stats_part as scratch.
-hipError_t launch_crt_norms(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
-                            const SplitPlan& sp, int* ta, int* tb, float* sig_a, float* sig_b, hipStream_t stream);
+hipError_t launch_crt_stats_norms(int layout_a, int layout_b, const double* a, int64_t lda, const double* b,
+                                  int64_t ldb, const SplitPlan& sp, int* ta, int* tb, float* sig_a, float* sig_b,
+                                  hipStream_t stream);
 void crt_tables(int nmod, CrtTables* T);
 void crt_plan_sizes(int64_t m, int64_t n, int64_t k, int nmod, CrtPlan* p);
 hipError_t launch_crt(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,

@@ -453,8 +453,7 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
     float* sig_a = reinterpret_cast<float*>(tb + p.npad);
     float* sig_b = sig_a + p.mpad;
     sp.stats_part = reinterpret_cast<double*>(sb + stat_bytes);
-    JLC_HIP_CHECK(jlc::launch_split_stats(la, lb, a, lda, b, ldb, &sp, stream));
-    JLC_HIP_CHECK(jlc::launch_crt_norms(la, lb, a, lda, b, ldb, sp, ta, tb, sig_a, sig_b, stream));
+    JLC_HIP_CHECK(jlc::launch_crt_stats_norms(la, lb, a, lda, b, ldb, sp, ta, tb, sig_a, sig_b, stream));
     int pa, pb;
     if (nmod == 0) {
         std::vector<float> host((size_t)(p.mpad + p.npad));

@@ -36,6 +36,7 @@
 #include <math.h>
 
 #include "internal.h"
+#include "split_stats.h"
 
 // The reconstruction must round exactly as written: no contraction of a*b+c.
 #pragma clang fp contract(off)
@@ -46,7 +47,7 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int NONFINITE = -100000;
+constexpr int NONFINITE = STAT_NONFINITE;
 constexpr int CT = 256;        // C tile rows and columns
 constexpr int KSTAGE = 64;     // K per LDS stage (two 32-deep MFMA steps)
 constexpr int KPAD = 256;      // K padding: a multiple of four stages
@@ -169,16 +170,114 @@ __global__ __launch_bounds__(256) void crt_norm_cols_finish(const NormPart* __re
     norm_finish(S, cnt, e[r], spread[r], &t[r], &sig[r]);
 }
 
+// Statistics and norm exponent of k-contiguous rows in one read of X: one
+// workgroup per row, the row held in registers (Kd <= ROW_THREADS * ROW_REG). The
+// same per-entry statistics as the split forms' row statistics (max, log2
+// sum, count, Inf/NaN) give e and spread, then the integer norm units of the
+// same registers give t and sig; the max and the integer sums do not depend on
+// the reduction order, so e and t are those of the two-pass kernels.
+constexpr int ROW_THREADS = 512, ROW_WAVES = ROW_THREADS / 64, ROW_REG = 32;
+
+__global__ __launch_bounds__(ROW_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void crt_row_stats_norms(const double* __restrict__ X, int64_t sr, int Kd,
+                                                           int* __restrict__ e, float* __restrict__ spread,
+                                                           int* __restrict__ t, float* __restrict__ sig) {
+    __shared__ double smx[ROW_WAVES], sslog[ROW_WAVES], scnt[ROW_WAVES];
+    __shared__ int sbad[ROW_WAVES];
+    __shared__ unsigned long long sS[ROW_WAVES];
+    __shared__ unsigned int sc[ROW_WAVES];
+    const int r = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double* row = X + (int64_t)r * sr;
+    double v[ROW_REG];
+#pragma unroll
+    for (int j = 0; j < ROW_REG; j++) {
+        const int k = threadIdx.x + ROW_THREADS * j;
+        v[j] = k < Kd ? row[k] : 0.0;
+    }
+    double mx = 0.0, slog = 0.0, cnt = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < ROW_REG; j++) stat_add(fabs(v[j]), mx, slog, cnt, bad);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, off, 64));
+        slog += __shfl_xor(slog, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
+        bad |= __shfl_xor((int)bad, off, 64) != 0;
+    }
+    if (lane == 0) {
+        smx[w] = mx;
+        sslog[w] = slog;
+        scnt[w] = cnt;
+        sbad[w] = bad;
+    }
+    __syncthreads();
+    mx = smx[0];
+    slog = sslog[0];
+    cnt = scnt[0];
+    int anybad = sbad[0];
+#pragma unroll
+    for (int i = 1; i < ROW_WAVES; i++) {
+        mx = fmax(mx, smx[i]);
+        slog += sslog[i];
+        cnt += scnt[i];
+        anybad |= sbad[i];
+    }
+    bad = anybad != 0;
+    int ex = 0;
+    if (mx > 0.0) frexp(mx, &ex);
+    if (bad) ex = NONFINITE;
+    unsigned long long S = 0ull;
+    unsigned int n = 0u;
+    if (ex != NONFINITE) {
+#pragma unroll
+        for (int j = 0; j < ROW_REG; j++) {
+            const unsigned long long u = norm_units(v[j], ex);
+            S += u;
+            n += u != 0ull;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        S += __shfl_xor(S, off, 64);
+        n += __shfl_xor(n, off, 64);
+    }
+    if (lane == 0) {
+        sS[w] = S;
+        sc[w] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        S = 0ull;
+        n = 0u;
+#pragma unroll
+        for (int i = 0; i < ROW_WAVES; i++) {
+            S += sS[i];
+            n += sc[i];
+        }
+        const float sp = (bad || cnt == 0.0) ? 0.0f : (float)(ex - slog / cnt);
+        e[r] = ex;
+        spread[r] = sp;
+        norm_finish(S, n, ex, sp, &t[r], &sig[r]);
+    }
+}
+
 // Residues through signed byte digits. An integer image |s| < 2^66 is split
 // into three signed 22-bit chunks (s = c2 2^44 + c1 2^22 + c0, each with the
 // sign of s) and each chunk into balanced base-256 digits (c = d0 + d1 2^8 +
-// d2 2^16, |d| <= 128), nine digits in all, packed four to a register. Then
-// x = sum_d digit * (2^weight mod m) is three v_dot4 int8 dot products, exact
-// with |x| <= 9 * 128 * 128 < 2^18, and congruent to s mod m. x is exact in
-// FP32 and |x / m| < 2^11, so rint(x * fl(1/m)) is rint(x / m) (the quotient
-// error is below 2^-12, under the 1/(2m) distance of x/m from any half
-// integer for odd m; m = 256 divides exactly), and r = x - q m is the
-// symmetric residue; for m = 256 r = +128 stores the byte of -128.
+// d2 2^16, |d| <= 128), nine digits in all: eight packed four to a register,
+// the top one (c2's d2) kept as a plain int. Then x = sum_d digit * (2^weight
+// mod m) is the top digit's 24-bit product followed by two v_dot4 int8 dot
+// products, exact with |x| <= 9 * 128 * 128 < 2^18, and congruent to s mod m.
+// The reduction stays in three full-rate instructions:
+//   tq = fma(x, fl(1/m), 1.5 * 2^23) = 1.5 * 2^23 + q,  q = rint(x / m),
+// since |x * fl(1/m)| < 2^11 keeps the sum in [2^23, 2^24) where the FP32
+// spacing is 1; the product is within 2^-14 of x / m, under the 1/(2m)
+// distance of x / m from any half integer for odd m (m = 256 is exact), so q
+// is rint(x / m). The low 24 bits of tq are 2^22 + q (bit 23 clear), and
+//   mad_i24(bits(tq), -m, x) = x - q m - 2^22 m
+// has, 2^22 m being a multiple of 256, the two's-complement byte of the
+// symmetric residue r = x - q m as its low byte (for m = 256, r = +-128 both
+// give the byte of -128). v_perm gathers four such low bytes into a word.
 __device__ __forceinline__ void balanced_bytes(int c, int& d0, int& d1, int& d2) {
     d0 = ((c + 128) & 255) - 128;
     c = (c - d0) >> 8;
@@ -191,36 +290,43 @@ __device__ __forceinline__ int byte4(int a, int b, int c, int d) {
                  ((unsigned)(d & 255) << 24));
 }
 
-// Digits of one image, packed: D0 = (c0: d0 d1 d2, c1: d0), D1 = (c1: d1 d2, c2: d0 d1), D2 = (c2: d2).
-__device__ __forceinline__ void image_digits(int c2, int c1, int c0, int& D0, int& D1, int& D2) {
-    int a0, a1, a2, b0, b1, b2, e0, e1, e2;
+// Digits of one image: D0 = (c0: d0 d1 d2, c1: d0), D1 = (c1: d1 d2, c2: d0 d1) packed, E2 = c2's d2.
+__device__ __forceinline__ void image_digits(int c2, int c1, int c0, int& D0, int& D1, int& E2) {
+    int a0, a1, a2, b0, b1, b2, e0, e1;
     balanced_bytes(c0, a0, a1, a2);
     balanced_bytes(c1, b0, b1, b2);
-    balanced_bytes(c2, e0, e1, e2);
+    balanced_bytes(c2, e0, e1, E2);
     D0 = byte4(a0, a1, a2, b0);
     D1 = byte4(b1, b2, e0, e1);
-    D2 = e2 & 255;
 }
 
-__device__ __forceinline__ v4i residues16(const int (&D0)[16], const int (&D1)[16], const int (&D2)[16],
+// Low bytes of a, b, c, d as one word (a in the lowest byte). Selector bytes
+// 0-3 pick bytes of the second operand, 4-7 of the first, 0x0c gives zero.
+__device__ __forceinline__ int low_bytes4(int a, int b, int c, int d) {
+    const unsigned lo = __builtin_amdgcn_perm((unsigned)b, (unsigned)a, 0x0c0c0400u);
+    const unsigned hi = __builtin_amdgcn_perm((unsigned)d, (unsigned)c, 0x0c0c0400u);
+    return (int)__builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
+__device__ __forceinline__ v4i residues16(const int (&D0)[16], const int (&D1)[16], const int (&E2)[16],
                                           const CrtTables& T, int t) {
-    const int K0 = T.kdig[t][0], K1 = T.kdig[t][1], K2 = T.kdig[t][2];
-    const float mf = (float)T.mod[t];
+    const int K0 = T.kdig[t][0], K1 = T.kdig[t][1];
+    const int w2 = (int)(signed char)(T.kdig[t][2] & 255);
+    const int negm = -(int)T.mod[t];
     const float invf = T.invf[t];
     int q[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-        const int x = __builtin_amdgcn_sdot4(D2[j], K2, __builtin_amdgcn_sdot4(D1[j], K1,
-                                             __builtin_amdgcn_sdot4(D0[j], K0, 0, false), false), false);
-        const float xf = (float)x;
-        // x - q m is an integer below 2^18: exact in the FMA
-        q[j] = (int)fmaf(-rintf(xf * invf), mf, xf);
+        const int x = __builtin_amdgcn_sdot4(D0[j], K0, __builtin_amdgcn_sdot4(D1[j], K1, __mul24(E2[j], w2), false),
+                                             false);
+        const float tq = fmaf((float)x, invf, 12582912.0f);
+        q[j] = __mul24(__float_as_int(tq), negm) + x;
     }
     v4i w;
-    w.x = byte4(q[0], q[1], q[2], q[3]);
-    w.y = byte4(q[4], q[5], q[6], q[7]);
-    w.z = byte4(q[8], q[9], q[10], q[11]);
-    w.w = byte4(q[12], q[13], q[14], q[15]);
+    w.x = low_bytes4(q[0], q[1], q[2], q[3]);
+    w.y = low_bytes4(q[4], q[5], q[6], q[7]);
+    w.z = low_bytes4(q[8], q[9], q[10], q[11]);
+    w.w = low_bytes4(q[12], q[13], q[14], q[15]);
     return w;
 }
 
@@ -260,7 +366,7 @@ __global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, in
             xv[j] = X[rowoff + (int64_t)(k < Kd ? k : Kd - 1) * sk];
         }
     }
-    int D0[16], D1[16], D2[16];
+    int D0[16], D1[16], E2[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int k = k0 + j;
@@ -271,10 +377,10 @@ __global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, in
         const double h = trunc(s * 0x1p-44);
         const double rest = fma(-h, 0x1p44, s);
         const double mid = trunc(rest * 0x1p-22);
-        image_digits((int)h, (int)mid, (int)fma(-mid, 0x1p22, rest), D0[j], D1[j], D2[j]);
+        image_digits((int)h, (int)mid, (int)fma(-mid, 0x1p22, rest), D0[j], D1[j], E2[j]);
     }
     const int64_t off = (int64_t)blk * 64 + lane;
-    for (int t = 0; t < T.nmod; t++) out[t * plane_stride + off] = residues16(D0, D1, D2, T, t);
+    for (int t = 0; t < T.nmod; t++) out[t * plane_stride + off] = residues16(D0, D1, E2, T, t);
 }
 
 // Column strips of `group` tile rows (256 x 256 tiles), planes in blockIdx.y.
@@ -667,32 +773,40 @@ struct Big {
 const int CRT_MODULI[CRT_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
                                         223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
 
-hipError_t launch_crt_norms(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
-                            const SplitPlan& sp, int* ta, int* tb, float* sig_a, float* sig_b, hipStream_t stream) {
-    const int M = (int)sp.m, N = (int)sp.n, K = (int)sp.k;
+hipError_t launch_crt_stats_norms(int layout_a, int layout_b, const double* a, int64_t lda, const double* b,
+                                  int64_t ldb, const SplitPlan& sp, int* ta, int* tb, float* sig_a, float* sig_b,
+                                  hipStream_t stream) {
+    const int K = (int)sp.k;
     NormPart* part = reinterpret_cast<NormPart*>(sp.stats_part);
-    auto rows = [&](const double* x, int64_t sr, int R, const int* e, const float* spread, int* t, float* sig) {
-        hipLaunchKernelGGL(crt_norm_rows, dim3((R + 3) / 4), dim3(256), 0, stream, x, sr, R, K, e, spread, t, sig);
-        return hipGetLastError();
-    };
-    auto cols = [&](const double* x, int64_t sk, int R, const int* e, const float* spread, int* t, float* sig) {
+    auto one = [&](bool kcontig, const double* x, int64_t s, int R, int* e, float* spread, int* t, float* sig) {
+        // Rows long enough to fill a workgroup and short enough for its registers: one read.
+        if (kcontig && K >= 2048 && K <= ROW_THREADS * ROW_REG) {
+            hipLaunchKernelGGL(crt_row_stats_norms, dim3(R), dim3(ROW_THREADS), 0, stream, x, s, K, e, spread, t,
+                               sig);
+            return hipGetLastError();
+        }
+        hipError_t err = launch_operand_stats(kcontig, x, s, R, K, sp.stats_part, e, spread, stream);
+        if (err != hipSuccess) return err;
+        if (kcontig) {
+            hipLaunchKernelGGL(crt_norm_rows, dim3((R + 3) / 4), dim3(256), 0, stream, x, s, R, K, e, spread, t,
+                               sig);
+            return hipGetLastError();
+        }
         const int kparts = split_stats_kparts(R, K);
         const int krange = (K + kparts - 1) / kparts;
-        hipLaunchKernelGGL(crt_norm_cols_partial, dim3((R + 63) / 64, kparts), dim3(256), 0, stream, x, sk, R, K,
+        hipLaunchKernelGGL(crt_norm_cols_partial, dim3((R + 63) / 64, kparts), dim3(256), 0, stream, x, s, R, K,
                            krange, e, part);
-        hipError_t err = hipGetLastError();
+        err = hipGetLastError();
         if (err != hipSuccess) return err;
         hipLaunchKernelGGL(crt_norm_cols_finish, dim3((R + 255) / 256), dim3(256), 0, stream, part, R, kparts, e,
                            spread, t, sig);
         return hipGetLastError();
     };
     // Rows of op(A): ROW -> element (i,k) at a[i*lda + k]; COL -> a[k*lda + i].
-    hipError_t err = layout_a == JLC_LAYOUT_ROW ? rows(a, lda, M, sp.ea, sp.spread_a, ta, sig_a)
-                                                : cols(a, lda, M, sp.ea, sp.spread_a, ta, sig_a);
+    hipError_t err = one(layout_a == JLC_LAYOUT_ROW, a, lda, (int)sp.m, sp.ea, sp.spread_a, ta, sig_a);
     if (err != hipSuccess) return err;
     // Columns of op(B): COL -> element (k,j) at b[j*ldb + k]; ROW -> b[k*ldb + j].
-    return layout_b == JLC_LAYOUT_COL ? rows(b, ldb, N, sp.eb, sp.spread_b, tb, sig_b)
-                                      : cols(b, ldb, N, sp.eb, sp.spread_b, tb, sig_b);
+    return one(layout_b == JLC_LAYOUT_COL, b, ldb, (int)sp.n, sp.eb, sp.spread_b, tb, sig_b);
 }
 
 double crt_log2_product(int nmod) {

@@ -38,6 +38,7 @@
 #include <math.h>
 
 #include "internal.h"
+#include "split_stats.h"
 
 namespace jlc {
 namespace {
@@ -46,7 +47,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef double d2 __attribute__((ext_vector_type(2)));
 
-constexpr int NONFINITE = -100000;  // exponent marker for rows / columns holding Inf or NaN
+constexpr int NONFINITE = STAT_NONFINITE;
 constexpr int TM = 128;             // C tile rows
 constexpr int TN = 64;              // C tile cols
 constexpr int KS = 32;              // K per MFMA step
@@ -57,12 +58,6 @@ constexpr int KS = 32;              // K per MFMA step
 // entries: the bits between the row's scale and its typical entry (geometric
 // mean, so heavy-tailed rows show their full width), read by the slice-count
 // and moduli-count choosers.
-__device__ __forceinline__ double log2_abs(double v) {
-    int fe;
-    const double mnt = frexp(v, &fe);  // v = mnt * 2^fe, mnt in [0.5, 1)
-    return (double)fe + (double)__log2f((float)mnt);
-}
-
 __global__ __launch_bounds__(256) void row_stats_kcontig(const double* __restrict__ X, int64_t sr, int R, int Kd,
                                                          int* __restrict__ e, float* __restrict__ spread) {
     const int lane = threadIdx.x & 63;
@@ -71,15 +66,7 @@ __global__ __launch_bounds__(256) void row_stats_kcontig(const double* __restric
     const double* row = X + (int64_t)r * sr;
     double mx = 0.0, slog = 0.0, cnt = 0.0;
     bool bad = false;
-    for (int k = lane; k < Kd; k += 64) {
-        double v = fabs(row[k]);
-        bad |= !(v <= 1.7976931348623157e308);
-        mx = fmax(mx, v);
-        if (v > 0.0 && v <= 1.7976931348623157e308) {
-            slog += log2_abs(v);
-            cnt += 1.0;
-        }
-    }
+    for (int k = lane; k < Kd; k += 64) stat_add(fabs(row[k]), mx, slog, cnt, bad);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         mx = fmax(mx, __shfl_xor(mx, off, 64));
@@ -104,15 +91,6 @@ __global__ __launch_bounds__(256) void row_stats_kcontig(const double* __restric
 struct StatPart {
     double mx, slog, cnt, pad;
 };
-
-__device__ __forceinline__ void stat_add(double v, double& mx, double& slog, double& cnt, bool& bad) {
-    bad |= !(v <= 1.7976931348623157e308);
-    mx = fmax(mx, v);
-    if (v > 0.0 && v <= 1.7976931348623157e308) {
-        slog += log2_abs(v);
-        cnt += 1.0;
-    }
-}
 
 __global__ __launch_bounds__(256) void col_stats_partial(const double* __restrict__ X, int64_t sk, int R, int Kd,
                                                          int krange, StatPart* __restrict__ part) {
@@ -392,6 +370,13 @@ hipError_t launch_col_stats(const double* x, int64_t sk, int R, int K, double* p
     return hipGetLastError();
 }
 }  // namespace
+
+hipError_t launch_operand_stats(bool kcontig, const double* x, int64_t stride, int R, int K, double* part, int* e,
+                                float* spread, hipStream_t stream) {
+    if (!kcontig) return launch_col_stats(x, stride, R, K, part, e, spread, stream);
+    hipLaunchKernelGGL(row_stats_kcontig, dim3((R + 3) / 4), dim3(256), 0, stream, x, stride, R, K, e, spread);
+    return hipGetLastError();
+}
 
 hipError_t launch_split_stats(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,
                               SplitPlan* p, hipStream_t stream) {

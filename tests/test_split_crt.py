@@ -30,9 +30,11 @@ def _store(X, layout, ld, off, pad_rows):
     return buf
 
 
-def _case(rng, la, lb, m, n, k, nmod, alpha, beta, phi=1.0, pad=0, off=0):
+def _case(rng, la, lb, m, n, k, nmod, alpha, beta, phi=1.0, pad=0, off=0, plant=None):
     A = _operand(rng, m, k, phi)
     B = _operand(rng, k, n, phi)
+    if plant is not None:
+        plant(A, B)
     lda = (k if la == 0 else m) + pad
     ldb = (n if lb == 0 else k) + pad
     ldc = n + pad
@@ -83,6 +85,41 @@ def test_modular_fold_beyond_65536_k():
     # INT32 sums are folded modulo m every 65536 k; 70000 crosses one fold.
     rng = np.random.default_rng(70)
     _case(rng, 0, 0, 24, 20, 70000, 16, 1.0, 0.0, phi=0.5)
+
+
+def _plant_zero_and_wide(A, B):
+    A[3] = 0.0                  # all-zero row: exponent 0, norm exponent 0
+    A[4, ::7] *= 2.0 ** 300     # a row whose norm sits far above most entries
+    A[5, 1] = 2.0 ** -1000      # a lone subnormal-range entry
+    B[:, 2] = 0.0               # all-zero column
+
+
+@pytest.mark.parametrize("la,lb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_long_rows_one_read_statistics(la, lb):
+    # Depths in [2048, 16384] take the one-read statistics and norm kernel on
+    # k-contiguous operands (row-major op(A), column-major op(B)); 2047 and
+    # 16385 the two-pass kernels. Exponents and norm exponents come from a max
+    # and integer sums, so every path must give the restatement's bits.
+    rng = np.random.default_rng(40 + 2 * la + lb)
+    for k in (2047, 2048, 3001, 16384, 16385):
+        _case(rng, la, lb, 21, 13, k, 12, 1.0, 0.0, phi=2.0, pad=1, off=1, plant=_plant_zero_and_wide)
+
+
+def test_long_nonfinite_rows_are_nan():
+    rng = np.random.default_rng(41)
+    m, n, k = 9, 7, 4096
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (k, n))
+    A[2, 4000] = np.inf
+    A[6, 17] = np.nan
+    for lb in (0, 1):
+        b = B.ravel().copy() if lb == 0 else B.T.ravel().copy()
+        c = np.zeros(m * n)
+        native.dgemm(0, lb, m, n, k, 1.0, A.ravel().copy(), k, b, n if lb == 0 else k, 0.0, c, n,
+                     native.mode_slices(native.MODE_SPLIT_CRT, 14))
+        c = c.reshape(m, n)
+        assert np.all(np.isnan(c[[2, 6]]))
+        assert np.all(np.isfinite(np.delete(c, [2, 6], axis=0)))
 
 
 def test_nonfinite_rows_and_columns_are_nan():
