@@ -186,7 +186,8 @@ def bench_batched(args) -> int:
                    "flops": w.flops, "bytes": w.bytes},
         "tflops": round(w.flops / (wall_ms * 1e-3) / 1e12, 3),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "grouped_f64",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic("grouped_f64_100k") if w.count == 100000 else None, "kernel": "grouped_f64",
                      "kernel_ms": round(dev_ms, 4)},
         "per_call_ms": None if per_ms is None else round(per_ms, 2),
         "speedup_vs_per_call": None if per_ms is None else round(per_ms / wall_ms, 1),

@@ -189,7 +189,9 @@ int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream
 /* ---- tuning table: the autotuner's choices, read by dispatch. Keys:
  * hip_min_flops (offload gate, the HIP analogue of GemmDispatch.CUDA_THRESHOLD,
  * src/main/java/net/faulj/compute/GemmDispatch.java:32), splitk_tiles_per_cu,
- * splitk_min_kslice, grouped_waves_per_simd. Text file, one "key value" per line.
+ * splitk_min_kslice, grouped_waves_per_simd, grouped_xcds (tile lists dealt per XCD;
+ * 1 = plain order), crt_bits_offset, crt_bits_slope, crt_waves, crt_group, crt_xcd.
+ * Text file, one "key value" per line.
  * On first use the library loads $JLC_TUNE_TABLE, or else the table shipped
  * beside it (<libdir>/../tuning/mi355x.tune); jlc_tune_load replaces it later. */
 int jlc_tune_load(const char* path);

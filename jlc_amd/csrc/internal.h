@@ -33,6 +33,7 @@ struct TuneTable {
     int splitk_tiles_per_cu = 2;      // split K until slices * tiles >= this * CUs (0 = never split)
     int splitk_min_kslice = 1024;     // smallest K per slice
     int grouped_waves_per_simd = 0;   // persistent grouped grid (0 = occupancy limit)
+    int grouped_xcds = 8;             // grouped tile list dealt per XCD (workgroup b on XCD b % this; 1 = off)
     double crt_bits_offset = 49.4;    // CRT form: norm bits = ceil(offset + log2(k)/2 + slope * max sig)
     double crt_bits_slope = 0.43;
     int crt_waves = 8;                // CRT product kernel: 4 (128 x 128 per wavefront) or 8 (128 x 64)
@@ -111,7 +112,11 @@ struct GroupedDesc {
 
 // Persistent grid size (workgroups) for the grouped kernel on a device with device_cus CUs.
 int grouped_grid(int device_cus, int waves_per_simd);
-// tiles: (problem << 32) | (tile_row << 16) | tile_col, 32 x 32 tiles.
+constexpr int GROUPED_WAVES_PER_GROUP = 4;  // wavefronts per grouped workgroup
+constexpr uint64_t GROUPED_SKIP = ~0ull;     // tile-list entry that names no tile
+// tiles: (problem << 32) | (tile_row << 16) | tile_col, 32 x 32 tiles; wavefront w of the
+// grid takes entries w, w + W, w + 2W, ... (W = grid * GROUPED_WAVES_PER_GROUP) and skips
+// GROUPED_SKIP entries.
 hipError_t launch_grouped_f64(const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
                               hipStream_t stream);
 

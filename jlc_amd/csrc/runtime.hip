@@ -635,6 +635,7 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"splitk_tiles_per_cu", nullptr, &jlc::TuneTable::splitk_tiles_per_cu},
         {"splitk_min_kslice", nullptr, &jlc::TuneTable::splitk_min_kslice},
         {"grouped_waves_per_simd", nullptr, &jlc::TuneTable::grouped_waves_per_simd},
+        {"grouped_xcds", nullptr, &jlc::TuneTable::grouped_xcds},
         {"crt_bits_offset", &jlc::TuneTable::crt_bits_offset, nullptr},
         {"crt_bits_slope", &jlc::TuneTable::crt_bits_slope, nullptr},
         {"crt_waves", nullptr, &jlc::TuneTable::crt_waves},
@@ -692,6 +693,7 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "splitk_tiles_per_cu %d\n", t.splitk_tiles_per_cu);
     std::fprintf(f, "splitk_min_kslice %d\n", t.splitk_min_kslice);
     std::fprintf(f, "grouped_waves_per_simd %d\n", t.grouped_waves_per_simd);
+    std::fprintf(f, "grouped_xcds %d\n", t.grouped_xcds);
     std::fprintf(f, "crt_bits_offset %.17g\n", t.crt_bits_offset);
     std::fprintf(f, "crt_bits_slope %.17g\n", t.crt_bits_slope);
     std::fprintf(f, "crt_waves %d\n", t.crt_waves);
@@ -1032,8 +1034,7 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     if (count >= (int64_t)1 << 31) return fail(JLC_ERR_INVALID_ARG, "more than 2^31-1 problems");
     if ((mode & 0xff) != JLC_MODE_FP64 || (mode >> 8) != 0)
         return fail(JLC_ERR_UNSUPPORTED, "grouped launches serve JLC_MODE_FP64 only");
-    // Tile list: one entry per 32 x 32 tile, sorted by decreasing k.
-    std::vector<std::pair<int32_t, uint64_t>> order;
+    // Problems longest K first (stable); each cut into chunks of whole tile rows of at most 16 tiles.
     for (int64_t p = 0; p < count; p++) {
         const jlc_gemm_desc& d = descs[p];
         int rc = validate_gemm(d.layout_a, d.layout_b, d.m, d.n, d.k, d.alpha, d.a, d.lda, d.b, d.ldb, d.c, d.ldc,
@@ -1043,29 +1044,58 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
             return fail(rc, "descriptor %lld: %s", (long long)p, msg.c_str());
         }
         if (d.m > 32 * 65535 || d.n > 32 * 65535) return fail(JLC_ERR_INVALID_ARG, "descriptor %lld too large", (long long)p);
-        const int tm = (d.m + 31) / 32, tn = (d.n + 31) / 32;
-        for (int i = 0; i < tm; i++)
-            for (int j = 0; j < tn; j++)
-                order.emplace_back(d.k, ((uint64_t)p << 32) | ((uint64_t)i << 16) | (uint64_t)j);
     }
-    if (order.size() >= ((size_t)1 << 31)) return fail(JLC_ERR_INVALID_ARG, "too many tiles");
-    std::stable_sort(order.begin(), order.end(),
-                     [](const std::pair<int32_t, uint64_t>& x, const std::pair<int32_t, uint64_t>& y) {
-                         return x.first > y.first;
-                     });
-    std::vector<uint64_t> tiles(order.size());
-    for (size_t i = 0; i < order.size(); i++) tiles[i] = order[i].second;
+    std::vector<int64_t> probs((size_t)count);
+    for (int64_t p = 0; p < count; p++) probs[(size_t)p] = p;
+    std::stable_sort(probs.begin(), probs.end(), [&](int64_t x, int64_t y) { return descs[x].k > descs[y].k; });
     int rc = ensure_init();
     if (rc != JLC_OK) return rc;
+    hipDeviceProp_t prop;
+    const int dev = g_device.load();
+    const int grid = jlc::grouped_grid(hipGetDeviceProperties(&prop, dev) == hipSuccess ? prop.multiProcessorCount
+                                                                                        : 256,
+                                       jlc::tune_snapshot().grouped_waves_per_simd);
+    const int64_t W = (int64_t)grid * jlc::GROUPED_WAVES_PER_GROUP;
+    // Workgroups are dispatched round-robin over the XCDs (workgroup b on XCD b % X), so XCD x owns
+    // wavefronts 4b + i for b = x (mod X). Each chunk goes to the XCD with the least work so far
+    // (work = tiles * max(k, 16)), and each XCD deals its sequence over its own wavefronts.
+    int X = std::max(1, jlc::tune_snapshot().grouped_xcds);
+    if (grid % X != 0) X = 1;
+    const int64_t Wx = W / X;
+    std::vector<std::vector<uint64_t>> seq((size_t)X);
+    std::vector<double> work((size_t)X, 0.0);
+    for (int64_t p : probs) {
+        const jlc_gemm_desc& d = descs[p];
+        const int tm = (int)((d.m + 31) / 32), tn = (int)((d.n + 31) / 32);
+        const int rows = std::max(1, 16 / std::max(1, tn));
+        for (int r0 = 0; r0 < tm; r0 += rows) {
+            const int r1 = std::min(tm, r0 + rows);
+            const size_t x = (size_t)(std::min_element(work.begin(), work.end()) - work.begin());
+            work[x] += (double)(r1 - r0) * tn * std::max<int64_t>(d.k, 16);
+            for (int i = r0; i < r1; i++)
+                for (int j = 0; j < tn; j++) seq[x].push_back(((uint64_t)p << 32) | ((uint64_t)i << 16) | (uint64_t)j);
+        }
+    }
+    size_t rounds = 0;
+    for (const auto& q : seq) rounds = std::max(rounds, (size_t)((q.size() + Wx - 1) / Wx));
+    if (rounds * (size_t)W >= ((size_t)1 << 31)) return fail(JLC_ERR_INVALID_ARG, "too many tiles");
+    std::vector<uint64_t> tiles(rounds * (size_t)W, jlc::GROUPED_SKIP);
+    size_t used = 0;
+    for (int x = 0; x < X; x++)
+        for (size_t i = 0; i < seq[(size_t)x].size(); i++) {
+            const int64_t j = (int64_t)(i % (size_t)Wx), r = (int64_t)(i / (size_t)Wx);
+            const int64_t b = x + (int64_t)X * (j / jlc::GROUPED_WAVES_PER_GROUP);
+            const size_t t = (size_t)(r * W + b * jlc::GROUPED_WAVES_PER_GROUP + j % jlc::GROUPED_WAVES_PER_GROUP);
+            tiles[t] = seq[(size_t)x][i];
+            used = std::max(used, t + 1);
+        }
+    tiles.resize(used);
     jlc_grouped_plan* pl = new jlc_grouped_plan;
-    pl->device = g_device.load();
+    pl->device = dev;
     pl->count = count;
     pl->ntiles = (int)tiles.size();
-    hipDeviceProp_t prop;
-    hipError_t e = hipGetDeviceProperties(&prop, pl->device);
-    pl->grid = jlc::grouped_grid(e == hipSuccess ? prop.multiProcessorCount : 256,
-                                 jlc::tune_snapshot().grouped_waves_per_simd);
-    e = hipMalloc(&pl->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::GroupedDesc));
+    pl->grid = grid;
+    hipError_t e = hipMalloc(&pl->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::GroupedDesc));
     if (e == hipSuccess) e = hipMalloc(&pl->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(uint64_t));
     if (e == hipSuccess && count)
         e = hipMemcpy(pl->d_descs, descs, (size_t)count * sizeof(jlc::GroupedDesc), hipMemcpyHostToDevice);

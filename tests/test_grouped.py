@@ -80,6 +80,29 @@ def test_graph_replays_match_eager_run():
         _check_problem(w, i, eager, "graph")
 
 
+@pytest.mark.parametrize("xcds,wps", [(1, 0), (8, 0), (8, 1), (3, 0)])
+def test_tile_placement_does_not_change_results(xcds, wps):
+    # The tile list's placement (per-XCD dealing, skip entries, grid width)
+    # only moves tiles between wavefronts: every tile is computed once, by the
+    # same instructions, so the bits must match the default placement's.
+    dims, alpha, beta = batched_shapes(600, seed=11, lo=1, hi=300)
+    ref_w = BatchedWorkload(dims, alpha, beta, seed=12)
+    native.dgemm_grouped_dev(ref_w.descs())
+    ref = ref_w.dc.to_numpy()
+    old = (native.tune_get("grouped_xcds"), native.tune_get("grouped_waves_per_simd"))
+    try:
+        native.tune_set("grouped_xcds", xcds)
+        native.tune_set("grouped_waves_per_simd", wps)
+        w = BatchedWorkload(dims, alpha, beta, seed=12)
+        native.dgemm_grouped_dev(w.descs())
+        assert np.array_equal(w.dc.to_numpy(), ref)
+    finally:
+        native.tune_set("grouped_xcds", old[0])
+        native.tune_set("grouped_waves_per_simd", old[1])
+    for i in range(0, w.count, 29):
+        _check_problem(w, i, ref, "placement")
+
+
 def test_invalid_descriptor_rejected():
     d = native.GemmDesc(0, 0, 4, 4, 4, 0, 1.0, 0.0, None, None, None, 4, 4, 4)
     with pytest.raises(native.JlcError) as ei:

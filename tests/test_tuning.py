@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def test_table_round_trip(tmp_path):
     old = {k: native.tune_get(k) for k in ("hip_min_flops", "splitk_tiles_per_cu", "splitk_min_kslice",
-                                           "grouped_waves_per_simd")}
+                                           "grouped_waves_per_simd", "grouped_xcds")}
     try:
         native.tune_set("hip_min_flops", 12345678.0)
         native.tune_set("splitk_tiles_per_cu", 3)
