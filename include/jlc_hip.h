@@ -141,8 +141,13 @@ typedef struct jlc_gemm_desc {
 } jlc_gemm_desc;
 
 typedef struct jlc_grouped_plan jlc_grouped_plan;
-/* Validates the descriptors, builds the tile list (sorted longest-K first)
- * and uploads both to the device. Only JLC_MODE_FP64 is served. */
+/* Validates the descriptors, builds the tile list (longest K first, each
+ * problem's tiles dealt to one XCD) and uploads both to the device. mode is
+ * JLC_MODE_FP64, or JLC_MODE_SPLIT_INT8 with an explicit slice count in [3, 9]
+ * (jlc_mode_slices): every problem then computed exactly as jlc_dgemm's INT8
+ * slice form computes it with that count (no-counterpart extension: the
+ * reference has no batched GEMM entry point; its TINY / SMALL kernels,
+ * SpecializedKernels.java:79-158, serve one problem per call). */
 int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode, jlc_grouped_plan** plan);
 /* One persistent launch over every tile of the plan. */
 int jlc_grouped_plan_run(jlc_grouped_plan* plan, void* stream, jlc_stats* stats);

@@ -32,6 +32,8 @@ namespace jlc {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 // Global-address-space views: the descriptor's pointers would otherwise compile
 // to flat accesses, which also occupy the LDS counter and path.
 typedef const __attribute__((address_space(1))) double* gcptr;
@@ -206,6 +208,163 @@ __global__ __launch_bounds__(64 * WAVES_PER_GROUP) __attribute__((amdgpu_waves_p
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-INT8 form of the grouped GEMM: every problem computed as the
+// single-GEMM INT8 slice form computes it (split_int8.hip; the same exponents,
+// digits, diagonal sums, K chunks, recombination and alpha / beta steps), one
+// wavefront per 32 x 32 tile and no workspace, so a plan stays one graph node.
+//  Pass 1 reads the tile's 32 rows of op(A) and 32 columns of op(B) over all K
+//  for their exponents (frexp of max |x|; Inf / NaN marks the row or column).
+//  Pass 2 slices each 32-deep block on the fly: lane l holds row / column l%32
+//  and k in [16 (l/32), +16) of the block, the MFMA fragment order of
+//  v_mfma_i32_32x32x32_i8, each value cut into S signed 7-bit digits packed 16
+//  to four registers; the S(S+1)/2 slice products accumulate per diagonal.
+constexpr int SPLIT_NONFINITE = -100000;
+
+template <int S>
+__device__ __forceinline__ void slice16(gcptr p, int64_t ks, int k0, int K, int e, v4i (&out)[S]) {
+    unsigned q[S][4];
+#pragma unroll
+    for (int t = 0; t < S; t++) q[t][0] = q[t][1] = q[t][2] = q[t][3] = 0u;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int k = k0 + j;
+        const double x = p[(int64_t)min(k, K - 1) * ks];
+        double v = (k < K && e != SPLIT_NONFINITE) ? ldexp(x, -e) : 0.0;
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            v *= 128.0;
+            const double dg = trunc(v);
+            v -= dg;
+            q[t][j >> 2] |= ((unsigned)(int)dg & 255u) << (8 * (j & 3));
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < S; t++) out[t] = v4i{(int)q[t][0], (int)q[t][1], (int)q[t][2], (int)q[t][3]};
+}
+
+// Frexp exponent of max |x| over the lane's 16-value share of every 32-deep block, combined with the
+// other half-wave's share: the exponent of the whole row / column, or SPLIT_NONFINITE.
+__device__ __forceinline__ int split_exponent(gcptr p, int64_t ks, int K, int h) {
+    double mx = 0.0;
+    int bad = 0;
+    for (int k0 = 16 * h; k0 < K; k0 += 32) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int k = k0 + j;
+            const double v = fabs(p[(int64_t)min(k, K - 1) * ks]);
+            if (k < K) {
+                bad |= !(v <= 1.7976931348623157e308);
+                mx = fmax(mx, v);
+            }
+        }
+    }
+    mx = fmax(mx, __shfl_xor(mx, 32, 64));
+    bad |= __shfl_xor(bad, 32, 64);
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);
+    return bad ? SPLIT_NONFINITE : e;
+}
+
+template <int S>
+__device__ __forceinline__ void split_tile(const GroupedDesc& d, int i0, int j0, int lane) {
+    const int K = d.k;
+    const int r32 = lane & 31, h = lane >> 5;
+    const bool a_row = d.layout_a == JLC_LAYOUT_ROW, b_row = d.layout_b == JLC_LAYOUT_ROW;
+    // Rows past m and columns past n read the last real one (their outputs are never stored).
+    const int arow = min(i0 + r32, d.m - 1), bcol = min(j0 + r32, d.n - 1);
+    const gcptr pa = (gcptr)d.a + (a_row ? (int64_t)arow * d.lda : (int64_t)arow);
+    const gcptr pb = (gcptr)d.b + (b_row ? (int64_t)bcol : (int64_t)bcol * d.ldb);
+    const int64_t ask = a_row ? 1 : d.lda, bsk = b_row ? d.ldb : 1;
+    const int ea = split_exponent(pa, ask, K, h);
+    const int eb = split_exponent(pb, bsk, K, h);
+    // INT32 diagonal sums stay exact for `chunk` 32-deep blocks: S * chunk * 32 * 127^2 < 2^31.
+    const int chunk = (int)(2147483647LL / ((long long)S * 32 * 127 * 127));
+    const int kblocks = (K + 31) / 32;
+    const gptr c = (gptr)d.c;
+    const int col = j0 + r32;
+    for (int c0 = 0; c0 < kblocks; c0 += chunk) {
+        v16i acc[S];
+#pragma unroll
+        for (int dd = 0; dd < S; dd++) acc[dd] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int c1 = min(kblocks, c0 + chunk);
+        for (int kb = c0; kb < c1; kb++) {
+            v4i af[S], bf[S];
+            slice16<S>(pa, ask, 32 * kb + 16 * h, K, ea, af);
+            slice16<S>(pb, bsk, 32 * kb + 16 * h, K, eb, bf);
+#pragma unroll
+            for (int t = 0; t < S; t++)
+#pragma unroll
+                for (int u = S - 1 - t; u >= 0; u--)
+                    acc[t + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[t], bf[u], acc[t + u], 0, 0, 0);
+        }
+        // Accumulator element r of lane l holds C(8*(r/4) + 4*(l/32) + r%4, l%32) of the tile.
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int ri = 8 * (r >> 2) + 4 * h + (r & 3);
+            const int ei = __shfl(ea, ri, 64);  // lane ri holds row i0 + ri's exponent
+            const int row = i0 + ri;
+            if (row < d.m && col < d.n) {
+                double sum = 0.0;
+#pragma unroll
+                for (int dd = S - 1; dd >= 0; dd--) sum += (double)acc[dd][r] * ldexp(1.0, -7 * (dd + 2));
+                const double v = (ei == SPLIT_NONFINITE || eb == SPLIT_NONFINITE) ? __builtin_nan("")
+                                                                                  : ldexp(sum, ei + eb);
+                const gptr pc = c + (int64_t)row * d.ldc + col;
+                if (c0 == 0) *pc = (d.beta == 0.0) ? d.alpha * v : fma(d.alpha, v, d.beta * *pc);
+                else *pc = fma(d.alpha, v, *pc);
+            }
+        }
+    }
+}
+
+// C = beta * C for a tile whose problem has alpha == 0 or K == 0 (op(A), op(B) unread), as the
+// single-GEMM entry points scale C without reaching the split kernels.
+__device__ __forceinline__ void scale_tile(const GroupedDesc& d, int i0, int j0, int lane) {
+    const gptr c = (gptr)d.c;
+    const int col = j0 + (lane & 31);
+    if (col >= d.n) return;
+    for (int r = lane >> 5; r < 32; r += 2) {
+        const int row = i0 + r;
+        if (row >= d.m) break;
+        const gptr pc = c + (int64_t)row * d.ldc + col;
+        *pc = d.beta == 0.0 ? 0.0 : d.beta * *pc;
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(64 * WAVES_PER_GROUP) void grouped_split_i8(const GroupedDesc* __restrict__ descs,
+                                                                         const uint64_t* __restrict__ tiles,
+                                                                         int ntiles) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_GROUP + (threadIdx.x >> 6));
+    const int nwaves = gridDim.x * WAVES_PER_GROUP;
+    const int rounds = (ntiles + nwaves - 1) / nwaves;
+    for (int round = 0; round < rounds; round++) {
+        const int t = wave + round * nwaves;
+        __syncthreads();
+        const uint64_t e = t < ntiles ? tiles[t] : GROUPED_SKIP;
+        if (e == GROUPED_SKIP) continue;
+        const int p = __builtin_amdgcn_readfirstlane((int)(e >> 32));
+        const int tm = (int)((e >> 16) & 0xffff), tn = (int)(e & 0xffff);
+        const GroupedDesc d = descs[p];
+        const int i0 = tm * 32, j0 = tn * 32;
+        if (d.alpha == 0.0 || d.k <= 0) scale_tile(d, i0, j0, lane);
+        else split_tile<S>(d, i0, j0, lane);
+    }
+}
+
+template <int S>
+int split_grid(int device_cus, int waves_per_simd) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_split_i8<S>, 64 * WAVES_PER_GROUP, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (waves_per_simd > 0 && waves_per_simd < per_cu) per_cu = waves_per_simd;
+    return device_cus * per_cu;
+}
+
 }  // namespace
 
 int grouped_grid(int device_cus, int waves_per_simd) {
@@ -219,6 +378,36 @@ int grouped_grid(int device_cus, int waves_per_simd) {
     // A tuned width below the occupancy limit (waves per SIMD == workgroups per CU here).
     if (waves_per_simd > 0 && waves_per_simd < per_cu) per_cu = waves_per_simd;
     return device_cus * per_cu;
+}
+
+int grouped_split_grid(int slices, int device_cus, int waves_per_simd) {
+    switch (slices) {
+        case 3: return split_grid<3>(device_cus, waves_per_simd);
+        case 4: return split_grid<4>(device_cus, waves_per_simd);
+        case 5: return split_grid<5>(device_cus, waves_per_simd);
+        case 6: return split_grid<6>(device_cus, waves_per_simd);
+        case 7: return split_grid<7>(device_cus, waves_per_simd);
+        case 8: return split_grid<8>(device_cus, waves_per_simd);
+        case 9: return split_grid<9>(device_cus, waves_per_simd);
+        default: return 0;
+    }
+}
+
+hipError_t launch_grouped_split_i8(int slices, const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
+                                   hipStream_t stream) {
+    if (ntiles <= 0) return hipSuccess;
+    const dim3 g(grid), b(64 * WAVES_PER_GROUP);
+    switch (slices) {
+        case 3: hipLaunchKernelGGL(grouped_split_i8<3>, g, b, 0, stream, descs, tiles, ntiles); break;
+        case 4: hipLaunchKernelGGL(grouped_split_i8<4>, g, b, 0, stream, descs, tiles, ntiles); break;
+        case 5: hipLaunchKernelGGL(grouped_split_i8<5>, g, b, 0, stream, descs, tiles, ntiles); break;
+        case 6: hipLaunchKernelGGL(grouped_split_i8<6>, g, b, 0, stream, descs, tiles, ntiles); break;
+        case 7: hipLaunchKernelGGL(grouped_split_i8<7>, g, b, 0, stream, descs, tiles, ntiles); break;
+        case 8: hipLaunchKernelGGL(grouped_split_i8<8>, g, b, 0, stream, descs, tiles, ntiles); break;
+        case 9: hipLaunchKernelGGL(grouped_split_i8<9>, g, b, 0, stream, descs, tiles, ntiles); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_grouped_f64(const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,

@@ -119,6 +119,11 @@ constexpr uint64_t GROUPED_SKIP = ~0ull;     // tile-list entry that names no ti
 // GROUPED_SKIP entries.
 hipError_t launch_grouped_f64(const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
                               hipStream_t stream);
+// Split-INT8 form of the grouped launch (slices in [SPLIT_MIN_SLICES, SPLIT_MAX_SLICES]): the grid that
+// fills the device for that kernel, and the launch over the same tile list.
+int grouped_split_grid(int slices, int device_cus, int waves_per_simd);
+hipError_t launch_grouped_split_i8(int slices, const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
+                                   hipStream_t stream);
 
 // ---- QR
 struct QrBatchDesc {

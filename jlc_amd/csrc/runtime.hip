@@ -1001,13 +1001,14 @@ struct jlc_grouped_plan {
     int64_t count = 0;
     int ntiles = 0;
     int grid = 0;
+    int slices = 0;  // 0: FP64 MFMA tiles; 3..9: the INT8 slice form with that many slices
     jlc::GroupedDesc* d_descs = nullptr;
     uint64_t* d_tiles = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
-static int stats_for_stream(hipStream_t s, jlc_stats* stats, double t0, int ntiles, bool start) {
+static int stats_for_stream(hipStream_t s, jlc_stats* stats, double t0, int slices, bool start) {
     if (!stats) return JLC_OK;
     if (start) {
         JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
@@ -1020,10 +1021,10 @@ static int stats_for_stream(hipStream_t s, jlc_stats* stats, double t0, int ntil
     jlc::TileInfo info;
     info.tile_m = 32;
     info.tile_n = 32;
-    info.tile_k = 16;
+    info.tile_k = slices ? 32 : 16;
+    info.slices = slices;
     info.launches = 1;
     fill_stats(stats, info, JLC_KERNEL_GROUPED, f, now_ms() - t0);
-    (void)ntiles;
     return JLC_OK;
 }
 
@@ -1032,8 +1033,18 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     *plan = nullptr;
     if (count < 0 || (count > 0 && !descs)) return fail(JLC_ERR_INVALID_ARG, "bad descriptor array");
     if (count >= (int64_t)1 << 31) return fail(JLC_ERR_INVALID_ARG, "more than 2^31-1 problems");
-    if ((mode & 0xff) != JLC_MODE_FP64 || (mode >> 8) != 0)
-        return fail(JLC_ERR_UNSUPPORTED, "grouped launches serve JLC_MODE_FP64 only");
+    // FP64 MFMA tiles, or the INT8 slice form with an explicit slice count (the same slices for every
+    // problem: a count chosen from the data would need the data on the host at plan time).
+    const int form = mode & 0xff, slices = (mode >> 8) & 0xff;
+    if (form == JLC_MODE_FP64) {
+        if (slices != 0) return fail(JLC_ERR_INVALID_ARG, "JLC_MODE_FP64 takes no slice count");
+    } else if (form == JLC_MODE_SPLIT_INT8) {
+        if (slices < jlc::SPLIT_MIN_SLICES || slices > jlc::SPLIT_MAX_SLICES)
+            return fail(JLC_ERR_INVALID_ARG, "grouped JLC_MODE_SPLIT_INT8 needs an explicit slice count in [%d, %d]",
+                        jlc::SPLIT_MIN_SLICES, jlc::SPLIT_MAX_SLICES);
+    } else {
+        return fail(JLC_ERR_UNSUPPORTED, "grouped launches serve JLC_MODE_FP64 and JLC_MODE_SPLIT_INT8");
+    }
     // Problems longest K first (stable); each cut into chunks of whole tile rows of at most 16 tiles.
     for (int64_t p = 0; p < count; p++) {
         const jlc_gemm_desc& d = descs[p];
@@ -1052,9 +1063,9 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     if (rc != JLC_OK) return rc;
     hipDeviceProp_t prop;
     const int dev = g_device.load();
-    const int grid = jlc::grouped_grid(hipGetDeviceProperties(&prop, dev) == hipSuccess ? prop.multiProcessorCount
-                                                                                        : 256,
-                                       jlc::tune_snapshot().grouped_waves_per_simd);
+    const int cus = hipGetDeviceProperties(&prop, dev) == hipSuccess ? prop.multiProcessorCount : 256;
+    const int wps = jlc::tune_snapshot().grouped_waves_per_simd;
+    const int grid = form == JLC_MODE_FP64 ? jlc::grouped_grid(cus, wps) : jlc::grouped_split_grid(slices, cus, wps);
     const int64_t W = (int64_t)grid * jlc::GROUPED_WAVES_PER_GROUP;
     // Workgroups are dispatched round-robin over the XCDs (workgroup b on XCD b % X), so XCD x owns
     // wavefronts 4b + i for b = x (mod X). Each chunk goes to the XCD with the least work so far
@@ -1095,6 +1106,7 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     pl->count = count;
     pl->ntiles = (int)tiles.size();
     pl->grid = grid;
+    pl->slices = form == JLC_MODE_FP64 ? 0 : slices;
     hipError_t e = hipMalloc(&pl->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::GroupedDesc));
     if (e == hipSuccess) e = hipMalloc(&pl->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(uint64_t));
     if (e == hipSuccess && count)
@@ -1109,15 +1121,21 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     return JLC_OK;
 }
 
+static hipError_t launch_plan(const jlc_grouped_plan* plan, hipStream_t s) {
+    return plan->slices ? jlc::launch_grouped_split_i8(plan->slices, plan->d_descs, plan->d_tiles, plan->ntiles,
+                                                       plan->grid, s)
+                        : jlc::launch_grouped_f64(plan->d_descs, plan->d_tiles, plan->ntiles, plan->grid, s);
+}
+
 int jlc_grouped_plan_run(jlc_grouped_plan* plan, void* stream, jlc_stats* stats) {
     if (!plan) return fail(JLC_ERR_INVALID_ARG, "plan must not be null");
     double t0 = now_ms();
     int rc = ensure_init();
     if (rc != JLC_OK) return rc;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
-    if ((rc = stats_for_stream(s, stats, t0, plan->ntiles, true)) != JLC_OK) return rc;
-    JLC_HIP_CHECK(jlc::launch_grouped_f64(plan->d_descs, plan->d_tiles, plan->ntiles, plan->grid, s));
-    return stats_for_stream(s, stats, t0, plan->ntiles, false);
+    if ((rc = stats_for_stream(s, stats, t0, plan->slices, true)) != JLC_OK) return rc;
+    JLC_HIP_CHECK(launch_plan(plan, s));
+    return stats_for_stream(s, stats, t0, plan->slices, false);
 }
 
 int jlc_grouped_plan_replay(jlc_grouped_plan* plan, void* stream, jlc_stats* stats) {
@@ -1131,7 +1149,7 @@ int jlc_grouped_plan_replay(jlc_grouped_plan* plan, void* stream, jlc_stats* sta
         hipStream_t cap;
         JLC_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) e = jlc::launch_grouped_f64(plan->d_descs, plan->d_tiles, plan->ntiles, plan->grid, cap);
+        if (e == hipSuccess) e = launch_plan(plan, cap);
         hipGraph_t graph = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &graph);
         if (e == hipSuccess) e = e2;
@@ -1144,9 +1162,9 @@ int jlc_grouped_plan_replay(jlc_grouped_plan* plan, void* stream, jlc_stats* sta
         }
         plan->graph = graph;
     }
-    if ((rc = stats_for_stream(s, stats, t0, plan->ntiles, true)) != JLC_OK) return rc;
+    if ((rc = stats_for_stream(s, stats, t0, plan->slices, true)) != JLC_OK) return rc;
     JLC_HIP_CHECK(hipGraphLaunch(plan->exec, s));
-    return stats_for_stream(s, stats, t0, plan->ntiles, false);
+    return stats_for_stream(s, stats, t0, plan->slices, false);
 }
 
 int jlc_grouped_plan_destroy(jlc_grouped_plan* plan) {

@@ -232,6 +232,14 @@ class DeviceArray:
             check(load().jlc_memcpy_h2d(d.ptr, arr.ctypes.data, arr.size * 8))
         return d
 
+    def copy_from(self, arr: np.ndarray) -> None:
+        """Overwrites the allocation with arr (same element count)."""
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        if arr.size != self.count:
+            raise ValueError(f"{arr.size} elements for a {self.count}-element device array")
+        if arr.size:
+            check(load().jlc_memcpy_h2d(self.ptr, arr.ctypes.data, arr.size * 8))
+
     def to_numpy(self, shape=None) -> np.ndarray:
         out = np.empty(self.count, dtype=np.float64)
         if self.count:

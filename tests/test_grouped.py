@@ -103,6 +103,79 @@ def test_tile_placement_does_not_change_results(xcds, wps):
         _check_problem(w, i, ref, "placement")
 
 
+def _check_split_problem(w, i, c_now, slices, tag):
+    m, n, k = (int(x) for x in w.dims[i])
+    la, lb = (int(x) for x in w.layouts[i])
+    lda, ldb = w.lds(i)
+    A, B, C0, Cn = w.problem(i, c_now)
+    exp = C0.copy()
+    alpha, beta = float(w.alpha[i]), float(w.beta[i])
+    if alpha == 0.0 or k == 0:
+        exp = np.zeros_like(C0) if beta == 0.0 else beta * C0   # C = beta * C, op(A), op(B) unread
+    elif m and n:
+        orc.split_gemm(la, lb, m, n, k, alpha, A, lda, B, ldb, beta, exp, n, slices)
+    same = (Cn == exp) | (np.isnan(Cn) & np.isnan(exp))
+    assert same.all(), (tag, i, m, n, k, la, lb, int((~same).sum()))
+
+
+@pytest.mark.parametrize("slices", [3, 5, 7, 9])
+def test_grouped_split_int8_bitwise_against_restatement(slices):
+    # Each problem must carry the single-GEMM INT8 slice form's bits: same
+    # exponents, digits, diagonal sums and FP64 recombination.
+    dims, alpha, beta = batched_shapes(700, seed=60 + slices, lo=1, hi=300)
+    rng = np.random.default_rng(61 + slices)
+    layouts = rng.integers(0, 2, size=(len(dims), 2))
+    alpha[::50] = 0.0                  # alpha == 0: C = beta * C
+    w = BatchedWorkload(dims, alpha, beta, seed=62 + slices, layouts=layouts)
+    # wide exponent ranges inside some problems, and Inf / NaN rows and columns in others
+    for i in range(0, w.count, 17):
+        m, n, k = (int(x) for x in dims[i])
+        a0 = int(w.a_off[i])
+        w.a_host[a0:a0 + m * k] *= np.exp2(rng.integers(-40, 40, m * k))
+    for i in range(3, w.count, 41):
+        m, n, k = (int(x) for x in dims[i])
+        if m * k:
+            w.a_host[int(w.a_off[i]) + rng.integers(0, m * k)] = np.inf
+        if k * n:
+            w.b_host[int(w.b_off[i]) + rng.integers(0, k * n)] = np.nan
+    w.da.copy_from(w.a_host)
+    w.db.copy_from(w.b_host)
+    st = native.dgemm_grouped_dev(w.descs(), native.mode_slices(native.MODE_SPLIT_INT8, slices))
+    assert st.kernel_name == "grouped" and st.slices == slices
+    c = w.dc.to_numpy()
+    for i in range(w.count):
+        _check_split_problem(w, i, c, slices, f"split{slices}")
+
+
+def test_grouped_split_int8_chunked_depth_and_replay():
+    # K = 15000 crosses the INT32 chunk (462 blocks of 32 at 9 slices): one mid-K flush.
+    dims = np.array([[40, 33, 15000], [5, 70, 14785], [64, 64, 300], [3, 3, 3]])
+    alpha = np.array([1.5, -0.5, 2.0, 1.0])
+    beta = np.array([0.5, 0.0, -1.0, 0.25])
+    w = BatchedWorkload(dims, alpha, beta, seed=70)
+    plan = native.GroupedPlan(w.descs(), native.mode_slices(native.MODE_SPLIT_INT8, 9))
+    plan.run()
+    native.synchronize()
+    first = w.dc.to_numpy()
+    for i in range(w.count):
+        _check_split_problem(w, i, first, 9, "chunked")
+    for rep in range(3):
+        w.reset_c()
+        native.synchronize()
+        plan.replay()
+        native.synchronize()
+        assert np.array_equal(w.dc.to_numpy(), first), rep
+    plan.destroy()
+
+
+def test_grouped_split_int8_slice_count_checked():
+    d = native.GemmDesc(0, 0, 4, 4, 4, 0, 1.0, 0.0, None, None, None, 4, 4, 4)
+    for mode in (native.MODE_SPLIT_INT8, native.mode_slices(native.MODE_SPLIT_INT8, 2),
+                 native.mode_slices(native.MODE_SPLIT_INT8, 10), native.MODE_SPLIT_CRT):
+        with pytest.raises(native.JlcError):
+            native.GroupedPlan([d], mode)
+
+
 def test_invalid_descriptor_rejected():
     d = native.GemmDesc(0, 0, 4, 4, 4, 0, 1.0, 0.0, None, None, None, 4, 4, 4)
     with pytest.raises(native.JlcError) as ei:
