@@ -158,6 +158,63 @@ def bench_batched(args) -> int:
         per_ms = (time.perf_counter() - t0) * 1e3
         log(f"per-call launches: {per_ms:.1f} ms")
     gbs = w.bytes / (dev_ms * 1e-3) / 1e9
+    # The INT8 slice form of the same batch. Slice count: the library's chooser (automatic split form)
+    # on a 256^3 problem from the batch's generator (uniform[-1,1]), unless --slices fixes it.
+    slices = args.slices
+    if not slices:
+        rng = np.random.default_rng(args.seed + 2)
+        a = rng.uniform(-1.0, 1.0, 256 * 256)
+        b = rng.uniform(-1.0, 1.0, 256 * 256)
+        slices = native.dgemm(0, 0, 256, 256, 256, 1.0, a, 256, b, 256, 0.0, np.zeros(256 * 256), 256,
+                              native.MODE_SPLIT_INT8).slices
+    plan_s = native.GroupedPlan(descs, native.mode_slices(native.MODE_SPLIT_INT8, slices))
+    plan_s.replay(stream)
+    split_wall, split_dev = time_form(lambda: plan_s.replay(stream), args.warmup, args.steps, stream)
+    log(f"grouped INT8 slice form (S = {slices}): {split_wall:.3f} ms wall, {split_dev:.3f} ms device")
+    split_per_ms = None
+    if not args.no_per_call:
+        smode = native.mode_slices(native.MODE_SPLIT_INT8, slices)
+
+        def split_per_call():
+            for d in descs:
+                native.dgemm_dev(d.layout_a, d.layout_b, d.m, d.n, d.k, d.alpha, d.a, d.lda, d.b, d.ldb, d.beta,
+                                 d.c, d.ldc, smode, stream, stats=False)
+        native.synchronize()
+        t0 = time.perf_counter()
+        split_per_call()
+        native.synchronize()
+        split_per_ms = (time.perf_counter() - t0) * 1e3
+        log(f"per-call launches, INT8 slice form: {split_per_ms:.1f} ms")
+    errors = None
+    if not args.no_error:
+        # Both forms from the same initial C, against double-double alpha*A*B + beta*C on every 100th problem.
+        from oracle import oracle as orc
+        outs = {}
+        for name, pl in (("fp64", plan), ("split_int8", plan_s)):
+            w.reset_c()
+            native.synchronize()
+            pl.run(stream)
+            native.synchronize()
+            outs[name] = w.dc.to_numpy()
+        errs = {name: [0.0, 0.0] for name in outs}
+        for i in range(0, w.count, 100):
+            m, n, k = (int(x) for x in w.dims[i])
+            A, B, C0, _ = w.problem(i)
+            al, be = float(w.alpha[i]), float(w.beta[i])
+            hi, lo = orc.dd_gemm(0, 0, m, n, k, al, A, k, B, n, be, C0, n)
+            exact = hi + lo
+            scale = abs(al) * (np.abs(A.reshape(m, k)) @ np.abs(B.reshape(k, n))).ravel() + abs(be) * np.abs(C0)
+            for name, cc in outs.items():
+                got = cc[int(w.c_off[i]):int(w.c_off[i]) + m * n]
+                err = np.abs(got - hi - lo)
+                nz = exact != 0.0
+                if nz.any():
+                    errs[name][0] = max(errs[name][0], float(np.max(err[nz] / np.abs(exact[nz]))))
+                errs[name][1] = max(errs[name][1], float(np.max(err / np.maximum(scale, 1e-300))) / 2.0 ** -53)
+        errors = {name: {"max_rel_err": v[0], "max_componentwise_err_units_u": v[1],
+                         "problems": len(range(0, w.count, 100))} for name, v in errs.items()}
+        log(f"errors vs dd (every 100th problem): {errors}")
+    plan_s.destroy()
     cpu = None
     if not args.no_cpu_baseline:
         # the reference's CPU path per problem: MicroKernel restatement on one thread, one call per problem
@@ -189,6 +246,13 @@ def bench_batched(args) -> int:
                      "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic("grouped_f64_100k") if w.count == 100000 else None, "kernel": "grouped_f64",
                      "kernel_ms": round(dev_ms, 4)},
+        "error": None if errors is None else errors["fp64"],
+        "split_int8": {"slices": int(slices), "ms_per_step": round(split_wall, 4),
+                       "device_ms": round(split_dev, 4), "problems_per_s": round(w.count / (split_wall * 1e-3), 1),
+                       "vs_fp64_grouped": round(wall_ms / split_wall, 3),
+                       "per_call_ms": None if split_per_ms is None else round(split_per_ms, 2),
+                       "speedup_vs_per_call": None if split_per_ms is None else round(split_per_ms / split_wall, 1),
+                       "error": None if errors is None else errors["split_int8"]},
         "per_call_ms": None if per_ms is None else round(per_ms, 2),
         "speedup_vs_per_call": None if per_ms is None else round(per_ms / wall_ms, 1),
         "plan_build_ms": round(plan_ms, 1), "cpu_baseline": cpu,
