@@ -42,11 +42,35 @@ __device__ __forceinline__ int slot(int kp, int r) { return kp * SROW + (r ^ (kp
 // KCONTIG: element (r, k) at base[r*ld + k]: thread reads the pair (r, 2kp..2kp+1) directly.
 // !KCONTIG: element (r, k) at base[k*ld + r]: thread reads rows k = 2kp, 2kp+1 at r, r+1 and
 //           regroups them into the pairs of r and r+1.
-// Out-of-range elements read as zero.
-template <bool KCONTIG, bool VEC, bool MASK>
+// MODE: LD_INTERIOR, every element in range; LD_EDGE, every k in range but r may pass rdim: r is
+// clamped to rdim - 1 (those rows / columns only reach outputs that are never stored) and every load
+// stays unconditional; LD_MASK, k may pass kdim too: out-of-range elements read as zero.
+constexpr int LD_INTERIOR = 0, LD_EDGE = 1, LD_MASK = 2;
+
+template <bool KCONTIG, bool VEC, int MODE>
 __device__ __forceinline__ void load_tile(const double* __restrict__ base, int64_t ld, int rdim, int kdim, int r0,
                                           int k0, int t, d2 (&regs)[4]) {
-    if (!MASK) {
+    if (MODE == LD_EDGE) {
+        if (KCONTIG) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = min(r0 + (t >> 3) + 32 * i, rdim - 1);
+                const double* p = base + (int64_t)r * ld + k0 + (t & 7) * 2;
+                regs[i] = VEC ? *reinterpret_cast<const d2*>(p) : d2{p[0], p[1]};
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int r = r0 + (t & 63) * 2;
+                const double* p = base + (int64_t)(k0 + 2 * ((t >> 6) + 4 * i)) * ld;
+                const int64_t c0 = min(r, rdim - 1), c1 = min(r + 1, rdim - 1);
+                regs[2 * i] = d2{p[c0], p[c0 + ld]};
+                regs[2 * i + 1] = d2{p[c1], p[c1 + ld]};
+            }
+        }
+        return;
+    }
+    if (MODE == LD_INTERIOR) {
         // Interior tile: every element is in range.
         if (KCONTIG) {
 #pragma unroll
@@ -165,9 +189,10 @@ __device__ __forceinline__ void tile_coords(int bid, int nblocks, int tiles_m, i
     tn = in / gsize;
 }
 
-// Main loop and epilogue of one C tile; MASK selects bounds-checked loads and
-// stores (edge tiles) or unchecked ones (interior tiles).
-template <bool A_COL, bool B_COL, bool VEC, bool MASK>
+// Main loop and epilogue of one C tile. MODE (see load_tile): LD_INTERIOR for interior tiles with
+// K a multiple of BK; LD_EDGE for the rest, which clamp rows / columns on every K tile, zero-fill the
+// last K tile when K is not a multiple of BK (LD_MASK loads), and skip stores outside C.
+template <bool A_COL, bool B_COL, bool VEC, int MODE>
 __device__ __forceinline__ void gemm_tile(int M, int N, int K, double alpha, const double* __restrict__ A,
                                           int64_t lda, const double* __restrict__ B, int64_t ldb, double beta,
                                           double* __restrict__ C, int64_t ldc, int m0, int n0, d2 (*As)[BK / 2 * SROW],
@@ -188,14 +213,23 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, double alpha, con
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
+    constexpr bool MASKED = MODE != LD_INTERIOR;
+    // Edge tiles clamp rows / columns on every K tile and zero-fill only the last K tile, when K is
+    // not a multiple of BK (tail is uniform per workgroup).
+    const int nk = (K + BK - 1) / BK;
+    const bool tail = MASKED && K % BK != 0;
     d2 ra[4], rb[4];
-    load_tile<A_KC, VEC, MASK>(A, lda, M, K, m0, 0, t, ra);
-    load_tile<B_KC, VEC, MASK>(B, ldb, N, K, n0, 0, t, rb);
+    if (MASKED && tail && nk == 1) {
+        load_tile<A_KC, VEC, LD_MASK>(A, lda, M, K, m0, 0, t, ra);
+        load_tile<B_KC, VEC, LD_MASK>(B, ldb, N, K, n0, 0, t, rb);
+    } else {
+        load_tile<A_KC, VEC, MODE>(A, lda, M, K, m0, 0, t, ra);
+        load_tile<B_KC, VEC, MODE>(B, ldb, N, K, n0, 0, t, rb);
+    }
     store_tile<A_KC>(As[0], t, ra);
     store_tile<B_KC>(Bs[0], t, rb);
     __syncthreads();
 
-    const int nk = (K + BK - 1) / BK;
     const int g = lane >> 4;   // k group within an MFMA step
     const int c16 = lane & 15; // column within a 16-wide fragment
     // Branch-free steady state: every iteration but the last prefetches the
@@ -205,8 +239,13 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, double alpha, con
     int kt = 0;
     for (; kt < nk - 1; kt++) {
         const int cur = kt & 1;
-        load_tile<A_KC, VEC, MASK>(A, lda, M, K, m0, (kt + 1) * BK, t, ra);
-        load_tile<B_KC, VEC, MASK>(B, ldb, N, K, n0, (kt + 1) * BK, t, rb);
+        if (MASKED && tail && kt + 1 == nk - 1) {
+            load_tile<A_KC, VEC, LD_MASK>(A, lda, M, K, m0, (kt + 1) * BK, t, ra);
+            load_tile<B_KC, VEC, LD_MASK>(B, ldb, N, K, n0, (kt + 1) * BK, t, rb);
+        } else {
+            load_tile<A_KC, VEC, MODE>(A, lda, M, K, m0, (kt + 1) * BK, t, ra);
+            load_tile<B_KC, VEC, MODE>(B, ldb, N, K, n0, (kt + 1) * BK, t, rb);
+        }
         mfma_steps(As[cur], Bs[cur], 0, 1, wm, wn, g, c16, acc);
         store_tile<A_KC>(As[cur ^ 1], t, ra);
         store_tile<B_KC>(Bs[cur ^ 1], t, rb);
@@ -222,11 +261,11 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, double alpha, con
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int col = n0 + wn * 64 + j * 16 + c16;
-            if (MASK && col >= N) continue;
+            if (MASKED && col >= N) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = m0 + wm * 64 + i * 16 + g + 4 * r;
-                if (!MASK || row < M) {
+                if (!MASKED || row < M) {
                     double* p = C + (int64_t)row * ldc + col;
                     double v = alpha * acc[i][j][r];
                     if (beta != 0.0) v += beta * *p;
@@ -262,9 +301,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     const int n0 = tn * BN;
     // Uniform per workgroup: interior tiles take the unchecked path.
     if (m0 + BM <= M && n0 + BN <= N && K % BK == 0)
-        gemm_tile<A_COL, B_COL, VEC, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
+        gemm_tile<A_COL, B_COL, VEC, LD_INTERIOR>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
     else
-        gemm_tile<A_COL, B_COL, VEC, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
+        gemm_tile<A_COL, B_COL, VEC, LD_EDGE>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
 }
 
 __global__ __launch_bounds__(256) void scale_kernel(int M, int N, double beta, double* __restrict__ C, int64_t ldc) {

@@ -256,8 +256,11 @@ def _block_index(off, ld, rows, cols):
 @pytest.mark.parametrize("la,lb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_all_layouts_ragged_edges(la, lb):
     rng = np.random.default_rng(100 + 10 * la + lb)
+    # K a multiple of 16 with ragged M / N takes the clamped edge loads over every K tile; other K
+    # zero-fill only the last K tile (one K tile, two, or many before it).
     shapes = [(1, 1, 1), (1, 7, 1), (2, 3, 5), (17, 33, 15), (127, 129, 16), (128, 128, 17), (129, 127, 255),
-              (200, 1, 300), (257, 65, 130), (300, 250, 37)]
+              (200, 1, 300), (257, 65, 130), (300, 250, 37), (130, 260, 64), (1, 300, 32), (333, 7, 48),
+              (65, 129, 160), (129, 257, 33), (256, 130, 1000)]
     for m, n, k in shapes:
         alpha = float(rng.choice([1.0, -0.5, 2.5]))
         beta = float(rng.choice([0.0, 1.0, -0.5]))
