@@ -133,11 +133,11 @@ struct QrBatchDesc {
     double* tau;  // min(m, n)
 };
 int qr_panel_chunk();
-// Factors panel columns [k0, k1) of the column-major m-row matrix at (in place, tau[k0..k1)).
-// part: 2*G+1 doubles, dots: G*128 doubles, vtmp: m doubles, G = ceil((m-k0)/qr_panel_chunk()); k1 - k0 <= 128.
-hipError_t launch_qr_panel(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part, double* dots,
-                           double* vtmp,
-                           hipStream_t stream);
+// Factors panel columns [k0, k1) of the column-major m-row matrix at (in place, tau[k0..k1)), one
+// launch per column; part holds 2 * (2G + 1 + 128) doubles, dots 2 * G * 128, xbuf 2 * m
+// (G = ceil((m - k0) / qr_panel_chunk())); k1 - k0 <= 128.
+hipError_t launch_qr_panel_fused(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part,
+                                 double* dots, double* xbuf, hipStream_t stream);
 hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb, double* vc, hipStream_t stream);
 hipError_t launch_build_t(const double* g, int nb, const double* tau, double* T, hipStream_t stream);
 hipError_t launch_extract_r(const double* at, int64_t lda, int n, double* r, hipStream_t stream);

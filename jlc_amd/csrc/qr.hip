@@ -13,21 +13,15 @@
 // W = V^T C, W = T^T W, C -= V W (updateTrailingMatrix, :167-209) — the
 // three GEMMs of that update run on this library's GEMM entry points.
 //
-// Panel kernels. Rows k0..m-1 of the panel are cut into chunks of PCHUNK rows
+// Panel kernel. Rows k0..m-1 of the panel are cut into chunks of PCHUNK rows
 // and the panel columns into groups of CW; a launch has one workgroup per
 // (row chunk, column group), so every column step runs on thousands of
-// workgroups with eight independent columns per thread. Per column k:
-//   panel_update: apply reflector k-1 to columns k..k1-1 (w_j reduced from the
-//     dot partials, every workgroup in the same order), store v_{k-1}, and
-//     produce the per-chunk (max, scaled sum of squares) of column k below the
-//     pivot; the owner of row k also saves the pivot x0.
-//   panel_reflect: every workgroup combines the norm partials in the same
-//     order (so all agree on beta and tau), forms its part of v into a side
-//     buffer, and emits per-chunk dot products v . a_j for j = k+1..k1-1; the
-//     owner of row k writes R(k,k) = beta and tau[k].
-// No workgroup reads data another one writes in the same launch (v travels
-// through the side buffer and reaches the matrix one launch later), so no
-// grid synchronisation is needed.
+// workgroups with eight independent columns per thread, and one launch per
+// column (panel_col) forms reflector k from partials the previous launch left,
+// applies it, and leaves the partials of column k+1: its norm partials, and the
+// dot products of its entries with the later columns, from which reflector
+// k+1's own dot products follow without another pass. No workgroup reads data
+// another one writes in the same launch, so no grid synchronisation is needed.
 #include <math.h>
 
 #include "internal.h"
@@ -70,9 +64,9 @@ __device__ double block_max(double v, double* red) {
 }
 
 // Norm partial layout: part[b*2 + 0] = chunk max |x|, part[b*2 + 1] = sum (x/max)^2;
-// part[2*G] = pivot x0 of the current column.
-// Grid of both panel kernels: x = row chunk b (PCHUNK rows from k0), y = group of CW
-// consecutive columns starting at the first column the launch works on.
+// part[2*G] = pivot x0 of the column.
+// Grid of the panel kernel: x = row chunk b (PCHUNK rows from k0), y = group of CW
+// consecutive columns starting after the launch's pivot column.
 constexpr int CW = 8;
 
 // Reflector scalars of column k from the norm partials, combined in a fixed
@@ -113,114 +107,121 @@ __device__ Reflector reflector_from(const double* __restrict__ part, int G, int 
     return r;
 }
 
-// Column k's step, first half. With k > k0: apply reflector k-1 (v from vtmp,
-// tau[k-1], w_j = sum over row chunks of dots in a fixed order) to columns
-// k + CW*y .. ; the y == 0 groups also store v_{k-1} into the matrix (nobody
-// reads column k-1 in this launch) and, for every k, emit the norm partials of
-// column k (its first column) after the update.
-__global__ __launch_bounds__(PTHREADS) void panel_update(double* __restrict__ at, int64_t lda, int m, int k0, int k1,
-                                                         int k, const double* __restrict__ tau,
-                                                         const double* __restrict__ vtmp,
-                                                         const double* __restrict__ dots, int G,
-                                                         double* __restrict__ part) {
-    __shared__ double red[4];
-    __shared__ double w_s[CW];
-    const int b = blockIdx.x;
-    const int row = k0 + b * PCHUNK + threadIdx.x;
-    const int j0 = k + CW * blockIdx.y;
-    const int jn = min(CW, k1 - j0);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (k > k0) {
-        const int kp = k - 1;
-        const double t = tau[kp];
-        // w_j: wave wv reduces columns wv and wv + 4 of the group, lanes over chunks, fixed tree
-        for (int c = wv; c < CW; c += 4) {
-            double s = 0.0;
-            if (c < jn)
-                for (int q = lane; q < G; q += 64) s += dots[(int64_t)q * 128 + (j0 + c - k0)];
-            s = wave_sum(s);
-            if (lane == 0) w_s[c] = s;
-        }
-        __syncthreads();
-        if (t != 0.0 && row >= kp && row < m) {
-            const double v = row == kp ? 1.0 : vtmp[row];
-            const double tv = t * v;
-#pragma unroll
-            for (int c = 0; c < CW; c++)
-                if (c < jn) at[(int64_t)(j0 + c) * lda + row] -= tv * w_s[c];
-        }
-        if (blockIdx.y == 0 && row > kp && row < m) at[(int64_t)kp * lda + row] = vtmp[row];
-    }
-    if (blockIdx.y == 0) {
-        const bool live = row > k && row < m;
-        const double x = live ? at[(int64_t)k * lda + row] : 0.0;
-        const double mx = block_max(fabs(x), red);
-        double sq = 0.0;
-        if (mx > 0.0) {
-            const double y = x / mx;
-            sq = y * y;
-        }
-        sq = block_sum(sq, red);
-        if (threadIdx.x == 0) {
-            part[2 * b] = mx;
-            part[2 * b + 1] = sq;
-        }
-        if (row == k) part[2 * G] = at[(int64_t)k * lda + k];
-    }
-}
+// One launch per panel column (panel_col). Launch k forms reflector k and applies it, and prepares
+// column k+1 for the next launch; the first launch of a panel (k = k0 - 1) only prepares column k0.
+// Column k+1's data travels in one of two buffer sets (P read, Q written; they alternate):
+//   part (2G + 1): per row chunk the max |x| and sum (x / max)^2 of column k+1 below its pivot, then
+//        the pivot x0;
+//   dots (G x 128): per row chunk, sum over the rows below column k+1's pivot of (x_i / chunk max) a_j[i]
+//        for the panel columns j > k+1 (after reflector k);
+//   prow (128): a_j at column k+1's pivot row; xb (m): column k+1 below its pivot.
+// With v = (1 at the pivot, x * v0inv below), v . a_j = prow[j] + sum over chunks of
+// (v0inv * chunk max) * dots[chunk][j]: the reflector's dot products come out of the pass that made
+// its column, so a column costs one launch. Every workgroup reduces the partials in the same order
+// and gets the same bits. Column k+1 itself is not written back (only its row k, R(k, k+1)): every
+// workgroup recomputes its rows after reflector k from the unmodified matrix, and the next launch
+// reads them from xb; column k is not read from the matrix at all (its x comes from xb), so launch k
+// writes v and R(k, k) straight into it.
+struct PanelSet {
+    double* part;
+    double* dots;
+    double* prow;
+    double* xb;
+};
 
-// Column k's step, second half: every workgroup derives the same reflector from
-// the partials; v = x * v0inv below the pivot (1 at the pivot) is computed from
-// the unscaled column, which nobody writes in this launch. The y == 0 groups
-// store v into vtmp; the owner of row k writes R(k,k) = beta and tau[k]; every
-// group emits the per-chunk dot products v . a_j for its columns j > k.
-__global__ __launch_bounds__(PTHREADS) void panel_reflect(double* __restrict__ at, int64_t lda, int m, int k0,
-                                                          int k1, int k, double* __restrict__ tau, int G,
-                                                          const double* __restrict__ part,
-                                                          double* __restrict__ vtmp, double* __restrict__ dots) {
-    __shared__ double v_s[PCHUNK];
+__global__ __launch_bounds__(PTHREADS) void panel_col(double* __restrict__ at, int64_t lda, int m, int k0, int k1,
+                                                      int k, double* __restrict__ tau, int G, PanelSet P,
+                                                      PanelSet Q) {
     __shared__ double red[4];
-    const int b = blockIdx.x;
-    const int row = k0 + b * PCHUNK + threadIdx.x;
-    const Reflector rf = reflector_from(part, G, k, m, red);
-    double v = 0.0;
-    if (row > k && row < m) {
-        v = at[(int64_t)k * lda + row];
-        if (rf.v0inv != 0.0) v *= rf.v0inv;
-        if (blockIdx.y == 0) vtmp[row] = v;
-    } else if (row == k) {
-        v = 1.0;
+    __shared__ double w_s[CW + 1];       // w_j of the group's columns, then w of column k+1
+    __shared__ double xs_s[PCHUNK];      // column k+1 below its pivot, divided by the chunk max
+    __shared__ double col_s[CW][PCHUNK]; // the group's columns after reflector k, below column k+1's pivot
+    const int b = blockIdx.x, y = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int i = k0 + b * PCHUNK + tid;  // this thread's row
+    const bool first = k < k0;
+    const int j1 = k + 1;                 // next pivot column
+    const int jb = j1 + CW * y;           // first column of this group
+    // Loads that need nothing from this launch go out first and land during the partial reductions:
+    // the group's columns, column k+1 (both unwritten here except by their owners, after the loads)
+    // and column k's copy.
+    double acol[CW];
+#pragma unroll
+    for (int c = 0; c < CW; c++) {
+        const int j = jb + c;
+        acol[c] = (j > j1 && j < k1 && i < m && i >= k) ? at[(int64_t)j * lda + i] : 0.0;
     }
-    v_s[threadIdx.x] = (row >= k && row < m) ? v : 0.0;
-    if (row == k && blockIdx.y == 0) {
-        at[(int64_t)k * lda + k] = rf.beta;
-        tau[k] = rf.tau;
+    const double xold = (j1 < k1 && i < m && i > k) ? at[(int64_t)j1 * lda + i] : 0.0;
+    const double rk1 = (!first && y == 0 && i == k && j1 < k1) ? at[(int64_t)j1 * lda + k] : 0.0;
+    const double xk = (!first && i > k && i < m) ? P.xb[i] : 0.0;
+    const Reflector rf = first ? Reflector{0.0, 0.0, 0.0} : reflector_from(P.part, G, k, m, red);
+    // w_j = tau * v . a_j for the group's columns (slots 0..CW-1) and column k+1 (slot CW).
+    for (int c = wv; c <= CW; c += 4) {
+        const int j = c < CW ? jb + c : j1;
+        double s = 0.0;
+        if (!first && j < k1) {
+            for (int q = lane; q < G; q += 64) s += (rf.v0inv * P.part[2 * q]) * P.dots[(int64_t)q * 128 + (j - k0)];
+            s = wave_sum(s);
+        }
+        if (lane == 0) w_s[c] = (first || j >= k1) ? 0.0 : rf.tau * (P.prow[j - k0] + s);
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j0 = k + 1 + CW * blockIdx.y;
-    const int rbase = k0 + b * PCHUNK;
-    for (int c = wv; c < CW; c += 4) {
-        const int j = j0 + c;
-        if (j >= k1) break;
-        double s = 0.0;
-        for (int i = lane; i < PCHUNK; i += 64) {
-            const int r = rbase + i;
-            if (r >= k && r < m) s += v_s[i] * at[(int64_t)j * lda + r];
+    // v of column k at this row (0 above the pivot; the first launch applies nothing).
+    double v = 0.0;
+    if (!first && i >= k && i < m) v = i == k ? 1.0 : xk * rf.v0inv;
+    if (!first && y == 0) {
+        if (i > k && i < m) at[(int64_t)k * lda + i] = rf.v0inv != 0.0 ? xk * rf.v0inv : xk;
+        if (i == k) {
+            at[(int64_t)k * lda + k] = rf.beta;
+            tau[k] = rf.tau;
         }
+    }
+    // Column k+1 after reflector k: row k becomes R(k, k+1) (group 0 writes it), rows from k+1 on are
+    // recomputed by every group.
+    double xn = 0.0;
+    if (j1 < k1 && i < m) {
+        if (i > k) xn = xold - v * w_s[CW];
+        else if (!first && i == k && y == 0) at[(int64_t)j1 * lda + k] = rk1 - w_s[CW];
+    }
+    // The group's own columns (those after column k+1): update in place, keep rows below k+1 in LDS.
+#pragma unroll
+    for (int c = 0; c < CW; c++) {
+        const int j = jb + c;
+        double a = 0.0;
+        if (j > j1 && j < k1 && i < m && i >= k) {
+            a = acol[c];
+            if (!first) {
+                a -= v * w_s[c];
+                at[(int64_t)j * lda + i] = a;
+            }
+            if (i == j1) Q.prow[j - k0] = a;
+        }
+        col_s[c][tid] = i > j1 ? a : 0.0;
+    }
+    if (j1 >= k1) return;  // the panel's last column: nothing to prepare (uniform per launch)
+    const double xl = (i > j1 && i < m) ? xn : 0.0;
+    const double mb = block_max(fabs(xl), red);
+    const double xsc = mb > 0.0 ? xl / mb : 0.0;
+    xs_s[tid] = xsc;
+    if (y == 0) {
+        const double sq = block_sum(xsc * xsc, red);
+        if (tid == 0) {
+            Q.part[2 * b] = mb;
+            Q.part[2 * b + 1] = sq;
+        }
+        if (i == j1) Q.part[2 * G] = xn;
+        if (i > j1 && i < m) Q.xb[i] = xn;
+    }
+    __syncthreads();
+    for (int c = wv; c < CW; c += 4) {
+        const int j = jb + c;
+        if (j <= j1 || j >= k1) continue;
+        double s = 0.0;
+        for (int t = lane; t < PCHUNK; t += 64) s += xs_s[t] * col_s[c][t];
         s = wave_sum(s);
-        if (lane == 0) dots[(int64_t)b * 128 + (j - k0)] = s;
+        if (lane == 0) Q.dots[(int64_t)b * 128 + (j - k0)] = s;
     }
 }
-
-// After the panel's last column: store its v (rows below the pivot) from vtmp.
-__global__ __launch_bounds__(256) void panel_store_v(double* __restrict__ at, int64_t lda, int m, int k,
-                                                     const double* __restrict__ vtmp) {
-    const int row = k + 1 + blockIdx.x * 256 + threadIdx.x;
-    if (row < m) at[(int64_t)k * lda + row] = vtmp[row];
-}
-
-// Apply the last reflector of a panel (column k1-1) has no panel columns left; nothing to do.
 
 // Vc (nb x mk, row-major): Vc[j][i] = v_j at panel row i (unit diagonal, zeros above).
 __global__ __launch_bounds__(256) void build_vc(const double* __restrict__ at, int64_t lda, int m, int k0, int nb,
@@ -436,21 +437,23 @@ __global__ __launch_bounds__(256) void qr_batched_small(const QrBatchDesc* __res
 
 int qr_panel_chunk() { return PCHUNK; }
 
-hipError_t launch_qr_panel(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part, double* dots,
-                           double* vtmp, hipStream_t stream) {
+hipError_t launch_qr_panel_fused(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part,
+                                 double* dots, double* xbuf, hipStream_t stream) {
     const int G = (m - k0 + PCHUNK - 1) / PCHUNK;
-    for (int k = k0; k < k1; k++) {
-        const int gu = k > k0 ? (k1 - k + CW - 1) / CW : 1;
-        hipLaunchKernelGGL(panel_update, dim3(G, gu), dim3(PTHREADS), 0, stream, at, lda, m, k0, k1, k, tau, vtmp,
-                           dots, G, part);
-        const int gr = k + 1 < k1 ? (k1 - k - 1 + CW - 1) / CW : 1;
-        hipLaunchKernelGGL(panel_reflect, dim3(G, gr), dim3(PTHREADS), 0, stream, at, lda, m, k0, k1, k, tau, G,
-                           part, vtmp, dots);
+    const int64_t partn = 2 * (int64_t)G + 1;
+    PanelSet S[2];
+    for (int p = 0; p < 2; p++) {
+        S[p].part = part + p * (partn + 128);
+        S[p].prow = S[p].part + partn;
+        S[p].dots = dots + p * (int64_t)G * 128;
+        S[p].xb = xbuf + p * (int64_t)m;
     }
-    const int last = k1 - 1;
-    if (last + 1 < m)
-        hipLaunchKernelGGL(panel_store_v, dim3((m - last - 1 + 255) / 256), dim3(256), 0, stream, at, lda, m, last,
-                           vtmp);
+    for (int k = k0 - 1; k < k1; k++) {
+        const int groups = k + 1 < k1 ? (k1 - k - 1 + CW - 1) / CW : 1;
+        const int rd = (k - k0 + 2) & 1, wr = rd ^ 1;  // launch k reads set rd, writes set wr
+        hipLaunchKernelGGL(panel_col, dim3(G, groups), dim3(PTHREADS), 0, stream, at, lda, m, k0, k1, k, tau, G,
+                           S[rd], S[wr]);
+    }
     return hipGetLastError();
 }
 

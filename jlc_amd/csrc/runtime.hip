@@ -1245,9 +1245,10 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
     const int64_t G = (m + jlc::qr_panel_chunk() - 1) / jlc::qr_panel_chunk();
     DevBuf part, dots, vtmp, vc, g, t, w1, w2;
-    hipError_t e = part.alloc((size_t)(2 * G + 1) * 8);
-    if (e == hipSuccess) e = vtmp.alloc((size_t)m * 8);
-    if (e == hipSuccess) e = dots.alloc((size_t)G * 128 * 8);
+    // Two sets of panel partials (norm partials + pivot + pivot row), dot partials and column copies.
+    hipError_t e = part.alloc((size_t)2 * (2 * G + 1 + 128) * 8);
+    if (e == hipSuccess) e = vtmp.alloc((size_t)2 * m * 8);
+    if (e == hipSuccess) e = dots.alloc((size_t)2 * G * 128 * 8);
     if (e == hipSuccess) e = vc.alloc((size_t)nb * m * 8);
     if (e == hipSuccess) e = g.alloc((size_t)nb * nb * 8);
     if (e == hipSuccess) e = t.alloc((size_t)nb * nb * 8);
@@ -1259,8 +1260,9 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     for (int64_t k0 = 0; k0 < kmax; k0 += nb) {
         const int64_t k1 = k0 + nb < kmax ? k0 + nb : kmax;
         const int64_t nbk = k1 - k0, mk = m - k0, nt = n - k1;
-        JLC_HIP_CHECK(jlc::launch_qr_panel(a, lda, (int)m, (int)k0, (int)k1, tau, part.d(), dots.d(), vtmp.d(), s));
-        info.launches += 2 * (int)nbk + 1;
+        JLC_HIP_CHECK(jlc::launch_qr_panel_fused(a, lda, (int)m, (int)k0, (int)k1, tau, part.d(), dots.d(), vtmp.d(),
+                                                 s));
+        info.launches += (int)nbk + 1;
         if (nt <= 0) continue;
         if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
         // Transposed compact-WY update of C = A[k0:m, k1:n] held as C^T (nt x mk, row-major, ld = lda):
