@@ -29,8 +29,9 @@
 namespace jlc {
 namespace {
 
-constexpr int PCHUNK = 256;   // panel rows per workgroup
-constexpr int PTHREADS = 256;
+constexpr int PCHUNK = 512;   // panel rows per workgroup
+constexpr int PTHREADS = 512;
+constexpr int PWAVES = PTHREADS / 64;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -44,23 +45,31 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Block-wide sum / max over 256 threads (4 wavefronts); result in every thread.
+// Block-wide sum / max over NW wavefronts; result in every thread (red holds NW doubles).
+template <int NW = 4>
 __device__ double block_sum(double v, double* red) {
     v = wave_sum(v);
     const int w = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[w] = v;
     __syncthreads();
-    return red[0] + red[1] + red[2] + red[3];
+    double s = red[0];
+#pragma unroll
+    for (int i = 1; i < NW; i++) s += red[i];
+    return s;
 }
 
+template <int NW = 4>
 __device__ double block_max(double v, double* red) {
     v = wave_max(v);
     const int w = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[w] = v;
     __syncthreads();
-    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    double s = red[0];
+#pragma unroll
+    for (int i = 1; i < NW; i++) s = fmax(s, red[i]);
+    return s;
 }
 
 // Norm partial layout: part[b*2 + 0] = chunk max |x|, part[b*2 + 1] = sum (x/max)^2;
@@ -81,7 +90,7 @@ __device__ Reflector reflector_from(const double* __restrict__ part, int G, int 
     const double x0 = part[2 * G];
     double mloc = 0.0;
     for (int q = threadIdx.x; q < G; q += PTHREADS) mloc = fmax(mloc, part[2 * q]);
-    const double M = fmax(fabs(x0), block_max(mloc, red));
+    const double M = fmax(fabs(x0), block_max<PWAVES>(mloc, red));
     Reflector r{0.0, x0, 0.0};
     if (k < m - 1 && M >= 2.2250738585072014e-308) {
         double sloc = 0.0;
@@ -89,7 +98,7 @@ __device__ Reflector reflector_from(const double* __restrict__ part, int G, int 
             const double s = part[2 * q] / M;
             sloc += part[2 * q + 1] * s * s;
         }
-        const double sigma = block_sum(sloc, red);
+        const double sigma = block_sum<PWAVES>(sloc, red);
         const double xnorm = sqrt(sigma);
         const double x0n = x0 / M;
         if (xnorm == 0.0) {
@@ -132,7 +141,7 @@ struct PanelSet {
 __global__ __launch_bounds__(PTHREADS) void panel_col(double* __restrict__ at, int64_t lda, int m, int k0, int k1,
                                                       int k, double* __restrict__ tau, int G, PanelSet P,
                                                       PanelSet Q) {
-    __shared__ double red[4];
+    __shared__ double red[PWAVES];
     __shared__ double w_s[CW + 1];       // w_j of the group's columns, then w of column k+1
     __shared__ double xs_s[PCHUNK];      // column k+1 below its pivot, divided by the chunk max
     __shared__ double col_s[CW][PCHUNK]; // the group's columns after reflector k, below column k+1's pivot
@@ -156,7 +165,7 @@ __global__ __launch_bounds__(PTHREADS) void panel_col(double* __restrict__ at, i
     const double xk = (!first && i > k && i < m) ? P.xb[i] : 0.0;
     const Reflector rf = first ? Reflector{0.0, 0.0, 0.0} : reflector_from(P.part, G, k, m, red);
     // w_j = tau * v . a_j for the group's columns (slots 0..CW-1) and column k+1 (slot CW).
-    for (int c = wv; c <= CW; c += 4) {
+    for (int c = wv; c <= CW; c += PWAVES) {
         const int j = c < CW ? jb + c : j1;
         double s = 0.0;
         if (!first && j < k1) {
@@ -200,11 +209,11 @@ __global__ __launch_bounds__(PTHREADS) void panel_col(double* __restrict__ at, i
     }
     if (j1 >= k1) return;  // the panel's last column: nothing to prepare (uniform per launch)
     const double xl = (i > j1 && i < m) ? xn : 0.0;
-    const double mb = block_max(fabs(xl), red);
+    const double mb = block_max<PWAVES>(fabs(xl), red);
     const double xsc = mb > 0.0 ? xl / mb : 0.0;
     xs_s[tid] = xsc;
     if (y == 0) {
-        const double sq = block_sum(xsc * xsc, red);
+        const double sq = block_sum<PWAVES>(xsc * xsc, red);
         if (tid == 0) {
             Q.part[2 * b] = mb;
             Q.part[2 * b + 1] = sq;
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(PTHREADS) void panel_col(double* __restrict__ at, i
         if (i > j1 && i < m) Q.xb[i] = xn;
     }
     __syncthreads();
-    for (int c = wv; c < CW; c += 4) {
+    for (int c = wv; c < CW; c += PWAVES) {
         const int j = jb + c;
         if (j <= j1 || j >= k1) continue;
         double s = 0.0;
