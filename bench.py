@@ -387,6 +387,7 @@ def bench_qr(args) -> int:
     work = native.DeviceArray(m * n)
     tau = native.DeviceArray(n)
     forms = {}
+    panel_ms = []
     for name, mode in (("fp64", native.MODE_FP64), ("split_int8", native.MODE_SPLIT_INT8),
                        ("split_crt", native.MODE_SPLIT_CRT)):
         times = []
@@ -394,10 +395,12 @@ def bench_qr(args) -> int:
             native.dev_copy(work.ptr, A.ptr, m * n * 8, stream)
             native.synchronize()
             t1 = time.perf_counter()
-            native.dgeqrf_dev(m, n, work.ptr, m, tau.ptr, nb, mode, stream)
+            st = native.dgeqrf_dev(m, n, work.ptr, m, tau.ptr, nb, mode, stream)
             native.synchronize()
             if it >= args.warmup:
                 times.append((time.perf_counter() - t1) * 1e3)
+                if name == "fp64":
+                    panel_ms.append(st.main_ms)  # the panel kernels' device time (HIP events)
         ms = float(np.median(times))
         recon, orth = native.dqr_errors_dev(m, n, A.ptr, m, work.ptr, m, tau.ptr, nb)
         forms[name] = {"ms": round(ms, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 3),
@@ -418,12 +421,33 @@ def bench_qr(args) -> int:
            "recon_err": float(np.linalg.norm(As - Q @ R) / np.linalg.norm(As)),
            "orth_err": float(np.linalg.norm(Q.T @ Q - np.eye(cn)))}
     best = min(forms.values(), key=lambda f: f["ms"])
+    # Roofline of the factorization's dominant kernel, the panel (one panel_col launch per column plus one
+    # per panel): algorithmic bytes = per launch for column k of panel [k0, k1), the later panel columns'
+    # rows k..m-1 read and written, column k's copy read and its v written; plus each panel's first launch
+    # reading the panel once.
+    pbytes, launches = 0.0, 0
+    for k0 in range(0, n, nb):
+        k1 = min(k0 + nb, n)
+        pbytes += 8.0 * (m - k0) * (k1 - k0)
+        launches += 1
+        for k in range(k0, k1):
+            pbytes += 16.0 * (m - k) * (k1 - k - 1) + 16.0 * (m - k)
+            launches += 1
+    pms = float(np.mean(panel_ms))
+    gbs = pbytes / (pms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "panel_col",
+                "kernel_ms": round(pms / launches, 5), "launches": launches,
+                "note": f"{pbytes / launches:.4g} algorithmic bytes per launch; the panel kernels take "
+                        f"{pms:.1f} of the {forms['fp64']['ms']:.1f} ms (FP64 form)"}
+    cpu_baseline = {"value": round(cpu["gflops"] / 1e3, 6), "unit": "TFLOP/s", "cores": 1, "kind": "port",
+                    "sample": cpu["sample"] + f" ({cpu['seconds']} s)"}
     print(json.dumps({
         "metric": "blocked Householder QR 32768x4096 (nb=128), A = U diag(s) V^T, s geometric 1..1e-8",
         "value": best["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic", "config": {"workload": "qr 32768x4096 nb=128", "flops": flops},
-        "forms": forms, "cpu_path": cpu}), flush=True)
+        "forms": forms, "roofline": roofline, "cpu_baseline": cpu_baseline, "cpu_path": cpu}), flush=True)
     return 0
 
 
@@ -484,6 +508,14 @@ def bench_qr_batched(args) -> int:
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "20000 QRs, A = U diag(s) V^T per matrix", "flops": flops},
         "tflops": round(flops / (ms * 1e-3) / 1e12, 4), "max_recon_err": recon, "max_orth_err": orth,
+        # the matrices sit in LDS for the whole factorization: FP64 arithmetic against the FP64 peak
+        "roofline": {"bound": "mfma", "achieved": round(flops / (ms * 1e-3) / 1e12, 4), "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 5),
+                     "traffic": None, "kernel": "qr_batched_small", "kernel_ms": round(ms, 3),
+                     "note": "one workgroup per matrix, a column step per barrier sequence: latency-bound"},
+        "cpu_baseline": {"value": round(2000 / (cpu_ms * 1e-3), 1), "unit": "matrices/s", "cores": 1,
+                         "kind": "port", "sample": f"first 2000 matrices, oracle HouseholderQR restatement, "
+                                                   f"1 thread ({cpu_ms / 1e3:.2f} s)"},
         "cpu_path": {"sample": "first 2000 matrices, oracle HouseholderQR restatement, 1 thread",
                      "ms": round(cpu_ms, 1), "matrices_per_s": round(2000 / (cpu_ms * 1e-3), 1)}}), flush=True)
     return 0

@@ -1256,12 +1256,32 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     if (e == hipSuccess) e = w2.alloc((size_t)n * nb * 8);
     if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "QR workspace allocation failed: %s", hipGetErrorString(e));
     jlc::TileInfo info;
-    if (stats) JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    // With stats, an event pair around each panel's launches: main_ms is the panel kernels' device time.
+    struct Events {
+        std::vector<hipEvent_t> ev;
+        ~Events() {
+            if (g_alive.load())
+                for (hipEvent_t x : ev) (void)hipEventDestroy(x);
+        }
+    } pev;
+    if (stats) {
+        const int64_t panels = (kmax + nb - 1) / nb;
+        pev.ev.reserve((size_t)(2 * panels));
+        for (int64_t q = 0; q < 2 * panels; q++) {
+            hipEvent_t x;
+            JLC_HIP_CHECK(hipEventCreate(&x));
+            pev.ev.push_back(x);
+        }
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    }
     for (int64_t k0 = 0; k0 < kmax; k0 += nb) {
         const int64_t k1 = k0 + nb < kmax ? k0 + nb : kmax;
         const int64_t nbk = k1 - k0, mk = m - k0, nt = n - k1;
+        const size_t pi = (size_t)(2 * (k0 / nb));
+        if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi], s));
         JLC_HIP_CHECK(jlc::launch_qr_panel_fused(a, lda, (int)m, (int)k0, (int)k1, tau, part.d(), dots.d(), vtmp.d(),
                                                  s));
+        if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi + 1], s));
         info.launches += (int)nbk + 1;
         if (nt <= 0) continue;
         if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
@@ -1285,6 +1305,11 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         float f = 0.0f;
         JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
         kms = f;
+        for (size_t q = 0; q + 1 < pev.ev.size(); q += 2) {
+            float pf = 0.0f;
+            JLC_HIP_CHECK(hipEventElapsedTime(&pf, pev.ev[q], pev.ev[q + 1]));
+            info.main_ms += pf;
+        }
     } else {
         JLC_HIP_CHECK(hipStreamSynchronize(s));  // workspace is freed on return
     }
