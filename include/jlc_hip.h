@@ -177,7 +177,7 @@ int jlc_dorgqr_dev(int64_t m, int64_t n, const double* a, int64_t lda, const dou
 int jlc_dqr_errors_dev(int64_t m, int64_t n, const double* a_orig, int64_t lda_orig, const double* qr, int64_t lda,
                        const double* tau, int64_t nb, double* recon_err, double* orth_err);
 
-/* Batched small QR: each matrix (m*n <= 20480, n <= 256) factored by one
+/* Batched small QR: each matrix (m*n <= 20220, n <= 256) factored by one
  * workgroup in LDS, unblocked, same reflector convention. */
 typedef struct jlc_qr_desc {
     int32_t m, n;

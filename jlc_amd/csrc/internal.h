@@ -126,6 +126,9 @@ hipError_t launch_grouped_split_i8(int slices, const GroupedDesc* descs, const u
                                    hipStream_t stream);
 
 // ---- QR
+// Largest m * n of one batched small QR: the matrix tile and the kernel's 2080 bytes of static LDS fit
+// the 160 KiB a workgroup may hold.
+constexpr int QR_BATCH_MAX_MN = (163840 - 2080) / 8;
 struct QrBatchDesc {
     int32_t m, n;
     int64_t lda;

@@ -354,8 +354,9 @@ __global__ __launch_bounds__(256) void sumsq_partial(const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
 // Batched small QR: one workgroup factors one column-major m x n matrix held
-// in LDS (m*n <= 20480 doubles), unblocked, same reflector convention.
+// in LDS (m*n <= QR_BATCH_MAX_MN doubles, n <= 256), unblocked, same reflector convention.
 __global__ __launch_bounds__(256) void qr_batched_small(const QrBatchDesc* __restrict__ descs) {
     extern __shared__ double a_s[];
     __shared__ double red[4];

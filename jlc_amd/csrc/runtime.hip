@@ -1427,7 +1427,7 @@ struct jlc_qr_batched_plan {
 
 namespace {
 // LDS entries (doubles) of each class: 8, 4, 2 and 1 workgroups per CU.
-constexpr int QR_CLASS_MN[jlc_qr_batched_plan::NCLASS] = {2304, 4864, 9984, 20480};
+constexpr int QR_CLASS_MN[jlc_qr_batched_plan::NCLASS] = {2304, 4864, 9984, jlc::QR_BATCH_MAX_MN};
 
 hipError_t launch_qr_plan(const jlc_qr_batched_plan* p, hipStream_t s) {
     for (int c = 0; c < jlc_qr_batched_plan::NCLASS; c++) {
@@ -1448,9 +1448,9 @@ int jlc_qr_batched_plan_create(int64_t count, const jlc_qr_desc* descs, jlc_qr_b
         const jlc_qr_desc& d = descs[i];
         if (d.m < 1 || d.n < 1 || d.n > 256 || d.lda < d.m || !d.a || !d.tau)
             return fail(JLC_ERR_INVALID_ARG, "descriptor %lld: need m, n >= 1, n <= 256, lda >= m", (long long)i);
-        if ((int64_t)d.m * d.n > 20480)
-            return fail(JLC_ERR_INVALID_ARG, "descriptor %lld: m*n = %lld exceeds the 20480-entry LDS tile",
-                        (long long)i, (long long)d.m * d.n);
+        if ((int64_t)d.m * d.n > jlc::QR_BATCH_MAX_MN)
+            return fail(JLC_ERR_INVALID_ARG, "descriptor %lld: m*n = %lld exceeds the %d-entry LDS tile",
+                        (long long)i, (long long)d.m * d.n, jlc::QR_BATCH_MAX_MN);
         max_mn = std::max(max_mn, d.m * d.n);
     }
     int rc = ensure_init();

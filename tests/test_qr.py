@@ -118,6 +118,39 @@ def test_batched_small_qr_against_oracle_and_graph_replay():
 
 def test_batched_rejects_oversized_matrix():
     d = native.DeviceArray(8)
-    with pytest.raises(native.JlcError) as ei:
-        native.QrBatchedPlan([native.QrDesc(300, 100, 300, d.ptr, d.ptr)])
-    assert ei.value.status == native.JLC_ERR_INVALID_ARG
+    for m, n in ((300, 100), (500, 41), (2, 257)):  # m*n past the LDS tile (20220 entries), n past 256
+        with pytest.raises(native.JlcError) as ei:
+            native.QrBatchedPlan([native.QrDesc(m, n, m, d.ptr, d.ptr)])
+        assert ei.value.status == native.JLC_ERR_INVALID_ARG
+
+
+def test_batched_largest_tiles_and_special_columns():
+    # The largest accepted tiles (m*n = 20220 and n = 256), beside matrices whose first column is zero,
+    # zero below a negative pivot, or below the FP64 minimum.
+    rng = np.random.default_rng(78)
+    mats = [rng.standard_normal((674, 30)), rng.standard_normal((78, 256)), rng.standard_normal((30, 12)),
+            rng.standard_normal((25, 10)), rng.standard_normal((20, 8))]
+    # (the first column meets no earlier reflector, so each case happens as built)
+    mats[2][:, 0] = 0.0                 # tau = 0
+    mats[3][:, 0] = 0.0
+    mats[3][0, 0] = -1.5                # zero below a negative pivot: tau = 2, R(0,0) = 1.5
+    mats[4][:, 0] = 1e-310              # max below the FP64 minimum: tau = 0
+    flat = np.concatenate([np.asfortranarray(M).reshape(-1, order="F") for M in mats])
+    offs = np.concatenate([[0], np.cumsum([M.size for M in mats])[:-1]])
+    toffs = np.concatenate([[0], np.cumsum([min(M.shape) for M in mats])[:-1]])
+    da = native.DeviceArray.from_numpy(flat)
+    dt = native.DeviceArray(int(sum(min(M.shape) for M in mats)))
+    descs = [native.QrDesc(M.shape[0], M.shape[1], M.shape[0], da.at(int(o)), dt.at(int(t)))
+             for M, o, t in zip(mats, offs, toffs)]
+    native.dgeqrf_batched_dev(descs)
+    got, taus = da.to_numpy(), dt.to_numpy()
+    for i, M in enumerate(mats):
+        m, n = M.shape
+        kk = min(m, n)
+        f = got[offs[i]:offs[i] + m * n].copy()
+        tau = taus[toffs[i]:toffs[i] + kk]
+        ref = np.asfortranarray(M).reshape(-1, order="F").copy()
+        tau_ref = orc.qr_factor(ref, m, n)
+        np.testing.assert_allclose(tau, tau_ref[:kk], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(np.triu(f.reshape(n, m).T[:kk]), np.triu(ref.reshape(n, m).T[:kk]),
+                                   rtol=1e-9, atol=1e-12 * np.abs(M).max())
