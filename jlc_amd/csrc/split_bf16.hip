@@ -20,6 +20,7 @@
 #include <math.h>
 
 #include "internal.h"
+#include "split_stats.h"
 
 namespace jlc {
 namespace {
@@ -29,7 +30,7 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int NONFINITE = -100000;
+constexpr int NONFINITE = STAT_NONFINITE;  // exponent marker of rows / columns holding Inf or NaN
 constexpr int TM = 64, TN = 64, KS = 16;
 
 __device__ __forceinline__ unsigned short bf16_of_int(int q) {
