@@ -145,7 +145,7 @@ typedef struct jlc_grouped_plan jlc_grouped_plan;
 /* Validates the descriptors, builds the tile list (longest K first, each
  * problem's tiles dealt to one XCD) and uploads both to the device. mode is
  * JLC_MODE_FP64, or JLC_MODE_SPLIT_INT8 with an explicit slice count in [3, 9]
- * (jlc_mode_slices): every problem then computed exactly as jlc_dgemm's INT8
+ * (JLC_MODE_SLICES): every problem then computed exactly as jlc_dgemm's INT8
  * slice form computes it with that count (no-counterpart extension: the
  * reference has no batched GEMM entry point; its TINY / SMALL kernels,
  * SpecializedKernels.java:79-158, serve one problem per call). */
