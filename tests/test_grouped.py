@@ -61,6 +61,34 @@ def test_grouped_degenerate_problems():
                           orc.cpu_abs_tolerance(exp, k))
 
 
+def test_grouped_split_degenerate_problems():
+    # The INT8 slice form keeps the same BLAS semantics: alpha == 0 or k == 0 scale C without reading
+    # A or B, beta == 0 never reads C (NaN planted there stays out of the result).
+    dims = np.array([[0, 5, 3], [4, 0, 3], [5, 4, 0], [3, 3, 3], [1, 1, 1], [7, 2, 9], [40, 33, 70]])
+    alpha = np.array([1.0, 1.0, 2.5, 0.0, 1.0, -1.0, 1.5])
+    beta = np.array([0.0, 0.5, -0.5, 2.5, 0.0, 0.0, 0.0])
+    w = BatchedWorkload(dims, alpha, beta, seed=6)
+    A3 = w.a_off[3]
+    w.a_host[A3:A3 + 9] = np.nan
+    w.da.copy_from(w.a_host)
+    w.c_host[w.c_off[4]] = np.nan
+    w.c_host[w.c_off[5]:w.c_off[5] + 14] = np.nan
+    w.c_host[w.c_off[6]:w.c_off[6] + 40 * 33] = np.inf
+    w.dc.copy_from(w.c_host)
+    native.dgemm_grouped_dev(w.descs(), native.mode_slices(native.MODE_SPLIT_INT8, 7))
+    c = w.dc.to_numpy()
+    _, _, C0, C2 = w.problem(2, c)
+    assert np.array_equal(C2, -0.5 * C0)
+    _, _, C0, C3 = w.problem(3, c)
+    assert np.array_equal(C3, 2.5 * C0)
+    for i in (4, 5, 6):
+        m, n, k = (int(x) for x in w.dims[i])
+        A, B, _, Ci = w.problem(i, c)
+        exp = np.zeros(m * n)
+        orc.split_gemm(0, 0, m, n, k, float(alpha[i]), A, k, B, n, 0.0, exp, n, 7)
+        assert np.all(np.isfinite(Ci)) and np.array_equal(Ci, exp), i
+
+
 def test_graph_replays_match_eager_run():
     dims, alpha, beta = batched_shapes(800, seed=7)
     beta[:] = 0.0  # replays are then idempotent: every replay must give the same bits
