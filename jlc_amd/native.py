@@ -1,7 +1,7 @@
 """ctypes binding of libjlc_hip.so (the C ABI declared in include/jlc_hip.h).
 
 This is the Python analogue of the Java FFM downcall layer a JLC maintainer
-adds: plain pointers and sizes, an int status per call,
+adds beside GemmDispatch: plain pointers and sizes, an int status per call,
 and the thread-local error message turned into an exception. There is no
 fallback: if the shared library is missing, loading raises.
 """
