@@ -73,6 +73,33 @@ def test_graded_spectrum_split_trailing_updates():
         assert recon < 1e-13 and orth < 1e-12, (name, recon, orth)
 
 
+def test_wide_and_ragged_panels_against_oracle():
+    """n > m (columns past min(m, n) take every panel's update, no reflector of their own) and a ragged
+    last panel, on the caller's stream: R and tau as the oracle restatement, A = QR to FP64 level with Q
+    formed by the oracle from the first min(m, n) columns."""
+    rng = np.random.default_rng(91)
+    stream = native.stream()
+    for (m, n, nb) in [(300, 700, 64), (777, 520, 32), (130, 131, 128)]:
+        A = rng.standard_normal((m, n))
+        a_cm = np.asfortranarray(A).reshape(-1, order="F").copy()
+        da = native.DeviceArray.from_numpy(a_cm)
+        kk = min(m, n)
+        dt = native.DeviceArray(kk)
+        native.synchronize()
+        native.dgeqrf_dev(m, n, da.ptr, m, dt.ptr, nb, native.MODE_FP64, stream)
+        native.synchronize()
+        F, tau = da.to_numpy().reshape(n, m).T, dt.to_numpy()
+        ref = a_cm.copy()
+        tau_ref = orc.qr_factor(ref, m, n, nb=nb)
+        Fr = ref.reshape(n, m).T
+        np.testing.assert_allclose(tau, tau_ref, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(np.triu(F[:kk]), np.triu(Fr[:kk]), rtol=0, atol=1e-10 * np.abs(Fr).max())
+        f = np.asfortranarray(F[:, :kk]).reshape(-1, order="F").copy()
+        Q = orc.qr_form_q(f, tau, m, kk).reshape(kk, m).T
+        assert np.linalg.norm(A - Q @ np.triu(F[:kk])) / np.linalg.norm(A) < 1e-14, (m, n, nb)
+        assert np.linalg.norm(Q.T @ Q - np.eye(kk)) < 1e-12, (m, n, nb)
+
+
 def _batch(rng, count, lo=16, hi=192):
     ms = rng.integers(lo, hi + 1, count)
     ns = np.maximum(1, ms // 2)
