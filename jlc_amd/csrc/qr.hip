@@ -253,69 +253,80 @@ __global__ __launch_bounds__(256) void build_vc(const double* __restrict__ at, i
 // Van Loan's compact-WY merge), all block pairs of one level at once; log2(nb) levels instead of nb
 // dependent column steps. The nb x nb LDS array holds T in its upper triangle (diagonal tau) and G in
 // its strict lower triangle; each level computes X = G12 T22 into registers, stores it in T12's place,
-// then T12 = -T11 X the same way (at most nb^2/4 entries a level: 16 per thread).
-__global__ __launch_bounds__(256) void build_t(const double* __restrict__ g, int nb, const double* __restrict__ tau,
-                                               double* __restrict__ T) {
+// then T12 = -T11 X the same way.
+// Work split of a level: the P block pairs side by side make W = b * P (pair, column) slots (W <= 64
+// for nb <= 128), and thread t takes slot t % W at rows t / W + k * (BT_THREADS / W), k < BT_ROWS, so the
+// column operand (T22(l, j), then X(l, j)) is loaded once per l for all of the thread's rows, and the row
+// operands are the same for every lane of a wavefront at b >= 8 (LDS broadcast).
+constexpr int BT_THREADS = 512, BT_ROWS = 8;  // rows per thread: b <= 64 and BT_THREADS / W >= 8
+
+__global__ __launch_bounds__(BT_THREADS) void build_t(const double* __restrict__ g, int nb,
+                                                      const double* __restrict__ tau, double* __restrict__ T) {
     extern __shared__ double S[];  // nb * nb
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int r = idx / nb, c = idx % nb;
-        if (r < c) S[c * nb + r] = g[idx];  // G(r, c) in the lower triangle
-        else if (r == c) S[idx] = tau[r];
+    const int tid = threadIdx.x;
+    // G = V^T V is symmetric: its own lower triangle goes in place. The loads are unconditional (index
+    // clamped into the matrix) so that all of them are in flight at once.
+    constexpr int FILL = 128 * 128 / BT_THREADS;
+    double gv[FILL];
+#pragma unroll
+    for (int q = 0; q < FILL; q++) gv[q] = g[min(tid + q * BT_THREADS, nb * nb - 1)];
+#pragma unroll
+    for (int q = 0; q < FILL; q++) {
+        const int idx = tid + q * BT_THREADS;
+        if (idx < nb * nb) {
+            const int r = idx / nb;
+            if (r > idx - r * nb) S[idx] = gv[q];
+        }
     }
+    if (tid < nb) S[tid * nb + tid] = tau[tid];
     __syncthreads();
     for (int b = 1; b < nb; b *= 2) {
-        const int total = ((nb + 2 * b - 1) / (2 * b)) * b * b;  // block pairs x b x b entries
-        double val[16];
-        // X = G12 * T22: X(i, j) = sum_{l <= j} G(s+i, s+b+l) T(s+b+l, s+b+j)
+        const int P = (nb + 2 * b - 1) / (2 * b), W = b * P, R = BT_THREADS / W;
+        const int c = tid % W, i0 = tid / W, j = c % b;
+        const int s0 = 2 * b * (c / b), b2 = min(b, nb - s0 - b);
+        const bool act = i0 < R && j < b2;
+        double x[BT_ROWS];
+        // X = G12 * T22: X(i, j) = sum_{l <= j} G(s0+i, s0+b+l) T(s0+b+l, s0+b+j)
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int e = threadIdx.x + 256 * q;
-            double x = 0.0;
-            if (e < total) {
-                const int pr = e / (b * b), rem = e % (b * b), i = rem / b, j = rem % b;
-                const int s0 = 2 * b * pr, b2 = min(b, nb - s0 - b);
-                if (j < b2)
-                    for (int l = 0; l <= j; l++) x += S[(s0 + b + l) * nb + s0 + i] * S[(s0 + b + l) * nb + s0 + b + j];
+        for (int k = 0; k < BT_ROWS; k++) x[k] = 0.0;
+        if (act) {
+            for (int l = 0; l <= j; l++) {
+                const int row = (s0 + b + l) * nb + s0;
+                const double tv = S[row + b + j];
+#pragma unroll
+                for (int k = 0; k < BT_ROWS; k++) x[k] += S[row + min(i0 + k * R, b - 1)] * tv;
             }
-            val[q] = x;
         }
         __syncthreads();
+        if (act) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int e = threadIdx.x + 256 * q;
-            if (e < total) {
-                const int pr = e / (b * b), rem = e % (b * b), i = rem / b, j = rem % b;
-                const int s0 = 2 * b * pr, b2 = min(b, nb - s0 - b);
-                if (j < b2) S[(s0 + i) * nb + s0 + b + j] = val[q];
-            }
+            for (int k = 0; k < BT_ROWS; k++)
+                if (i0 + k * R < b) S[(s0 + i0 + k * R) * nb + s0 + b + j] = x[k];
         }
         __syncthreads();
         // T12 = -T11 * X: T12(i, j) = -sum_{l >= i} T(s+i, s+l) X(l, j)
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int e = threadIdx.x + 256 * q;
-            double x = 0.0;
-            if (e < total) {
-                const int pr = e / (b * b), rem = e % (b * b), i = rem / b, j = rem % b;
-                const int s0 = 2 * b * pr, b2 = min(b, nb - s0 - b);
-                if (j < b2)
-                    for (int l = i; l < b; l++) x += S[(s0 + i) * nb + s0 + l] * S[(s0 + l) * nb + s0 + b + j];
+        for (int k = 0; k < BT_ROWS; k++) x[k] = 0.0;
+        if (act) {
+            for (int l = i0; l < b; l++) {
+                const double xv = S[(s0 + l) * nb + s0 + b + j];
+#pragma unroll
+                for (int k = 0; k < BT_ROWS; k++) {
+                    const int i = i0 + k * R;
+                    const double tv = S[(s0 + min(i, b - 1)) * nb + s0 + l];
+                    x[k] = fma(l >= i ? tv : 0.0, xv, x[k]);
+                }
             }
-            val[q] = -x;
         }
         __syncthreads();
+        if (act) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int e = threadIdx.x + 256 * q;
-            if (e < total) {
-                const int pr = e / (b * b), rem = e % (b * b), i = rem / b, j = rem % b;
-                const int s0 = 2 * b * pr, b2 = min(b, nb - s0 - b);
-                if (j < b2) S[(s0 + i) * nb + s0 + b + j] = val[q];
-            }
+            for (int k = 0; k < BT_ROWS; k++)
+                if (i0 + k * R < b) S[(s0 + i0 + k * R) * nb + s0 + b + j] = -x[k];
         }
         __syncthreads();
     }
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < nb * nb; idx += BT_THREADS) {
         const int r = idx / nb, c = idx % nb;
         T[idx] = r <= c ? S[idx] : 0.0;
     }
@@ -474,11 +485,11 @@ hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb,
 }
 
 hipError_t launch_build_t(const double* g, int nb, const double* tau, double* T, hipStream_t stream) {
-    // Each doubling level holds at most 16 entries per thread: nb <= 128 keeps every level within 4096.
+    // nb <= 128 keeps every level's (pair, column) slots within 64, so BT_ROWS rows per thread cover it.
     if (nb < 1 || nb > 128) return hipErrorInvalidValue;
     const size_t lds = (size_t)nb * nb * sizeof(double);
     (void)hipFuncSetAttribute((const void*)build_t, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(build_t, dim3(1), dim3(256), lds, stream, g, nb, tau, T);
+    hipLaunchKernelGGL(build_t, dim3(1), dim3(BT_THREADS), lds, stream, g, nb, tau, T);
     return hipGetLastError();
 }
 
