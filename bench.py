@@ -436,7 +436,8 @@ def bench_qr(args) -> int:
     pms = float(np.mean(panel_ms))
     gbs = pbytes / (pms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "panel_col",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": read_traffic("panel_col_qr_32768x4096"),
+                "kernel": "panel_col",
                 "kernel_ms": round(pms / launches, 5), "launches": launches,
                 "note": f"{pbytes / launches:.4g} algorithmic bytes per launch; the panel kernels take "
                         f"{pms:.1f} of the {forms['fp64']['ms']:.1f} ms (FP64 form)"}
