@@ -348,7 +348,8 @@ def bench_layouts(args) -> int:
         "config": {"workload": "layouts: 8192^3 row-major, transA 4096x8192x16384, colmajorA/B 6000^3, "
                                "matvec 32768^2; uniform[-1,1], alpha=1, beta=0"},
         "roofline": {"bound": "mfma", "achieved": head["tflops"], "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": head["frac_of_fp64_peak"], "traffic": None},
+                     "unit": "TFLOP/s", "frac": head["frac_of_fp64_peak"],
+                     "traffic": read_traffic("dgemm_f64_mfma_8192"), "kernel": "dgemm_f64_mfma"},
         "cpu_baseline": cpu, "cases": results}), flush=True)
     return 0
 
