@@ -7,7 +7,12 @@
  * thread-local message for the last failing call on the calling thread.
  * All entry points are thread-safe: each host thread gets its own HIP stream
  * and its own staging buffers, so concurrent callers (the reference calls
- * its GEMM from many ForkJoin threads) never share mutable state.
+ * its GEMM from many ForkJoin threads) never share mutable state. One thread's
+ * device-pointer calls may run on different streams: the split-form and
+ * split-K workspace the thread's calls share is ordered across streams (a
+ * call on another stream than the workspace's last user waits for that
+ * user's launches on the device). A stream passed to the library must stay
+ * alive until the work queued on it has finished.
  *
  * Index conventions (shared by every GEMM entry point). C is always row-major:
  * element (i,j) of C is C[i*ldc + j]. For op(A) (m x k) and op(B) (k x n):
@@ -216,6 +221,12 @@ int jlc_dev_copy(void* dst, const void* src, size_t bytes, void* stream);
 /* The calling thread's library stream (what a NULL stream argument means). */
 int jlc_stream_get(void** stream);
 int jlc_stream_synchronize(void* stream);
+/* A new non-blocking stream (default priority) on the library's device, for
+ * callers that queue work on several streams; jlc_stream_destroy releases it
+ * (the thread's library stream cannot be destroyed). No counterpart in the
+ * reference, whose CudaGemm uses the default stream (CudaGemm.java:85-112). */
+int jlc_stream_create(void** stream);
+int jlc_stream_destroy(void* stream);
 
 /* HIP-event timers for callers that time work on a stream (benchmarks). */
 typedef struct jlc_timer jlc_timer;

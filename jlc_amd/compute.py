@@ -158,12 +158,14 @@ class DispatchPolicy:
         return self.parallelism
 
     def shouldOffloadToHip(self, m: int, n: int, k: int) -> bool:
-        """Analogue of shouldOffloadToCuda (DispatchPolicy.java:219-236)."""
+        """Analogue of shouldOffloadToCuda (DispatchPolicy.java:219-236): the smallest
+        dimension against hip_min_dim, the elements of all three operands (m*k + k*n + m*n)
+        against hip_min_elements, then 2*m*n*k against hip_min_flops."""
         if not self.enable_hip:
             return False
-        if min(m, n, k) < self.hip_min_dim and n != 1:
+        if min(m, n, k) < self.hip_min_dim:
             return False
-        if m * n < self.hip_min_elements:
+        if m * k + k * n + m * n < self.hip_min_elements:
             return False
         return 2 * m * n * k >= self.hip_min_flops
 

@@ -81,6 +81,11 @@ struct ThreadCtx {
     // Pinned host staging for the host-pointer entry points.
     void* pin = nullptr;
     size_t pin_cap = 0;
+    // The split-form and split-K workspace (buf[3..7]) is shared by every call of this thread,
+    // whatever stream the call runs on: ws_ev is recorded after the last launch that used it,
+    // on ws_stream, and a call on another stream waits for it before touching the buffers.
+    hipEvent_t ws_ev = nullptr;
+    hipStream_t ws_stream = nullptr;
 
     ~ThreadCtx() {
         if (!g_alive.load() || device < 0) return;
@@ -91,6 +96,7 @@ struct ThreadCtx {
         if (ev1) (void)hipEventDestroy(ev1);
         if (ev2) (void)hipEventDestroy(ev2);
         if (ev3) (void)hipEventDestroy(ev3);
+        if (ws_ev) (void)hipEventDestroy(ws_ev);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -126,12 +132,45 @@ int ensure_init() {
         JLC_HIP_CHECK(hipEventCreate(&c.ev1));
         JLC_HIP_CHECK(hipEventCreate(&c.ev2));
         JLC_HIP_CHECK(hipEventCreate(&c.ev3));
+        JLC_HIP_CHECK(hipEventCreateWithFlags(&c.ws_ev, hipEventDisableTiming));
         c.device = dev;
     } else {
         JLC_HIP_CHECK(hipSetDevice(dev));
     }
     return JLC_OK;
 }
+
+// Orders work on stream s after every earlier launch of this thread that used the shared
+// workspace on another stream (a no-op when the last user ran on s: a stream is ordered).
+int ws_acquire(hipStream_t s) {
+    ThreadCtx& c = t_ctx;
+    if (c.ws_stream && c.ws_stream != s) JLC_HIP_CHECK(hipStreamWaitEvent(s, c.ws_ev, 0));
+    return JLC_OK;
+}
+
+// Marks the end of this call's workspace use on stream s.
+int ws_release(hipStream_t s) {
+    ThreadCtx& c = t_ctx;
+    JLC_HIP_CHECK(hipEventRecord(c.ws_ev, s));
+    c.ws_stream = s;
+    return JLC_OK;
+}
+
+// Scope of one call's workspace use: acquire() before the first launch that touches buf[3..7],
+// the release recorded when the call's launches have all been queued.
+struct WorkspaceUse {
+    hipStream_t s;
+    bool used = false;
+    explicit WorkspaceUse(hipStream_t s_) : s(s_) {}
+    int acquire() {
+        if (used) return JLC_OK;
+        used = true;
+        return ws_acquire(s);
+    }
+    ~WorkspaceUse() {
+        if (used) (void)ws_release(s);
+    }
+};
 
 int ensure_buf(int slot, size_t bytes) {
     ThreadCtx& c = t_ctx;
@@ -493,6 +532,46 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
     return JLC_OK;
 }
 
+// The FP64 kernels of a call with alpha != 0 and k > 0: the matvec when n == 1 and op(A) is
+// row-major (the n == 1 regime of GemmDispatch.selectKernel, GemmDispatch.java:80-82; x is the B
+// column, y the C column), else the FP64 MFMA kernel, split over K for long-K / few-tile shapes. The
+// split-K partial products go to the thread's grow-only workspace (ordered across streams by ws), or,
+// when `fixed` is given, to that caller-owned buffer of fixed_bytes; a call whose partials do not fit
+// it fails before anything is launched.
+int run_fp64(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+             const double* b, int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t stream,
+             jlc::TileInfo* info, int* kernel, WorkspaceUse* ws, double* fixed, size_t fixed_bytes) {
+    if (n == 1 && la == JLC_ROW) {
+        *kernel = JLC_KERNEL_GEMV;
+        int64_t incx = lb == JLC_ROW ? ldb : 1;
+        JLC_HIP_CHECK(jlc::launch_dgemv(m, k, alpha, a, lda, b, incx, beta, c, ldc, stream, info));
+        return JLC_OK;
+    }
+    *kernel = JLC_KERNEL_MFMA_F64;
+    const jlc::TuneTable tt = jlc::tune_snapshot();
+    const int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load(), tt.splitk_tiles_per_cu, tt.splitk_min_kslice);
+    if (slices > 1) {
+        const size_t need = (size_t)slices * (size_t)(m * n) * sizeof(double);
+        double* work = fixed;
+        if (fixed) {
+            if (need > fixed_bytes)
+                return fail(JLC_ERR_UNSUPPORTED,
+                            "split-K partials of %lld x %lld x %lld (%d slices) need %zu bytes, above the %zu-byte "
+                            "scratch", (long long)m, (long long)n, (long long)k, slices, need, fixed_bytes);
+        } else {
+            int rc = ws->acquire();
+            if (rc != JLC_OK) return rc;
+            if ((rc = ensure_buf(6, need)) != JLC_OK) return rc;
+            work = static_cast<double*>(t_ctx.buf[6]);
+        }
+        JLC_HIP_CHECK(jlc::launch_dgemm_f64_splitk(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices,
+                                                   work, stream, info));
+        return JLC_OK;
+    }
+    JLC_HIP_CHECK(jlc::launch_dgemm_f64(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, info));
+    return JLC_OK;
+}
+
 // Device-side dispatch shared by the host and device entry points.
 int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
                     const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
@@ -502,7 +581,12 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
         JLC_HIP_CHECK(jlc::launch_scale(m, n, beta, c, ldc, stream, info));
         return JLC_OK;
     }
+    WorkspaceUse ws(stream);
     int form = mode & 0xff;
+    if (form != JLC_MODE_FP64) {
+        int rc = ws.acquire();
+        if (rc != JLC_OK) return rc;
+    }
     if (form == JLC_MODE_SPLIT_CRT) {
         bool fallback = false;
         *kernel = JLC_KERNEL_SPLIT_CRT;
@@ -520,26 +604,7 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
         if (rc != JLC_OK || !fallback) return rc;
         // spread too wide for the form's slices: the FP64 kernel serves the call
     }
-    if (n == 1 && la == JLC_ROW) {
-        // The n == 1 regime is the matvec (GemmDispatch.selectKernel, GemmDispatch.java:80-82):
-        // x is the B column, y the C column.
-        *kernel = JLC_KERNEL_GEMV;
-        int64_t incx = lb == JLC_ROW ? ldb : 1;
-        JLC_HIP_CHECK(jlc::launch_dgemv(m, k, alpha, a, lda, b, incx, beta, c, ldc, stream, info));
-        return JLC_OK;
-    }
-    *kernel = JLC_KERNEL_MFMA_F64;
-    const jlc::TuneTable tt = jlc::tune_snapshot();
-    const int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load(), tt.splitk_tiles_per_cu, tt.splitk_min_kslice);
-    if (slices > 1) {
-        int rc = ensure_buf(6, (size_t)slices * (size_t)(m * n) * sizeof(double));
-        if (rc != JLC_OK) return rc;
-        JLC_HIP_CHECK(jlc::launch_dgemm_f64_splitk(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices,
-                                                   static_cast<double*>(t_ctx.buf[6]), stream, info));
-        return JLC_OK;
-    }
-    JLC_HIP_CHECK(jlc::launch_dgemm_f64(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, info));
-    return JLC_OK;
+    return run_fp64(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, info, kernel, &ws, nullptr, 0);
 }
 
 void fill_stats(jlc_stats* s, const jlc::TileInfo& info, int kernel, double kms, double tms) {
@@ -984,6 +1049,26 @@ int jlc_stream_get(void** stream) {
     int rc = ensure_init();
     if (rc != JLC_OK) return rc;
     *stream = t_ctx.stream;
+    return JLC_OK;
+}
+
+int jlc_stream_create(void** stream) {
+    if (!stream) return fail(JLC_ERR_INVALID_ARG, "stream must not be null");
+    *stream = nullptr;
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = nullptr;
+    JLC_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return JLC_OK;
+}
+
+int jlc_stream_destroy(void* stream) {
+    if (!stream) return JLC_OK;
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    if (stream == t_ctx.stream) return fail(JLC_ERR_INVALID_ARG, "the thread's library stream is not the caller's to destroy");
+    JLC_HIP_CHECK(hipStreamDestroy(static_cast<hipStream_t>(stream)));
     return JLC_OK;
 }
 

@@ -1,6 +1,8 @@
 // C++ host mirror: implementation over the C ABI (include/jlc_hip.h).
 #include "jlc_compute.hpp"
 
+#include <algorithm>
+
 #include <atomic>
 
 #include "../../include/jlc_hip.h"
@@ -69,10 +71,11 @@ DispatchPolicy DispatchPolicy::defaultPolicy() {
 }
 
 bool DispatchPolicy::shouldOffloadToHip(int64_t m, int64_t n, int64_t k) const {
-    // shouldOffloadToCuda analogue (DispatchPolicy.java:219-236).
+    // shouldOffloadToCuda analogue (DispatchPolicy.java:219-236): the smallest dimension against
+    // the minimum, the element count of all three operands against the minimum, then the flops.
     if (!enable_hip_) return false;
-    if (n != 1 && (m < hip_min_dim_ || n < hip_min_dim_ || k < hip_min_dim_)) return false;
-    if (m * n < hip_min_elements_) return false;
+    if (std::min(m, std::min(n, k)) < hip_min_dim_) return false;
+    if (m * k + k * n + m * n < hip_min_elements_) return false;
     return 2 * m * n * k >= hip_min_flops_;
 }
 

@@ -115,6 +115,8 @@ _SIGNATURES = {
     "jlc_dev_copy": (_i, [_p, _p, C.c_size_t, _p]),
     "jlc_stream_get": (_i, [C.POINTER(_p)]),
     "jlc_stream_synchronize": (_i, [_p]),
+    "jlc_stream_create": (_i, [C.POINTER(_p)]),
+    "jlc_stream_destroy": (_i, [_p]),
     "jlc_timer_create": (_i, [C.POINTER(_p)]),
     "jlc_timer_start": (_i, [_p, _p]),
     "jlc_timer_stop": (_i, [_p, _p, C.POINTER(_d)]),
@@ -294,6 +296,20 @@ def stream() -> int:
     p = C.c_void_p()
     check(load().jlc_stream_get(C.byref(p)))
     return p.value
+
+
+def stream_synchronize(stream_handle: int | None = None) -> None:
+    check(load().jlc_stream_synchronize(stream_handle))
+
+
+def stream_create() -> int:
+    p = C.c_void_p()
+    check(load().jlc_stream_create(C.byref(p)))
+    return p.value
+
+
+def stream_destroy(stream_handle: int) -> None:
+    check(load().jlc_stream_destroy(stream_handle))
 
 
 class Timer:

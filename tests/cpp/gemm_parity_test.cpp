@@ -98,7 +98,14 @@ static void cpu_tests() {
         CHECK(GemmDispatch::selectKernel(512, 512, 512, true, 1, off) == K::MICROKERNEL, "disabled");
         const auto dim = DispatchPolicy::builder().hipMinFlops(0).hipMinDim(64).build();
         CHECK(GemmDispatch::selectKernel(32, 128, 128, true, 1, dim) == K::MICROKERNEL, "min dim");
-        CHECK(GemmDispatch::selectKernel(128, 1, 128, true, 1, dim) == K::HIP, "matvec bypasses min dim");
+        // shouldOffloadToCuda (DispatchPolicy.java:226-235) applies the minimum dimension to n == 1 too.
+        CHECK(GemmDispatch::selectKernel(128, 1, 128, true, 1, dim) == K::MATVEC, "matvec under min dim");
+        // The element gate counts all three operands, m*k + k*n + m*n: a long-k product with a small C
+        // passes it (8*100000 + 100000*8 + 64 = 1,600,064 elements) ...
+        const auto elems = DispatchPolicy::builder().hipMinFlops(0).hipMinElements(1000000).build();
+        CHECK(GemmDispatch::selectKernel(8, 8, 100000, true, 1, elems) == K::HIP, "long-k elements");
+        // ... and a product whose three operands hold fewer elements does not (3 * 512^2 = 786,432).
+        CHECK(GemmDispatch::selectKernel(512, 512, 512, true, 4, elems) == K::PARALLEL_MICRO, "elements gate");
         CHECK(GemmDispatch::selectKernel(4, 4, 4, true, 1, HIP_POLICY) == K::HIP, "gate 0 routes all");
     });
     run("GemmParityTest.rejects_null_and_mismatch_with_message", [] {

@@ -26,3 +26,27 @@ def test_traffic_summaries_parse():
         v = bench.read_traffic(tag)
         assert v is not None and lo <= v <= hi, (tag, v)
     assert bench.read_traffic("no_such_kernel") is None
+
+
+def _form(name, tflops, kernel, slices, rel):
+    return {"form": name, "tflops": tflops, "kernel": kernel, "slices": slices, "error": {"max_rel_err": rel}}
+
+
+def test_value_form_skips_fp64_fallbacks():
+    """A split form answered by the FP64 kernel (automatic mode, spread too wide) has FP64's error, so
+    it would pass the error rule; it must never be picked, however the timing noise falls."""
+    import bench
+    fp64 = _form("fp64_mfma", 67.0, "mfma_f64", 0, 1e-12)
+    fallback = _form("split_bf16", 67.4, "mfma_f64", 0, 1e-12)
+    assert not bench.split_ran(fallback)
+    assert bench.choose_value_form(fp64, [fallback]) is fp64
+    crt = _form("split_crt", 120.0, "split_crt", 15, 5e-13)
+    assert bench.split_ran(crt)
+    assert bench.choose_value_form(fp64, [fallback, crt]) is crt
+    # a faster split form with a larger error than FP64's is not the value form
+    loose = _form("split_int8", 150.0, "split_int8", 5, 2e-12)
+    assert bench.choose_value_form(fp64, [loose, crt]) is crt
+    assert bench.choose_value_form(fp64, [loose], check_error=False) is loose
+    # slower split forms lose to FP64
+    slow = _form("split_int8", 50.0, "split_int8", 9, 1e-13)
+    assert bench.choose_value_form(fp64, [slow]) is fp64

@@ -62,3 +62,48 @@ def test_many_threads_match_sequential_results():
         assert got is not None, i
         assert got[1] == want[1], (i, got[1], want[1])
         assert np.array_equal(got[0], want[0]), i
+
+
+def test_one_thread_two_streams_match_sequential_results():
+    """One thread's device-pointer calls on two streams share the thread's split-form and split-K
+    workspace (residue planes, slices, partial sums). With explicit moduli / slice counts the calls are
+    asynchronous, so the second stream's call must wait for the first stream's kernels before it
+    overwrites that workspace. Results must equal the same calls made one at a time, bit for bit."""
+    rng = np.random.default_rng(17)
+    crt, i8 = native.mode_slices(native.MODE_SPLIT_CRT, 12), native.mode_slices(native.MODE_SPLIT_INT8, 5)
+    # (m, n, k, mode): the first case of each pair sizes the workspace, the second reuses it smaller
+    shapes = [(1536, 1536, 1536, crt), (1024, 1024, 1024, crt), (1280, 1280, 1280, i8), (768, 896, 640, i8),
+              (128, 128, 65536, native.MODE_FP64), (96, 112, 40000, native.MODE_FP64)]
+    cases = []
+    for m, n, k, mode in shapes:
+        A = rng.uniform(-1, 1, m * k)
+        B = rng.uniform(-1, 1, k * n)
+        cases.append((m, n, k, mode, native.DeviceArray.from_numpy(A), native.DeviceArray.from_numpy(B),
+                      native.DeviceArray(m * n)))
+
+    def call(case, stream, stats):
+        m, n, k, mode, da, db, dc = case
+        return native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, dc.ptr, n, mode,
+                                stream, stats=stats)
+
+    expected = []
+    for case in cases:  # one at a time, largest of each pair first
+        st = call(case, None, True)
+        expected.append((case[6].to_numpy(), st.kernel_name))
+    assert [e[1] for e in expected] == ["split_crt", "split_crt", "split_int8", "split_int8", "mfma_f64",
+                                        "mfma_f64"]
+    s1, s2 = native.stream_create(), native.stream_create()
+    try:
+        for rep in range(2):
+            for case in cases:  # clear C so stale results cannot pass
+                native.DeviceArray.copy_from(case[6], np.full(case[0] * case[1], np.nan))
+            for i in range(0, len(cases), 2):
+                call(cases[i], s1, False)       # asynchronous on stream 1
+                call(cases[i + 1], s2, False)   # same thread, same workspace, stream 2
+            native.stream_synchronize(s1)
+            native.stream_synchronize(s2)
+            for i, case in enumerate(cases):
+                assert np.array_equal(case[6].to_numpy(), expected[i][0]), (rep, i, case[:4])
+    finally:
+        native.stream_destroy(s1)
+        native.stream_destroy(s2)

@@ -39,6 +39,22 @@ def test_select_kernel_hip_gate():
     assert GemmDispatch.selectKernel(128, 4096, 4096, True, 1, dim) is not K.HIP
 
 
+def test_hip_gate_matches_should_offload_to_cuda():
+    """shouldOffloadToCuda (DispatchPolicy.java:226-235): minDim applies to every shape, n == 1
+    included, and the element gate counts m*k + k*n + m*n."""
+    dim = DispatchPolicy.builder().hipMinFlops(0).hipMinDim(64).build()
+    assert not dim.shouldOffloadToHip(128, 1, 128)
+    assert GemmDispatch.selectKernel(128, 1, 128, True, 1, dim) is K.MATVEC
+    assert dim.shouldOffloadToHip(64, 64, 64)
+    elems = DispatchPolicy.builder().hipMinFlops(0).hipMinElements(1_000_000).build()
+    # long k, small C: 8*100000 + 100000*8 + 8*8 = 1,600,064 elements pass the gate
+    assert elems.shouldOffloadToHip(8, 8, 100_000)
+    assert GemmDispatch.selectKernel(8, 8, 100_000, True, 1, elems) is K.HIP
+    # 3 * 512^2 = 786,432 elements stay below it
+    assert not elems.shouldOffloadToHip(512, 512, 512)
+    assert elems.shouldOffloadToHip(1000, 1000, 1)  # 1000*1 + 1*1000 + 1000*1000 = 1,002,000
+
+
 def test_rejects_null_and_dimension_mismatch():
     # GemmParityTest.gemmRejectsNullAndDimensionMismatch (GemmParityTest.java:104-115)
     a, b, c = Matrix(2, 3), Matrix(4, 2), Matrix(2, 2)
