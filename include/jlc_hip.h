@@ -120,6 +120,23 @@ int jlc_dgemm_dev(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, d
                   int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, int mode,
                   void* stream, jlc_stats* stats);
 
+/* ---- FP64 GEMM on device memory that allocates nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as jlc_dgemm_dev with mode JLC_MODE_FP64 (the
+ * same kernels, bit for bit); stream NULL is the null stream; always
+ * asynchronous. The split-K partial products that jlc_dgemm_dev keeps in a
+ * per-thread buffer go to a 256 MiB device array the library holds on each
+ * device for the life of the process (its address is resolved once per device
+ * by jlc_hip_init, which this call runs for device 0 if nothing has). A shape
+ * whose partials exceed it fails with JLC_ERR_UNSUPPORTED before anything is
+ * launched (the shipped tuning never makes one). Every call on a device
+ * shares that array: calls that overlap on one device (other threads or
+ * streams) race on it. */
+int jlc_dgemm_dev_noalloc(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha,
+                          const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
+                          int64_t ldc, void* stream);
+
 /* ---- MATVEC: y = alpha*A*x + beta*y, A row-major m x k (lda). Replaces
  * SpecializedKernels.matvec (src/main/java/net/faulj/compute/SpecializedKernels.java:27-71),
  * the n == 1 regime of GemmDispatch.selectKernel (GemmDispatch.java:80-82).

@@ -37,6 +37,17 @@ std::atomic<bool> g_alive{true};
 
 thread_local std::string t_error;
 
+// Scratch of the no-allocation entry point (jlc_dgemm_dev_noalloc): the split-K partial products of
+// the FP64 MFMA kernel. Every device the library runs on holds its own copy for the life of the
+// process; host code never uses the array's name, only the device address jlc_hip_init resolves per
+// device with hipGetSymbolAddress (g_scratch_ptr). 256 MiB holds every split-K call the shipped
+// tuning (4 tiles per CU, 256 CUs, at most 64 slices) can make: slices * m * n <= ceil(1024 / t) * t *
+// 128^2 doubles for t output tiles, at most 1533 * 2^14 doubles (201 MB) at t = 511.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)256 << 20;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+std::atomic<void*> g_scratch_ptr[MAX_DEVICES];
+
 std::mutex g_tune_mutex;
 jlc::TuneTable g_tune;
 std::once_flag g_tune_autoload;
@@ -798,6 +809,13 @@ int jlc_hip_init(int device) {
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
             g_cus.store(prop.multiProcessorCount);
     }
+    // Once per device, with the device current: the device address of this device's scratch copy.
+    if (device < MAX_DEVICES && !g_scratch_ptr[device].load()) {
+        void* ptr = nullptr;
+        JLC_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(JLC_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[device].store(ptr);
+    }
     static bool registered = false;
     if (!registered) {
         std::atexit(mark_dead);
@@ -856,6 +874,30 @@ int jlc_dgemm_dev(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, d
     }
     fill_stats(stats, info, kernel, kms, now_ms() - t0);
     return JLC_OK;
+}
+
+int jlc_dgemm_dev_noalloc(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha,
+                          const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
+                          int64_t ldc, void* stream) {
+    int rc = validate_gemm(layout_a, layout_b, m, n, k, alpha, a, lda, b, ldb, c, ldc, JLC_MODE_FP64);
+    if (rc != JLC_OK) return rc;
+    if (m == 0 || n == 0) return JLC_OK;
+    if (!env_flag("JLC_HIP_ENABLED", true)) return fail(JLC_ERR_DISABLED, "JLC_HIP_ENABLED disables the HIP path");
+    if (g_device.load() < 0 && (rc = jlc_hip_init(0)) != JLC_OK) return rc;
+    const int dev = g_device.load();
+    void* scr = dev < MAX_DEVICES ? g_scratch_ptr[dev].load() : nullptr;
+    if (!scr) return fail(JLC_ERR_UNSUPPORTED, "device %d has no resolved scratch (device index above %d?)", dev,
+                          MAX_DEVICES - 1);
+    JLC_HIP_CHECK(hipSetDevice(dev));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    jlc::TileInfo info;
+    int kernel = JLC_KERNEL_NONE;
+    if (alpha == 0.0 || k == 0) {
+        JLC_HIP_CHECK(jlc::launch_scale(m, n, beta, c, ldc, s, &info));
+        return JLC_OK;
+    }
+    return run_fp64(layout_a, layout_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, &info, &kernel, nullptr,
+                    static_cast<double*>(scr), NOALLOC_SCRATCH_BYTES);
 }
 
 int jlc_dgemm(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a,

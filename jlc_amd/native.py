@@ -105,6 +105,7 @@ _SIGNATURES = {
     "jlc_dgemm": (_i, [_i, _i, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i, C.POINTER(Stats)]),
     "jlc_dgemm_dev": (_i, [_i, _i, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i, _p,
                            C.POINTER(Stats)]),
+    "jlc_dgemm_dev_noalloc": (_i, [_i, _i, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p]),
     "jlc_dgemv": (_i, [_i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, C.POINTER(Stats)]),
     "jlc_dgemv_dev": (_i, [_i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p, C.POINTER(Stats)]),
     "jlc_dev_alloc": (_i, [C.POINTER(_p), C.c_size_t]),
