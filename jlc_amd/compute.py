@@ -17,6 +17,9 @@ reference's own tests:
   gemmStrided
 * BLAS3Kernels.gemm* /         BLAS3Kernels.java:35-139, :244-303
   gemmStrided*
+* Matrix.multiply /             Matrix.java:1120-1122; BlockedMultiply.java:43-105, with
+  BlockedMultiply.multiply      DispatchPolicy.selectForMultiply (:179-193, :254-279) and
+                                CudaGemm.multiply (CudaGemm.java:32-41) -> HipGemm.multiply
 
 This build contains no CPU GEMM. Where the reference would run one of its own
 CPU regimes (TINY / SMALL / MICROKERNEL / PARALLEL_MICRO), the mirror raises
@@ -86,6 +89,15 @@ class Matrix:
     def to_numpy(self) -> np.ndarray:
         return self._data.reshape(self._rows, self._cols)
 
+    @staticmethod
+    def zero(rows: int, cols: int) -> "Matrix":
+        """Matrix.zero (Matrix.java:1202-1204)."""
+        return Matrix(rows, cols)
+
+    def multiply(self, other: "Matrix") -> "Matrix":
+        """Matrix.multiply (Matrix.java:1120-1122): BlockedMultiply.multiply under the default policy."""
+        return BlockedMultiply.multiply(self, other)
+
 
 class OffHeapMatrix(Matrix):
     """FFM-segment-backed operand with a heap mirror (OffHeapMatrix.java:15-314).
@@ -129,6 +141,19 @@ class OffHeapMatrix(Matrix):
 # ----------------------------------------------------------------------------
 # Policy and dispatch
 
+class Algorithm(enum.Enum):
+    """DispatchPolicy.Algorithm (DispatchPolicy.java:58-67) with HIP beside CUDA."""
+    NAIVE = "NAIVE"
+    BLOCKED = "BLOCKED"
+    BLAS3 = "BLAS3"
+    SIMD = "SIMD"
+    CUDA = "CUDA"
+    PARALLEL = "PARALLEL"
+    STRASSEN = "STRASSEN"
+    SPECIALIZED = "SPECIALIZED"
+    HIP = "HIP"
+
+
 @dataclass(frozen=True)
 class DispatchPolicy:
     """Subset of DispatchPolicy (DispatchPolicy.java:57-679) that the GEMM route reads.
@@ -144,6 +169,18 @@ class DispatchPolicy:
     hip_min_flops: int = 200_000_000
     parallelism: int = 1
     mode: int = MODE_FP64
+    # Multiply-route thresholds, the reference's builder defaults (DispatchPolicy.java:424-437).
+    naive_threshold: int = 64
+    blocked_threshold: int = 256
+    strassen_threshold: int = 1024
+    enable_blas3: bool = True
+    blas3_threshold: int = 256
+    enable_simd: bool = True
+    simd_threshold: int = 256
+    enable_strassen: bool = False
+    simd_available: bool = True
+    # cudaDetector analogue (DispatchPolicy.java:440): None asks HipSupport.
+    hip_detector: object = None
 
     @staticmethod
     def builder() -> "_PolicyBuilder":
@@ -168,6 +205,35 @@ class DispatchPolicy:
         if m * k + k * n + m * n < self.hip_min_elements:
             return False
         return 2 * m * n * k >= self.hip_min_flops
+
+    def _hip_available(self) -> bool:
+        return bool(self.hip_detector()) if self.hip_detector is not None else HipSupport.isHipAvailable()
+
+    def _select_base(self, m: int, n: int, k: int, allow_hip: bool) -> Algorithm:
+        """selectBaseAlgorithm (DispatchPolicy.java:254-279) with the HIP gate in the CUDA slot."""
+        max_dim = max(m, n, k)
+        simd = self.enable_simd and max_dim < self.simd_threshold and self.simd_available
+        if max_dim <= self.naive_threshold:
+            return Algorithm.SIMD if simd else Algorithm.NAIVE
+        if allow_hip and self.enable_hip and self._hip_available() and self.shouldOffloadToHip(m, n, k):
+            return Algorithm.HIP
+        if self.enable_strassen and m == n == k and max_dim >= self.strassen_threshold:
+            return Algorithm.STRASSEN
+        if self.enable_blas3 and max_dim >= self.blas3_threshold:
+            return Algorithm.BLAS3
+        if simd:
+            return Algorithm.SIMD
+        if max_dim >= self.blocked_threshold:
+            return Algorithm.BLOCKED
+        return Algorithm.NAIVE
+
+    def selectForMultiply(self, m: int, n: int, k: int) -> Algorithm:
+        """selectForMultiply (DispatchPolicy.java:179-181)."""
+        return self._select_base(m, n, k, True)
+
+    def selectCpuAlgorithm(self, m: int, n: int, k: int) -> Algorithm:
+        """selectCpuAlgorithm (DispatchPolicy.java:191-193)."""
+        return self._select_base(m, n, k, False)
 
 
 class _PolicyBuilder:
@@ -196,6 +262,14 @@ class _PolicyBuilder:
 
     def mode(self, v: int):
         self._kw["mode"] = int(v)
+        return self
+
+    def naiveThreshold(self, v: int):
+        self._kw["naive_threshold"] = int(v)
+        return self
+
+    def hipDetector(self, fn):
+        self._kw["hip_detector"] = fn
         return self
 
     def build(self) -> DispatchPolicy:
@@ -275,6 +349,13 @@ class HipGemm:
         except native.JlcError:
             return False
         return True
+
+    @staticmethod
+    def multiply(a: Matrix, b: Matrix, mode: int = MODE_FP64) -> "Matrix | None":
+        """CudaGemm.multiply (CudaGemm.java:32-41): a new heap Matrix C = A B, or None when the device
+        call fails (the caller then runs its CPU algorithm)."""
+        c = Matrix.zero(a.getRowCount(), b.getColumnCount())
+        return c if HipGemm.gemm(a, b, c, 1.0, 0.0, mode) else None
 
 
 def _require_hip(kernel: "GemmDispatch.Kernel") -> None:
@@ -391,6 +472,36 @@ class BLAS3Kernels:
         if m <= 0 or n <= 0:
             return
         native.dgemm(ROW, COL, m, n, k, alpha, ad, lda, bd, ldb, beta, cd, ldc, mode, aOff, bOff, cOff)
+
+
+class BlockedMultiply:
+    """BlockedMultiply.multiply (BlockedMultiply.java:43-105), the product behind Matrix.multiply."""
+
+    @staticmethod
+    def multiply(a: Matrix, b: Matrix, policy: DispatchPolicy | None = None) -> Matrix:
+        if a is None or b is None:
+            raise IllegalArgumentException("Matrices must not be null")
+        m, k = a.getRowCount(), a.getColumnCount()
+        k2, n = b.getRowCount(), b.getColumnCount()
+        if k != k2:
+            raise IllegalArgumentException(f"Inner dimensions must agree for multiplication: {k} != {k2}")
+        if m == 0 or n == 0 or k == 0:
+            off = isinstance(a, OffHeapMatrix) or isinstance(b, OffHeapMatrix)
+            return OffHeapMatrix(m, n) if off else Matrix.zero(m, n)
+        if n == 1:
+            raise UnsupportedKernel("multiply: n == 1 runs multiplyVector, the Java host's CPU matvec "
+                                    "(BlockedMultiply.java:71-73)")
+        active = policy or DispatchPolicy.defaultPolicy()
+        algorithm = active.selectForMultiply(m, n, k)
+        if algorithm is Algorithm.HIP:
+            c = HipGemm.multiply(a, b, active.mode)
+            if c is not None:
+                return c
+            algorithm = active.selectCpuAlgorithm(m, n, k)
+            raise UnsupportedKernel(f"multiply: the HIP call failed ({native.last_error()}); the reference "
+                                    f"then runs its CPU algorithm {algorithm.value}")
+        raise UnsupportedKernel(f"multiply: dispatch selected the CPU algorithm {algorithm.value}; CPU "
+                                f"algorithms are the Java host's own code")
 
 
 HIP_POLICY = DispatchPolicy.builder().enableHip(True).hipMinFlops(0).parallelism(1).build()

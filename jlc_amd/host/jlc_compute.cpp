@@ -79,6 +79,33 @@ bool DispatchPolicy::shouldOffloadToHip(int64_t m, int64_t n, int64_t k) const {
     return 2 * m * n * k >= hip_min_flops_;
 }
 
+const char* DispatchPolicy::name(Algorithm a) {
+    switch (a) {
+        case Algorithm::NAIVE: return "NAIVE";
+        case Algorithm::BLOCKED: return "BLOCKED";
+        case Algorithm::BLAS3: return "BLAS3";
+        case Algorithm::SIMD: return "SIMD";
+        case Algorithm::CUDA: return "CUDA";
+        case Algorithm::PARALLEL: return "PARALLEL";
+        case Algorithm::STRASSEN: return "STRASSEN";
+        case Algorithm::SPECIALIZED: return "SPECIALIZED";
+        default: return "HIP";
+    }
+}
+
+DispatchPolicy::Algorithm DispatchPolicy::selectBase(int m, int n, int k, bool allowHip) const {
+    const int maxDim = std::max(m, std::max(n, k));
+    const bool simd = enable_simd_ && maxDim < simd_threshold_ && simd_available_;
+    if (maxDim <= naive_threshold_) return simd ? Algorithm::SIMD : Algorithm::NAIVE;
+    const bool hip = hip_detected_ < 0 ? HipSupport::isHipAvailable() : hip_detected_ == 1;
+    if (allowHip && hip && shouldOffloadToHip(m, n, k)) return Algorithm::HIP;
+    if (enable_strassen_ && m == n && n == k && maxDim >= strassen_threshold_) return Algorithm::STRASSEN;
+    if (enable_blas3_ && maxDim >= blas3_threshold_) return Algorithm::BLAS3;
+    if (simd) return Algorithm::SIMD;
+    if (maxDim >= blocked_threshold_) return Algorithm::BLOCKED;
+    return Algorithm::NAIVE;
+}
+
 GemmDispatch::Kernel GemmDispatch::selectKernel(int m, int n, int k, bool hipAvailable, int threads,
                                                 const DispatchPolicy& policy) {
     if (hipAvailable && policy.shouldOffloadToHip(m, n, k)) return Kernel::HIP;
@@ -122,6 +149,39 @@ bool HipGemm::gemm(const Matrix* a, const Matrix* b, Matrix* c, double alpha, do
                              b->getRawData().data(), n, beta, c->getRawData().data(), n, mode, nullptr);
     return rc == JLC_OK;
 }
+
+std::unique_ptr<Matrix> HipGemm::multiply(const Matrix* a, const Matrix* b, int mode) {
+    auto c = std::make_unique<Matrix>(a->getRowCount(), b->getColumnCount());
+    if (!gemm(a, b, c.get(), 1.0, 0.0, mode)) return nullptr;
+    return c;
+}
+
+std::unique_ptr<Matrix> BlockedMultiply::multiply(const Matrix* a, const Matrix* b, const DispatchPolicy* policy) {
+    if (!a || !b) throw IllegalArgumentException("Matrices must not be null");
+    const int m = a->getRowCount(), k = a->getColumnCount(), k2 = b->getRowCount(), n = b->getColumnCount();
+    if (k != k2)
+        throw IllegalArgumentException("Inner dimensions must agree for multiplication: " + std::to_string(k) +
+                                       " != " + std::to_string(k2));
+    if (m == 0 || n == 0 || k == 0) {
+        if (dynamic_cast<const OffHeapMatrix*>(a) || dynamic_cast<const OffHeapMatrix*>(b))
+            return std::make_unique<OffHeapMatrix>(m, n);
+        return std::make_unique<Matrix>(m, n);
+    }
+    if (n == 1)
+        throw UnsupportedKernel("multiply: n == 1 runs multiplyVector, the Java host's CPU matvec");
+    const DispatchPolicy active = policy ? *policy : DispatchPolicy::defaultPolicy();
+    DispatchPolicy::Algorithm algorithm = active.selectForMultiply(m, n, k);
+    if (algorithm == DispatchPolicy::Algorithm::HIP) {
+        if (auto c = HipGemm::multiply(a, b, active.mode())) return c;
+        algorithm = active.selectCpuAlgorithm(m, n, k);
+        throw UnsupportedKernel(std::string("multiply: the HIP call failed (") + jlc_last_error() +
+                                "); the reference then runs its CPU algorithm " + DispatchPolicy::name(algorithm));
+    }
+    throw UnsupportedKernel(std::string("multiply: dispatch selected the CPU algorithm ") +
+                            DispatchPolicy::name(algorithm));
+}
+
+std::unique_ptr<Matrix> Matrix::multiply(const Matrix& other) const { return BlockedMultiply::multiply(this, &other); }
 
 static void hip_or_throw(const Matrix* a, const Matrix* b, Matrix* c, double alpha, double beta, int mode) {
     if (!HipGemm::gemm(a, b, c, alpha, beta, mode)) throw DeviceError(JLC_ERR_HIP, jlc_last_error());

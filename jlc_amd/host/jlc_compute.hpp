@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -43,6 +44,10 @@ class Matrix {
     const std::vector<double>& getRawData() const { return data_; }
     double get(int i, int j) const { return data_[(size_t)i * cols_ + j]; }
     void set(int i, int j, double v) { data_[(size_t)i * cols_ + j] = v; }
+    // Matrix.zero (Matrix.java:1202-1204).
+    static Matrix zero(int rows, int cols) { return Matrix(rows, cols); }
+    // Matrix.multiply (Matrix.java:1120-1122): BlockedMultiply.multiply under the default policy.
+    std::unique_ptr<Matrix> multiply(const Matrix& other) const;
 
    protected:
     int rows_, cols_;
@@ -67,6 +72,9 @@ class OffHeapMatrix : public Matrix {
 // GEMM-route subset of DispatchPolicy (DispatchPolicy.java:57-679) with HIP gates.
 class DispatchPolicy {
    public:
+    // DispatchPolicy.Algorithm (DispatchPolicy.java:58-67) with HIP beside CUDA.
+    enum class Algorithm { NAIVE, BLOCKED, BLAS3, SIMD, CUDA, PARALLEL, STRASSEN, SPECIALIZED, HIP };
+    static const char* name(Algorithm a);
     class Builder;
     static Builder builder();
     // Default policy: the HIP gate comes from the library's tuning table.
@@ -74,14 +82,24 @@ class DispatchPolicy {
     int getParallelism() const { return parallelism_; }
     int mode() const { return mode_; }
     bool shouldOffloadToHip(int64_t m, int64_t n, int64_t k) const;
+    // selectForMultiply / selectCpuAlgorithm (DispatchPolicy.java:179-193): selectBaseAlgorithm
+    // (:254-279) with the HIP gate in the CUDA slot.
+    Algorithm selectForMultiply(int m, int n, int k) const { return selectBase(m, n, k, true); }
+    Algorithm selectCpuAlgorithm(int m, int n, int k) const { return selectBase(m, n, k, false); }
 
    private:
+    Algorithm selectBase(int m, int n, int k, bool allowHip) const;
     bool enable_hip_ = true;
     int64_t hip_min_dim_ = 0;
     int64_t hip_min_elements_ = 0;
     int64_t hip_min_flops_ = 200000000;
     int parallelism_ = 1;
     int mode_ = 0;
+    // Multiply-route thresholds: the reference builder's defaults (DispatchPolicy.java:424-437).
+    int naive_threshold_ = 64, blocked_threshold_ = 256, strassen_threshold_ = 1024;
+    int blas3_threshold_ = 256, simd_threshold_ = 256;
+    bool enable_blas3_ = true, enable_simd_ = true, enable_strassen_ = false, simd_available_ = true;
+    int hip_detected_ = -1;  // cudaDetector analogue: -1 asks HipSupport, 0 / 1 injected
 };
 
 class DispatchPolicy::Builder {
@@ -93,6 +111,10 @@ class DispatchPolicy::Builder {
     Builder& parallelism(int v) { p_.parallelism_ = v; return *this; }
     // Arithmetic form of the HIP kernel: JLC_MODE_FP64 / SPLIT_INT8 / SPLIT_BF16.
     Builder& mode(int v) { p_.mode_ = v; return *this; }
+    Builder& naiveThreshold(int v) { p_.naive_threshold_ = v; return *this; }
+    Builder& enableBlas3(bool v) { p_.enable_blas3_ = v; return *this; }
+    Builder& simdAvailable(bool v) { p_.simd_available_ = v; return *this; }
+    Builder& hipDetector(bool available) { p_.hip_detected_ = available ? 1 : 0; return *this; }
     DispatchPolicy build() const { return p_; }
 
    private:
@@ -125,6 +147,18 @@ class HipSupport {
 class HipGemm {
    public:
     static bool gemm(const Matrix* a, const Matrix* b, Matrix* c, double alpha, double beta, int mode = 0);
+    // CudaGemm.multiply (CudaGemm.java:32-41): a new heap Matrix A B, or null when the device call fails.
+    static std::unique_ptr<Matrix> multiply(const Matrix* a, const Matrix* b, int mode = 0);
+};
+
+// BlockedMultiply.multiply (BlockedMultiply.java:43-105), the product behind Matrix.multiply: null /
+// inner-dimension errors throw IllegalArgumentException; an empty product is a new zero matrix (an
+// OffHeapMatrix when either input is one); above the HIP gate a new Matrix from HipGemm.multiply.
+// n == 1 (multiplyVector) and the CPU algorithms throw UnsupportedKernel.
+class BlockedMultiply {
+   public:
+    static std::unique_ptr<Matrix> multiply(const Matrix* a, const Matrix* b,
+                                            const DispatchPolicy* policy = nullptr);
 };
 
 class OptimizedBLAS3 {

@@ -129,6 +129,44 @@ static void cpu_tests() {
         BLAS3Kernels::gemm(&a2, &b2, &c2, 2.0, 1.0, &HIP_POLICY);
         CHECK(BLAS3Kernels::gemm(&a, &b, &HIP_POLICY).getRowCount() == 0, "shape");
     });
+    // BlockedMultiply.multiply facade (BlockedMultiply.java:55-105) and selectBaseAlgorithm
+    // (DispatchPolicy.java:254-279), without a device.
+    run("BlockedMultiply.facade_and_selection", [] {
+        using A = DispatchPolicy::Algorithm;
+        std::string msg;
+        Matrix a(2, 3), b(4, 2);
+        CHECK(throws_iae([&] { BlockedMultiply::multiply(nullptr, &b); }, &msg), "null");
+        CHECK(throws_iae([&] { a.multiply(b); }, &msg), "inner");
+        CHECK(msg.find("3 != 4") != std::string::npos, "message names the dimensions: %s", msg.c_str());
+        const auto cpu = DispatchPolicy::builder().enableHip(false).build();
+        Matrix e1(3, 0), e2(0, 5);
+        auto z = BlockedMultiply::multiply(&e1, &e2, &cpu);
+        CHECK(z->getRowCount() == 3 && z->getColumnCount() == 5 && !dynamic_cast<OffHeapMatrix*>(z.get()),
+              "empty heap product");
+        for (double v : z->getRawData()) CHECK(v == 0.0, "zero entries");
+        OffHeapMatrix o1(0, 4);
+        Matrix m2(4, 2);
+        CHECK(dynamic_cast<OffHeapMatrix*>(BlockedMultiply::multiply(&o1, &m2, &cpu).get()) != nullptr,
+              "empty product with an off-heap input is off-heap");
+        Matrix v1(300, 300), v2(300, 1);
+        bool unsupported = false;
+        try {
+            BlockedMultiply::multiply(&v1, &v2, &cpu);
+        } catch (const UnsupportedKernel&) {
+            unsupported = true;
+        }
+        CHECK(unsupported, "n == 1 is the host's matvec");
+        const auto on = DispatchPolicy::builder().hipMinFlops(0).hipDetector(true).build();
+        const auto off = DispatchPolicy::builder().hipMinFlops(0).hipDetector(false).build();
+        CHECK(on.selectForMultiply(64, 64, 64) == A::SIMD, "naive threshold before the HIP gate");
+        CHECK(on.selectForMultiply(65, 65, 65) == A::HIP, "HIP above the naive threshold");
+        CHECK(off.selectForMultiply(65, 65, 65) == A::SIMD, "SIMD below 256");
+        CHECK(off.selectForMultiply(300, 300, 300) == A::BLAS3, "BLAS3 from 256");
+        CHECK(on.selectCpuAlgorithm(300, 300, 300) == A::BLAS3, "CPU choice ignores HIP");
+        const auto plain = DispatchPolicy::builder().hipDetector(false).simdAvailable(false).enableBlas3(false).build();
+        CHECK(plain.selectForMultiply(100, 100, 100) == A::NAIVE, "naive below 256");
+        CHECK(plain.selectForMultiply(300, 100, 100) == A::BLOCKED, "blocked from 256");
+    });
     run("OffHeapMatrix.sync_orders", [] {
         for (auto order : {OffHeapMatrix::Order::ROW_MAJOR, OffHeapMatrix::Order::COLUMN_MAJOR}) {
             OffHeapMatrix m(3, 5, order);
@@ -311,6 +349,25 @@ static void gpu_tests() {
             oracle_gemm(av.data(), bv.data(), expected.data(), m, k, n, 1.0, 0.0);
             assert_parity(expected, c.getRawData(), k, "large");
         }
+    });
+
+    // Matrix.multiply -> BlockedMultiply.multiply -> HipGemm.multiply (Matrix.java:1120-1122,
+    // BlockedMultiply.java:79-85, CudaGemm.java:32-41): a new heap Matrix, off-heap inputs read
+    // through their heap mirrors.
+    run("BlockedMultiply.multiply_through_hip", [] {
+        const int m = 300, k = 250, n = 200;
+        auto av = seeded((int64_t)m * k, 9101), bv = seeded((int64_t)k * n, 9102);
+        Matrix a(m, k, av), b(k, n, bv);
+        std::vector<double> expected((size_t)m * n, 0.0);
+        oracle_gemm(av.data(), bv.data(), expected.data(), m, k, n, 1.0, 0.0);
+        auto c = a.multiply(b);
+        CHECK(!dynamic_cast<OffHeapMatrix*>(c.get()), "heap result");
+        assert_parity(expected, c->getRawData(), k, "Matrix.multiply");
+        OffHeapMatrix oa(m, k);
+        oa.getRawData() = av;
+        auto c2 = BlockedMultiply::multiply(&oa, &b, &HIP_POLICY);
+        CHECK(c2->getRowCount() == m && c2->getColumnCount() == n, "shape");
+        assert_parity(expected, c2->getRawData(), k, "BlockedMultiply.multiply off-heap input");
     });
 
     // Default policy: the gate read from the tuning table sends 1024^3 to HIP.

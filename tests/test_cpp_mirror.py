@@ -38,11 +38,11 @@ def _run(args, timeout):
 
 def test_cpp_mirror_cpu_cases():
     out = _run(["--cpu"], 60)
-    assert "5 passed" in out
+    assert "6 passed" in out
 
 
 @pytest.mark.gpu
 def test_cpp_mirror_parity_on_gpu():
     out = _run([], 110)
     assert "[ FAIL ]" not in out
-    assert "17 passed" in out
+    assert "19 passed" in out

@@ -93,3 +93,46 @@ def test_offheap_sync_orders():
     m.getRawData()[:] = 0
     m.syncFromOffHeap()
     assert np.array_equal(m.getRawData(), np.arange(6.0))
+
+
+def test_multiply_facade_semantics_without_a_device():
+    """BlockedMultiply.multiply (BlockedMultiply.java:55-105): null and inner-dimension errors with a
+    message, empty products as a new zero matrix (off-heap when either input is), n == 1 to the Java
+    host's matvec, small sizes to its naive / SIMD loops."""
+    from jlc_amd.compute import BlockedMultiply
+    with pytest.raises(IllegalArgumentException) as ei:
+        BlockedMultiply.multiply(None, Matrix(2, 2))
+    assert str(ei.value)
+    with pytest.raises(IllegalArgumentException) as ei:
+        Matrix(2, 3).multiply(Matrix(4, 2))
+    assert "3 != 4" in str(ei.value)
+    z = BlockedMultiply.multiply(Matrix(3, 0), Matrix(0, 5), CPU)
+    assert type(z) is Matrix and (z.getRowCount(), z.getColumnCount()) == (3, 5) and not z.getRawData().any()
+    z = BlockedMultiply.multiply(OffHeapMatrix(0, 4), Matrix(4, 2), CPU)
+    assert isinstance(z, OffHeapMatrix) and (z.getRowCount(), z.getColumnCount()) == (0, 2)
+    z = BlockedMultiply.multiply(Matrix(2, 3), OffHeapMatrix(3, 0), CPU)
+    assert isinstance(z, OffHeapMatrix)
+    with pytest.raises(UnsupportedKernel):
+        BlockedMultiply.multiply(Matrix(300, 300), Matrix(300, 1), CPU)
+    with pytest.raises(UnsupportedKernel):
+        Matrix(8, 8).multiply(Matrix(8, 8))
+
+
+def test_select_for_multiply_order():
+    """selectBaseAlgorithm (DispatchPolicy.java:254-279): sizes up to naiveThreshold never offload,
+    then the HIP gate (in the CUDA slot), then STRASSEN / BLAS3 / SIMD / BLOCKED / NAIVE."""
+    from jlc_amd.compute import Algorithm
+    on = DispatchPolicy.builder().hipMinFlops(0).hipDetector(lambda: True).build()
+    off = DispatchPolicy.builder().hipMinFlops(0).hipDetector(lambda: False).build()
+    assert on.selectForMultiply(64, 64, 64) is Algorithm.SIMD      # maxDim <= 64: CPU even with HIP
+    assert on.selectForMultiply(65, 65, 65) is Algorithm.HIP
+    assert off.selectForMultiply(65, 65, 65) is Algorithm.SIMD     # 65 < simdThreshold 256
+    assert off.selectForMultiply(300, 300, 300) is Algorithm.BLAS3
+    assert on.selectCpuAlgorithm(300, 300, 300) is Algorithm.BLAS3
+    gated = DispatchPolicy.builder().hipMinFlops(10 ** 9).hipDetector(lambda: True).build()
+    assert gated.selectForMultiply(300, 300, 300) is Algorithm.BLAS3  # 5.4e7 flops < 1e9
+    assert gated.selectForMultiply(1000, 1000, 1000) is Algorithm.HIP
+    nosimd = DispatchPolicy(hip_min_flops=0, hip_detector=lambda: False, simd_available=False, enable_blas3=False)
+    assert nosimd.selectForMultiply(100, 100, 100) is Algorithm.NAIVE
+    assert nosimd.selectForMultiply(300, 100, 100) is Algorithm.BLOCKED
+    assert nosimd.selectForMultiply(10, 10, 10) is Algorithm.NAIVE

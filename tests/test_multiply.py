@@ -1,0 +1,71 @@
+"""Matrix.multiply on the GPU: Matrix.multiply (Matrix.java:1120-1122) -> BlockedMultiply.multiply
+(BlockedMultiply.java:43-105) -> the HIP member of selectForMultiply -> HipGemm.multiply (the twin of
+CudaGemm.multiply, CudaGemm.java:32-41), checked against GemmReference at the reference's CPU
+tolerance, and through the split form a policy selects bit for bit against its restatement."""
+import numpy as np
+import pytest
+
+from jlc_amd import native
+from jlc_amd.compute import BlockedMultiply, DispatchPolicy, Matrix, OffHeapMatrix
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _seeded(rows, cols, seed):
+    return Matrix.wrap(orc.seeded_array(rows * cols, seed), rows, cols)
+
+
+def test_matrix_multiply_routes_to_hip_under_the_default_policy():
+    for (m, k, n), seed in (((300, 250, 200), 9101), ((65, 65, 65), 9201), ((1000, 3, 777), 9301)):
+        a, b = _seeded(m, k, seed), _seeded(k, n, seed + 1)
+        assert DispatchPolicy.defaultPolicy().selectForMultiply(m, n, k).value == "HIP"
+        c = a.multiply(b)
+        assert type(c) is Matrix and (c.getRowCount(), c.getColumnCount()) == (m, n)
+        expected = orc.gemm(a.getRawData(), b.getRawData(), np.zeros(m * n), m, k, n, 1.0, 0.0)
+        orc.assert_parity("Matrix.multiply", m, k, n, 1.0, 0.0, seed, expected, c.getRawData(), 1e-12,
+                          orc.cpu_abs_tolerance(expected, k))
+
+
+def test_offheap_inputs_read_through_the_heap_mirror_and_give_a_heap_matrix():
+    m, k, n = 128, 200, 96
+    a = OffHeapMatrix(m, k, OffHeapMatrix.COLUMN_MAJOR)
+    a.getRawData()[:] = orc.seeded_array(m * k, 9401)
+    a.syncToOffHeap()
+    b = _seeded(k, n, 9402)
+    c = BlockedMultiply.multiply(a, b, DispatchPolicy.builder().hipMinFlops(0).build())
+    assert type(c) is Matrix
+    expected = orc.gemm(a.getRawData(), b.getRawData(), np.zeros(m * n), m, k, n, 1.0, 0.0)
+    orc.assert_parity("BlockedMultiply.multiply off-heap", m, k, n, 1.0, 0.0, 9401, expected, c.getRawData(),
+                      1e-12, orc.cpu_abs_tolerance(expected, k))
+
+
+def test_policy_mode_runs_the_split_form_bit_for_bit():
+    m, k, n = 200, 300, 150
+    a, b = _seeded(m, k, 9501), _seeded(k, n, 9502)
+    nmod = 14
+    pol = DispatchPolicy.builder().hipMinFlops(0).mode(native.mode_slices(native.MODE_SPLIT_CRT, nmod)).build()
+    c = BlockedMultiply.multiply(a, b, pol)
+    pa, pb = orc.crt_bits_for(nmod, k)
+    exp = np.zeros(m * n)
+    orc.crt_gemm(native.ROW, native.ROW, m, n, k, 1.0, a.getRawData(), k, b.getRawData(), n, 0.0, exp, n, nmod,
+                 pa, pb)
+    assert np.array_equal(c.getRawData(), exp)
+
+
+def test_q_times_r_reconstructs_a():
+    """The reference's QR acceptance reconstructs A as Q.multiply(R) (HouseholderQRTest.java:26-45)."""
+    rng = np.random.default_rng(9601)
+    m, n, nb = 512, 256, 64
+    A = rng.standard_normal((m, n))
+    da = native.DeviceArray.from_numpy(np.asfortranarray(A).reshape(-1, order="F").copy())
+    dt = native.DeviceArray(n)
+    native.dgeqrf_dev(m, n, da.ptr, m, dt.ptr, nb)
+    dq = native.DeviceArray(m * n)
+    native.dorgqr_dev(m, n, da.ptr, m, dt.ptr, nb, dq.ptr, m)
+    Qcm = dq.to_numpy().reshape(n, m).T           # column-major m x n on the device
+    R = np.triu(da.to_numpy().reshape(n, m).T[:n])
+    Q = Matrix.wrap(np.ascontiguousarray(Qcm).reshape(-1), m, n)
+    QR = Q.multiply(Matrix.wrap(R.reshape(-1).copy(), n, n))
+    recon = np.linalg.norm(QR.to_numpy() - A) / np.linalg.norm(A)
+    assert recon < 1e-12, recon
