@@ -137,6 +137,28 @@ int jlc_dgemm_dev_noalloc(int layout_a, int layout_b, int64_t m, int64_t n, int6
                           const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
                           int64_t ldc, void* stream);
 
+/* ---- compact-WY block update of a strided sub-matrix: the three-GEMM
+ * sequences of the reference's blocked reductions in one call. C is a
+ * row-major m x n block (ldc) of a larger matrix; T is kb x kb upper
+ * triangular, row-major (ldt); op(T) = T (trans_t == 0) or T^T (trans_t == 1).
+ *   side == JLC_LEFT  (V row-major m x kb, ldv):  C <- C - V op(T) (V^T C)
+ *     replaces the body of BlockedHessenbergQR.applyBlockLeft
+ *     (src/main/java/net/faulj/eigen/qr/BlockedHessenbergQR.java:277-306, trans_t = 1) and the left
+ *     block update of BlockedBidiagonalization
+ *     (src/main/java/net/faulj/decomposition/bidiagonal/BlockedBidiagonalization.java:379-401, trans_t = 0);
+ *   side == JLC_RIGHT (V row-major n x kb, ldv):  C <- C - ((C V) op(T)) V^T
+ *     replaces applyBlockRight (BlockedHessenbergQR.java:308-345, the colStart = k + 1 form its
+ *     callers use, trans_t = 0) and the right block update of BlockedBidiagonalization (:581-603).
+ * The two large products run in `mode` (as jlc_dgemm_dev), the kb x kb product in FP64; the W and
+ * W2 intermediates live in the calling thread's workspace. kb == 0 leaves C unchanged. The device
+ * form is asynchronous unless stats is non-NULL; the host form stages only V, T and C's block. */
+#define JLC_LEFT 0
+#define JLC_RIGHT 1
+int jlc_wy_apply_dev(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const double* v, int64_t ldv,
+                     const double* t, int64_t ldt, double* c, int64_t ldc, int mode, void* stream, jlc_stats* stats);
+int jlc_wy_apply(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const double* v, int64_t ldv,
+                 const double* t, int64_t ldt, double* c, int64_t ldc, int mode, jlc_stats* stats);
+
 /* ---- MATVEC: y = alpha*A*x + beta*y, A row-major m x k (lda). Replaces
  * SpecializedKernels.matvec (src/main/java/net/faulj/compute/SpecializedKernels.java:27-71),
  * the n == 1 regime of GemmDispatch.selectKernel (GemmDispatch.java:80-82).

@@ -17,6 +17,8 @@ reference's own tests:
   gemmStrided
 * BLAS3Kernels.gemm* /         BLAS3Kernels.java:35-139, :244-303
   gemmStrided*
+* BlockedHessenbergQR.         eigen/qr/BlockedHessenbergQR.java:277-345 (compact-WY updates)
+  applyBlockLeft / Right
 * Matrix.multiply /             Matrix.java:1120-1122; BlockedMultiply.java:43-105, with
   BlockedMultiply.multiply      DispatchPolicy.selectForMultiply (:179-193, :254-279) and
                                 CudaGemm.multiply (CudaGemm.java:32-41) -> HipGemm.multiply
@@ -502,6 +504,42 @@ class BlockedMultiply:
                                     f"then runs its CPU algorithm {algorithm.value}")
         raise UnsupportedKernel(f"multiply: dispatch selected the CPU algorithm {algorithm.value}; CPU "
                                 f"algorithms are the Java host's own code")
+
+
+class BlockedHessenbergQR:
+    """The compact-WY block updates of BlockedHessenbergQR (src/main/java/net/faulj/eigen/qr/
+    BlockedHessenbergQR.java:277-345), same arguments and in-place semantics, each three-GEMM sequence
+    one jlc_wy_apply call on the row-major n x n array `a` (only V, T and the updated block cross PCIe)."""
+
+    @staticmethod
+    def applyBlockLeft(a, n, k, colStart, V, T, m, panelWidth, mode: int = MODE_FP64) -> None:
+        """applyBlockLeft (:277-306): a[k+1:, colStart:] <- (I - V T^T V^T) a[k+1:, colStart:]."""
+        if colStart >= n:
+            return
+        native.wy_apply(native.LEFT, 1, m, n - colStart, panelWidth, V, panelWidth, T, panelWidth, a, n, mode,
+                        c_off=(k + 1) * n + colStart)
+
+    @staticmethod
+    def applyBlockRight(a, n, k, colStart, V, T, m, panelWidth, mode: int = MODE_FP64) -> None:
+        """applyBlockRight (:308-345): a[:, colStart:] -= (a[:, k+1:] V T) V[colStart-k-1:, :]^T."""
+        fullStart = k + 1
+        if fullStart >= n or colStart >= n:
+            return
+        nCols = n - colStart
+        colOffset = colStart - fullStart
+        if nCols <= 0 or colOffset < 0:
+            return
+        if colOffset == 0:  # the form the driver calls (:132-133): one WY update of a[:, k+1:]
+            native.wy_apply(native.RIGHT, 0, n, nCols, panelWidth, V, panelWidth, T, panelWidth, a, n, mode,
+                            c_off=colStart)
+            return
+        # W from all m trailing columns, the update on the last nCols of them
+        W = np.zeros(n * panelWidth)
+        native.dgemm(ROW, ROW, n, panelWidth, m, 1.0, a, n, V, panelWidth, 0.0, W, panelWidth, mode, a_off=fullStart)
+        W2 = np.zeros(n * panelWidth)
+        native.dgemm(ROW, ROW, n, panelWidth, panelWidth, 1.0, W, panelWidth, T, panelWidth, 0.0, W2, panelWidth)
+        native.dgemm(ROW, COL, n, nCols, panelWidth, -1.0, W2, panelWidth, V, panelWidth, 1.0, a, n, mode,
+                     b_off=colOffset * panelWidth, c_off=colStart)
 
 
 HIP_POLICY = DispatchPolicy.builder().enableHip(True).hipMinFlops(0).parallelism(1).build()

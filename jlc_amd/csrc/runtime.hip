@@ -85,14 +85,15 @@ struct ThreadCtx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // Staging: 0 = A, 1 = B, 2 = C; split workspace: 3 = exponents/spreads, 4 = A slices, 5 = B slices;
-    // 6 = split-K partial products; 7 = residue planes of C (CRT form).
-    static constexpr int NBUF = 8;
+    // 6 = split-K partial products; 7 = residue planes of C (CRT form); 8, 9 = the W and W2 products of a
+    // compact-WY update.
+    static constexpr int NBUF = 10;
     void* buf[NBUF] = {};
     size_t cap[NBUF] = {};
     // Pinned host staging for the host-pointer entry points.
     void* pin = nullptr;
     size_t pin_cap = 0;
-    // The split-form and split-K workspace (buf[3..7]) is shared by every call of this thread,
+    // The split-form, split-K and WY workspace (buf[3..9]) is shared by every call of this thread,
     // whatever stream the call runs on: ws_ev is recorded after the last launch that used it,
     // on ws_stream, and a call on another stream waits for it before touching the buffers.
     hipEvent_t ws_ev = nullptr;
@@ -618,6 +619,56 @@ int run_gemm_device(int la, int lb, int64_t m, int64_t n, int64_t k, double alph
     return run_fp64(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, info, kernel, &ws, nullptr, 0);
 }
 
+// The three products of a compact-WY update on device pointers (jlc_wy_apply_dev). W and W2 live in
+// the thread's workspace (slots 8 and 9); the large products run in `mode`, the small T product in FP64.
+int run_wy_apply(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const double* v, int64_t ldv,
+                 const double* t, int64_t ldt, double* c, int64_t ldc, int mode, hipStream_t s,
+                 jlc::TileInfo* info, int* kernel) {
+    WorkspaceUse ws(s);
+    int rc = ws.acquire();
+    if (rc != JLC_OK) return rc;
+    const size_t wbytes = (size_t)(side == JLC_LEFT ? kb * n : m * kb) * sizeof(double);
+    if ((rc = ensure_buf(8, wbytes)) != JLC_OK) return rc;
+    if ((rc = ensure_buf(9, wbytes)) != JLC_OK) return rc;
+    double* w = static_cast<double*>(t_ctx.buf[8]);
+    double* w2 = static_cast<double*>(t_ctx.buf[9]);
+    const int lt = trans_t ? JLC_COL : JLC_ROW;
+    int k2 = 0;
+    if (side == JLC_LEFT) {
+        // W = V^T C (kb x n); W2 = op(T) W; C -= V W2.
+        rc = run_gemm_device(JLC_COL, JLC_ROW, kb, n, m, 1.0, v, ldv, c, ldc, 0.0, w, n, mode, s, info, kernel);
+        if (rc == JLC_OK)
+            rc = run_gemm_device(lt, JLC_ROW, kb, n, kb, 1.0, t, ldt, w, n, 0.0, w2, n, JLC_MODE_FP64, s, info, &k2);
+        if (rc == JLC_OK)
+            rc = run_gemm_device(JLC_ROW, JLC_ROW, m, n, kb, -1.0, v, ldv, w2, n, 1.0, c, ldc, mode, s, info, &k2);
+    } else {
+        // W = C V (m x kb); W2 = W op(T); C -= W2 V^T.
+        rc = run_gemm_device(JLC_ROW, JLC_ROW, m, kb, n, 1.0, c, ldc, v, ldv, 0.0, w, kb, mode, s, info, kernel);
+        if (rc == JLC_OK)
+            rc = run_gemm_device(JLC_ROW, lt, m, kb, kb, 1.0, w, kb, t, ldt, 0.0, w2, kb, JLC_MODE_FP64, s, info, &k2);
+        if (rc == JLC_OK)
+            rc = run_gemm_device(JLC_ROW, JLC_COL, m, n, kb, -1.0, w2, kb, v, ldv, 1.0, c, ldc, mode, s, info, &k2);
+    }
+    return rc;
+}
+
+// Argument checks shared by jlc_wy_apply and jlc_wy_apply_dev.
+int validate_wy(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const double* v, int64_t ldv,
+                const double* t, int64_t ldt, const double* c, int64_t ldc, int mode) {
+    if (side != JLC_LEFT && side != JLC_RIGHT) return fail(JLC_ERR_INVALID_ARG, "side must be JLC_LEFT or JLC_RIGHT");
+    if (trans_t != 0 && trans_t != 1) return fail(JLC_ERR_INVALID_ARG, "trans_t must be 0 or 1");
+    if (m < 0 || n < 0 || kb < 0) return fail(JLC_ERR_INVALID_ARG, "negative dimension");
+    if (m > INT32_MAX || n > INT32_MAX || kb > INT32_MAX) return fail(JLC_ERR_INVALID_ARG, "dimension above 2^31-1");
+    int rc = validate_gemm(JLC_ROW, JLC_ROW, 0, 0, 0, 1.0, nullptr, 0, nullptr, 0, nullptr, 0, mode);
+    if (rc != JLC_OK) return rc;
+    if (m == 0 || n == 0 || kb == 0) return JLC_OK;
+    if (!v || !t || !c) return fail(JLC_ERR_INVALID_ARG, "V, T and C must not be null");
+    if (ldv < kb || ldt < kb || ldc < n)
+        return fail(JLC_ERR_INVALID_ARG, "need ldv >= kb, ldt >= kb, ldc >= n (ldv=%lld ldt=%lld ldc=%lld)",
+                    (long long)ldv, (long long)ldt, (long long)ldc);
+    return JLC_OK;
+}
+
 void fill_stats(jlc_stats* s, const jlc::TileInfo& info, int kernel, double kms, double tms) {
     if (!s) return;
     s->kernel_ms = kms;
@@ -873,6 +924,77 @@ int jlc_dgemm_dev(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, d
         }
     }
     fill_stats(stats, info, kernel, kms, now_ms() - t0);
+    return JLC_OK;
+}
+
+int jlc_wy_apply_dev(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const double* v, int64_t ldv,
+                     const double* t, int64_t ldt, double* c, int64_t ldc, int mode, void* stream, jlc_stats* stats) {
+    double t0 = now_ms();
+    int rc = validate_wy(side, trans_t, m, n, kb, v, ldv, t, ldt, c, ldc, mode);
+    if (rc != JLC_OK) return rc;
+    jlc::TileInfo info;
+    if (m == 0 || n == 0 || kb == 0) {
+        fill_stats(stats, info, JLC_KERNEL_NONE, 0.0, now_ms() - t0);
+        return JLC_OK;
+    }
+    if ((rc = ensure_init()) != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    if (stats) JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    int kernel = JLC_KERNEL_NONE;
+    if ((rc = run_wy_apply(side, trans_t, m, n, kb, v, ldv, t, ldt, c, ldc, mode, s, &info, &kernel)) != JLC_OK)
+        return rc;
+    double kms = 0.0;
+    if (stats) {
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev1, s));
+        JLC_HIP_CHECK(hipEventSynchronize(t_ctx.ev1));
+        float f = 0.0f;
+        JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
+        kms = f;
+        info.main_ms = 0.0;
+    }
+    fill_stats(stats, info, kernel, kms, now_ms() - t0);
+    return JLC_OK;
+}
+
+int jlc_wy_apply(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const double* v, int64_t ldv,
+                 const double* t, int64_t ldt, double* c, int64_t ldc, int mode, jlc_stats* stats) {
+    double t0 = now_ms();
+    int rc = validate_wy(side, trans_t, m, n, kb, v, ldv, t, ldt, c, ldc, mode);
+    if (rc != JLC_OK) return rc;
+    jlc::TileInfo info;
+    if (m == 0 || n == 0 || kb == 0) {
+        fill_stats(stats, info, JLC_KERNEL_NONE, 0.0, now_ms() - t0);
+        return JLC_OK;
+    }
+    if ((rc = ensure_init()) != JLC_OK) return rc;
+    ThreadCtx& ctx = t_ctx;
+    hipStream_t s = ctx.stream;
+    // Only the blocks the update touches cross PCIe, each packed with an even pitch: V (rows x kb),
+    // T (kb x kb) and the m x n block of C, which comes back the same way.
+    const int64_t vrows = side == JLC_LEFT ? m : n;
+    const int64_t dk = (kb + 1) & ~int64_t(1), dn = n;
+    if ((rc = ensure_buf(0, (size_t)(vrows * dk) * sizeof(double))) != JLC_OK) return rc;
+    if ((rc = ensure_buf(1, (size_t)(kb * dk) * sizeof(double))) != JLC_OK) return rc;
+    if ((rc = ensure_buf(2, (size_t)(m * dn) * sizeof(double))) != JLC_OK) return rc;
+    double* dv = static_cast<double*>(ctx.buf[0]);
+    double* dt = static_cast<double*>(ctx.buf[1]);
+    double* dc = static_cast<double*>(ctx.buf[2]);
+    const int64_t v_el = vrows * kb, t_el = kb * kb, c_el = m * n;
+    if ((rc = ensure_pin((size_t)(v_el + t_el + c_el) * sizeof(double))) != JLC_OK) return rc;
+    double* pin = static_cast<double*>(ctx.pin);
+    if ((rc = stage_to_device(dv, dk, v, ldv, vrows, kb, pin, s)) != JLC_OK) return rc;
+    if ((rc = stage_to_device(dt, dk, t, ldt, kb, kb, pin + v_el, s)) != JLC_OK) return rc;
+    if ((rc = stage_to_device(dc, dn, c, ldc, m, n, pin + v_el + t_el, s)) != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipEventRecord(ctx.ev0, s));
+    int kernel = JLC_KERNEL_NONE;
+    if ((rc = run_wy_apply(side, trans_t, m, n, kb, dv, dk, dt, dk, dc, dn, mode, s, &info, &kernel)) != JLC_OK)
+        return rc;
+    JLC_HIP_CHECK(hipEventRecord(ctx.ev1, s));
+    if ((rc = stage_from_device(c, ldc, dc, m, n, pin + v_el + t_el, s)) != JLC_OK) return rc;
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    float f = 0.0f;
+    JLC_HIP_CHECK(hipEventElapsedTime(&f, ctx.ev0, ctx.ev1));
+    fill_stats(stats, info, kernel, f, now_ms() - t0);
     return JLC_OK;
 }
 

@@ -106,6 +106,8 @@ _SIGNATURES = {
     "jlc_dgemm_dev": (_i, [_i, _i, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i, _p,
                            C.POINTER(Stats)]),
     "jlc_dgemm_dev_noalloc": (_i, [_i, _i, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p]),
+    "jlc_wy_apply_dev": (_i, [_i, _i, _i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64, _i, _p, C.POINTER(Stats)]),
+    "jlc_wy_apply": (_i, [_i, _i, _i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64, _i, C.POINTER(Stats)]),
     "jlc_dgemv": (_i, [_i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, C.POINTER(Stats)]),
     "jlc_dgemv_dev": (_i, [_i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p, C.POINTER(Stats)]),
     "jlc_dev_alloc": (_i, [C.POINTER(_p), C.c_size_t]),
@@ -206,6 +208,30 @@ def dgemm(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, a:
     rc = load().jlc_dgemm(layout_a, layout_b, m, n, k, alpha, _ptr(a, a_off), lda, _ptr(b, b_off), ldb, beta,
                           _ptr(c, c_off), ldc, mode, C.byref(st))
     check(rc)
+    return st
+
+
+LEFT = 0
+RIGHT = 1
+
+
+def wy_apply(side: int, trans_t: int, m: int, n: int, kb: int, v: np.ndarray | None, ldv: int,
+             t: np.ndarray | None, ldt: int, c: np.ndarray, ldc: int, mode: int = MODE_FP64, v_off: int = 0,
+             t_off: int = 0, c_off: int = 0) -> Stats:
+    """Host-memory compact-WY update through jlc_wy_apply (the m x n block of C at c_off updated)."""
+    st = Stats()
+    check(load().jlc_wy_apply(side, trans_t, m, n, kb, _ptr(v, v_off), ldv, _ptr(t, t_off), ldt, _ptr(c, c_off),
+                              ldc, mode, C.byref(st)))
+    return st
+
+
+def wy_apply_dev(side: int, trans_t: int, m: int, n: int, kb: int, v: int | None, ldv: int, t: int | None,
+                 ldt: int, c: int, ldc: int, mode: int = MODE_FP64, stream: int | None = None,
+                 stats: bool = True) -> Stats | None:
+    """Device-pointer compact-WY update through jlc_wy_apply_dev."""
+    st = Stats() if stats else None
+    check(load().jlc_wy_apply_dev(side, trans_t, m, n, kb, v, ldv, t, ldt, c, ldc, mode, stream,
+                                  C.byref(st) if st is not None else None))
     return st
 
 

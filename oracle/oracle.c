@@ -76,6 +76,38 @@ void oracle_gemm_general(int a_layout, int b_layout, int64_t m, int64_t n, int64
     }
 }
 
+/* The compact-WY updates of the reference's blocked reductions: the three strided GEMMs of
+ * BlockedHessenbergQR.applyBlockLeft (src/main/java/net/faulj/eigen/qr/BlockedHessenbergQR.java:277-306:
+ * W = V^T C by gemmStridedTransA, W2 = T^T W by gemmStridedTransA, C = C - V W2 by gemmStrided) and of
+ * applyBlockRight (:308-345, called with colStart = k + 1: W = C V, W2 = W T by gemmStrided, C = C - W2 V^T by
+ * gemmStridedColMajorB), and the left / right block updates of BlockedBidiagonalization
+ * (src/main/java/net/faulj/decomposition/bidiagonal/BlockedBidiagonalization.java:379-401 with W2 = T W,
+ * :581-603 with W2 = W T). W and W2 start as zero arrays, as the reference allocates them. */
+void oracle_wy_apply(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const double* v, int64_t ldv,
+                     const double* t, int64_t ldt, double* c, int64_t ldc, int threads) {
+    if (m <= 0 || n <= 0 || kb <= 0) return;
+    const int64_t wn = side == 0 ? kb * n : m * kb;
+    double* w = (double*)calloc((size_t)wn, sizeof(double));
+    double* w2 = (double*)calloc((size_t)wn, sizeof(double));
+    if (!w || !w2) {
+        free(w);
+        free(w2);
+        return;
+    }
+    const int lt = trans_t ? ORACLE_COL : ORACLE_ROW;
+    if (side == 0) {
+        oracle_gemm_general(ORACLE_COL, ORACLE_ROW, kb, n, m, 1.0, v, ldv, c, ldc, 0.0, w, n, threads);
+        oracle_gemm_general(lt, ORACLE_ROW, kb, n, kb, 1.0, t, ldt, w, n, 0.0, w2, n, threads);
+        oracle_gemm_general(ORACLE_ROW, ORACLE_ROW, m, n, kb, -1.0, v, ldv, w2, n, 1.0, c, ldc, threads);
+    } else {
+        oracle_gemm_general(ORACLE_ROW, ORACLE_ROW, m, kb, n, 1.0, c, ldc, v, ldv, 0.0, w, kb, threads);
+        oracle_gemm_general(ORACLE_ROW, lt, m, kb, kb, 1.0, w, kb, t, ldt, 0.0, w2, kb, threads);
+        oracle_gemm_general(ORACLE_ROW, ORACLE_COL, m, n, kb, -1.0, w2, kb, v, ldv, 1.0, c, ldc, threads);
+    }
+    free(w);
+    free(w2);
+}
+
 /* GemmReference.gemm (GemmReference.java:31-44): contiguous row-major, C updated in place here. */
 void oracle_gemm(const double* a, const double* b, double* c, int m, int k, int n, double alpha, double beta) {
     oracle_gemm_general(ORACLE_ROW, ORACLE_ROW, m, n, k, alpha, a, k, b, n, beta, c, n, 1);

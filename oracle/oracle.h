@@ -121,6 +121,14 @@ void oracle_crt_gemm(int a_layout, int b_layout, int64_t m, int64_t n, int64_t k
                      int pb, int threads);
 
 /* Blocked Householder QR restatement (column-major in/out). */
+/* Compact-WY block update of a row-major m x n block C (ldc) by V (row-major, ldv; m x kb for side 0,
+ * n x kb for side 1) and the kb x kb upper-triangular T (row-major, ldt; trans_t selects T^T):
+ *   side 0 (left):  W = V^T C, W2 = op(T) W, C = C - V W2
+ *   side 1 (right): W = C V,   W2 = W op(T), C = C - W2 V^T
+ * each product a GemmReference triple loop; see oracle.c for the reference routines restated. */
+void oracle_wy_apply(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const double* v, int64_t ldv,
+                     const double* t, int64_t ldt, double* c, int64_t ldc, int threads);
+
 int oracle_qr_factor(int64_t m, int64_t n, double* a_colmajor, int64_t lda, double* tau, int64_t nb);
 void oracle_qr_form_q(int64_t m, int64_t n, const double* qr_colmajor, int64_t lda, const double* tau,
                       double* q_colmajor, int64_t ldq);

@@ -37,6 +37,7 @@ def _load():
     lib.oracle_uniform_array.argtypes = [_p, _i64, _i64, _d, _d]
     lib.oracle_splitmix_next.argtypes = [C.POINTER(C.c_uint64)]
     lib.oracle_splitmix_next.restype = C.c_uint64
+    lib.oracle_wy_apply.argtypes = [C.c_int, C.c_int, _i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64, C.c_int]
     lib.oracle_gemm_general.argtypes = [C.c_int, C.c_int, _i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64,
                                         C.c_int]
     for name in ("oracle_gemm_strided", "oracle_gemm_strided_trans_a", "oracle_gemm_strided_col_major_a",
@@ -250,6 +251,13 @@ def dd_gemm(layout_a: int, layout_b: int, m: int, n: int, k: int, alpha: float, 
     _load().oracle_dd_gemm(layout_a, layout_b, m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc,
                            _ptr(hi), _ptr(lo), nthreads or threads())
     return hi, lo
+
+
+def wy_apply(side: int, trans_t: int, m: int, n: int, kb: int, v, ldv: int, t, ldt: int, c: np.ndarray, ldc: int,
+             c_off: int = 0, nthreads: int | None = None) -> None:
+    """Compact-WY block update of C in place (applyBlockLeft / applyBlockRight restated; oracle.c)."""
+    _load().oracle_wy_apply(side, trans_t, m, n, kb, _ptr(v), ldv, _ptr(t), ldt, _ptr(c, c_off), ldc,
+                            nthreads or threads())
 
 
 def qr_factor(a_colmajor: np.ndarray, m: int, n: int, lda: int | None = None, nb: int = 64):

@@ -17,9 +17,12 @@ def _seeded(rows, cols, seed):
 
 
 def test_matrix_multiply_routes_to_hip_under_the_default_policy():
-    for (m, k, n), seed in (((300, 250, 200), 9101), ((65, 65, 65), 9201), ((1000, 3, 777), 9301)):
+    pol = DispatchPolicy.defaultPolicy()
+    # below the tuned gate (2 m n k under hip_min_flops) the product stays on the host's CPU loops
+    assert pol.selectForMultiply(65, 65, 65).value == "SIMD"
+    for (m, k, n), seed in (((300, 250, 200), 9101), ((1030, 90, 300), 9201), ((129, 4096, 65), 9301)):
         a, b = _seeded(m, k, seed), _seeded(k, n, seed + 1)
-        assert DispatchPolicy.defaultPolicy().selectForMultiply(m, n, k).value == "HIP"
+        assert pol.selectForMultiply(m, n, k).value == "HIP"
         c = a.multiply(b)
         assert type(c) is Matrix and (c.getRowCount(), c.getColumnCount()) == (m, n)
         expected = orc.gemm(a.getRawData(), b.getRawData(), np.zeros(m * n), m, k, n, 1.0, 0.0)

@@ -1,0 +1,159 @@
+"""Compact-WY block updates: the three-GEMM sequences of BlockedHessenbergQR.applyBlockLeft /
+applyBlockRight (src/main/java/net/faulj/eigen/qr/BlockedHessenbergQR.java:277-345) and of the left / right
+block updates of BlockedBidiagonalization (src/main/java/net/faulj/decomposition/bidiagonal/
+BlockedBidiagonalization.java:379-401, :581-603), one jlc_wy_apply[_dev] call each.
+
+The oracle's restatement (oracle_wy_apply) is pinned on the CPU by the identity the update exists for:
+with T built by the reference's recurrence (buildT, BlockedHessenbergQR.java:239-274),
+I - V T V^T = H_0 H_1 ... H_{kb-1} for the reflectors H_j = I - tau_j v_j v_j^T, so each side and
+transpose choice equals a product of single reflectors applied one at a time (a transposed T or a swapped
+side fails it), and the update preserves Frobenius norms. The GPU tests compare the library with the
+restatement at n = 4096 with the reference's panel widths (32 for Hessenberg, 64 for bidiagonalization),
+at the reference's CPU tolerance (relTol 1e-12, GemmReference.java:175-177).
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from jlc_amd import native
+from oracle import oracle as orc
+
+
+def reflectors(rng, rows: int, kb: int):
+    """Unit lower-trapezoidal V (rows x kb), tau_j = 2 / v_j^T v_j, and T by buildT's recurrence:
+    T[i,i] = tau_i, T[:i, i] = T[:i, :i] w with w_l = -tau_i v_l^T v_i (BlockedHessenbergQR.java:239-274)."""
+    V = np.tril(0.3 * rng.standard_normal((rows, kb)), -1)
+    V[np.arange(kb), np.arange(kb)] = 1.0
+    tau = 2.0 / np.sum(V * V, axis=0)
+    T = np.zeros((kb, kb))
+    for i in range(kb):
+        T[i, i] = tau[i]
+        if i:
+            T[:i, i] = T[:i, :i] @ (-tau[i] * (V[:, :i].T @ V[:, i]))
+    return V, T, tau
+
+
+def product_of_reflectors(V, tau, reverse: bool) -> np.ndarray:
+    rows, kb = V.shape
+    H = np.eye(rows)
+    for j in (reversed(range(kb)) if reverse else range(kb)):
+        v = V[:, j]
+        H = H @ (np.eye(rows) - tau[j] * np.outer(v, v))
+    return H
+
+
+@pytest.mark.parametrize("side,trans_t", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_oracle_wy_apply_equals_product_of_single_reflectors(side, trans_t):
+    rng = np.random.default_rng(31 + 2 * side + trans_t)
+    m, n, kb = 40, 33, 7
+    rows = m if side == 0 else n
+    V, T, tau = reflectors(rng, rows, kb)
+    C = rng.uniform(-1, 1, (m, n))
+    got = C.reshape(-1).copy()
+    orc.wy_apply(side, trans_t, m, n, kb, np.ascontiguousarray(V).reshape(-1), kb, T.reshape(-1).copy(), kb, got, n)
+    got = got.reshape(m, n)
+    # I - V T V^T = H_0 ... H_{kb-1}; transposing T reverses the order (each H_j is symmetric)
+    H = product_of_reflectors(V, tau, reverse=bool(trans_t))
+    want = H @ C if side == 0 else C @ H
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(C))
+    assert abs(np.linalg.norm(got) - np.linalg.norm(C)) <= 1e-13 * np.linalg.norm(C)
+
+
+def test_oracle_wy_apply_zero_panel_is_identity():
+    C = np.arange(12.0)
+    got = C.copy()
+    orc.wy_apply(0, 0, 3, 4, 0, np.zeros(1), 1, np.zeros(1), 1, got, 4)
+    assert np.array_equal(got, C)
+
+
+# ------------------------------------------------------------------------------------------ GPU
+
+def _parity(name, expected, actual, k):
+    orc.assert_parity(name, expected.size, k, 1, 1.0, 1.0, 0, expected, actual, 1e-12,
+                      orc.cpu_abs_tolerance(expected, k))
+
+
+def _device_case(side, trans_t, n_full, r0, c0, kb, seed, mode=native.MODE_FP64):
+    """The block a[r0:, c0:] of a row-major n_full x n_full matrix updated by jlc_wy_apply_dev."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-1.0, 1.0, n_full * n_full)
+    m, n = n_full - r0, n_full - c0
+    V, T, _ = reflectors(rng, m if side == native.LEFT else n, kb)
+    Vf, Tf = np.ascontiguousarray(V).reshape(-1), T.reshape(-1).copy()
+    dA, dV, dT = (native.DeviceArray.from_numpy(x) for x in (A, Vf, Tf))
+    st = native.wy_apply_dev(side, trans_t, m, n, kb, dV.ptr, kb, dT.ptr, kb, dA.at(r0 * n_full + c0), n_full, mode)
+    expected = A.copy()
+    orc.wy_apply(side, trans_t, m, n, kb, Vf, kb, Tf, kb, expected, n_full, c_off=r0 * n_full + c0)
+    return expected, dA.to_numpy(), max(m, n), st
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("side,trans_t,r0,c0,kb,what", [
+    (native.LEFT, 1, 1, 0, 32, "Hessenberg applyBlockLeft, k = 0"),
+    (native.RIGHT, 0, 0, 1, 32, "Hessenberg applyBlockRight, k = 0"),
+    (native.LEFT, 1, 2017, 2016, 32, "Hessenberg applyBlockLeft, k = 2016"),
+    (native.RIGHT, 0, 0, 2017, 32, "Hessenberg applyBlockRight, k = 2016"),
+    (native.LEFT, 0, 64, 64, 64, "bidiagonal left update, panel 64"),
+    (native.RIGHT, 0, 64, 128, 64, "bidiagonal right update, panel 64"),
+])
+def test_device_wy_apply_n4096_against_restatement(side, trans_t, r0, c0, kb, what):
+    expected, got, k, st = _device_case(side, trans_t, 4096, r0, c0, kb, seed=zlib.crc32(what.encode()))
+    assert st.kernel_name == "mfma_f64"
+    _parity(what, expected, got, k)
+
+
+@pytest.mark.gpu
+def test_device_wy_apply_split_crt_mode():
+    expected, got, k, st = _device_case(native.LEFT, 0, 2048, 64, 64, 64, seed=77, mode=native.MODE_SPLIT_CRT)
+    assert st.kernel_name == "split_crt"
+    orc.assert_parity("WY left, CRT products", expected.size, k, 1, 1.0, 1.0, 0, expected, got, 1e-11,
+                      orc.cpu_abs_tolerance(expected, k))
+
+
+@pytest.mark.gpu
+def test_hessenberg_mirror_host_path_against_restatement():
+    """BlockedHessenbergQR.applyBlockLeft / applyBlockRight through the host mirror (jlc_wy_apply on heap
+    arrays), the driver's call pattern for one panel (BlockedHessenbergQR.java:121-133), and the general
+    applyBlockRight form with colStart > k + 1."""
+    from jlc_amd.compute import BlockedHessenbergQR
+    rng = np.random.default_rng(512)
+    n, k, pw = 512, 64, 32
+    m = n - (k + 1)
+    V, T, _ = reflectors(rng, m, pw)
+    Vf, Tf = np.ascontiguousarray(V).reshape(-1), T.reshape(-1).copy()
+    h = rng.uniform(-1, 1, n * n)
+    q = np.eye(n).reshape(-1).copy()
+    want_h, want_q = h.copy(), q.copy()
+    BlockedHessenbergQR.applyBlockLeft(h, n, k, k, Vf, Tf, m, pw)
+    BlockedHessenbergQR.applyBlockRight(h, n, k, k + 1, Vf, Tf, m, pw)
+    BlockedHessenbergQR.applyBlockRight(q, n, k, k + 1, Vf, Tf, m, pw)
+    orc.wy_apply(0, 1, m, n - k, pw, Vf, pw, Tf, pw, want_h, n, c_off=(k + 1) * n + k)
+    orc.wy_apply(1, 0, n, m, pw, Vf, pw, Tf, pw, want_h, n, c_off=k + 1)
+    orc.wy_apply(1, 0, n, m, pw, Vf, pw, Tf, pw, want_q, n, c_off=k + 1)
+    _parity("applyBlockLeft+Right on H", want_h, h, n)
+    _parity("applyBlockRight on Q", want_q, q, n)
+    # colStart = k + 4: W from all trailing columns, the update on the last n - colStart of them
+    a = rng.uniform(-1, 1, n * n)
+    want = a.reshape(n, n).copy()
+    BlockedHessenbergQR.applyBlockRight(a, n, k, k + 4, Vf, Tf, m, pw)
+    W2 = (want[:, k + 1:] @ V) @ T
+    want[:, k + 4:] -= W2 @ V[3:].T
+    _parity("applyBlockRight colStart = k + 4", want.reshape(-1), a, n)
+    # the reference's early returns leave the array untouched
+    before = a.copy()
+    BlockedHessenbergQR.applyBlockLeft(a, n, k, n, Vf, Tf, m, pw)
+    BlockedHessenbergQR.applyBlockRight(a, n, n - 1, n, Vf, Tf, m, pw)
+    assert np.array_equal(a, before)
+
+
+@pytest.mark.gpu
+def test_wy_apply_arguments():
+    d = native.DeviceArray(64)
+    with pytest.raises(native.JlcError):
+        native.wy_apply_dev(2, 0, 4, 4, 2, d.ptr, 2, d.ptr, 2, d.ptr, 4)
+    with pytest.raises(native.JlcError):
+        native.wy_apply_dev(native.LEFT, 0, 4, 4, 2, d.ptr, 1, d.ptr, 2, d.ptr, 4)  # ldv < kb
+    c = np.arange(16.0)
+    native.wy_apply(native.LEFT, 0, 4, 4, 0, None, 1, None, 1, c, 4)  # kb == 0: unchanged
+    assert np.array_equal(c, np.arange(16.0))
