@@ -55,15 +55,29 @@ def make_operand(rng: np.random.Generator, rows: int, cols: int, phi: float) -> 
     return out
 
 
-def sampled_error(a: np.ndarray, b: np.ndarray, c: np.ndarray, m: int, n: int, k: int, rows, cols) -> dict:
-    """Max relative and componentwise error of C on a rows x cols grid against dd sums."""
+def dd_sample(a: np.ndarray, b: np.ndarray, m: int, n: int, k: int, rows, cols) -> tuple:
+    """Double-double dot products of C = A B on the rows x cols grid: (hi, lo, sum |a||b|)."""
     from oracle import oracle as orc
     A = a.reshape(m, k)[rows, :].copy().reshape(-1)
     B = np.ascontiguousarray(b.reshape(k, n)[:, cols]).reshape(-1)
     ii, jj = np.meshgrid(np.arange(len(rows)), np.arange(len(cols)), indexing="ij")
-    hi, lo, absum = orc.dd_entries(0, 0, k, A, k, B, len(cols), ii.ravel(), jj.ravel())
+    return orc.dd_entries(0, 0, k, A, k, B, len(cols), ii.ravel(), jj.ravel())
+
+
+def fetch_rows(dc, n: int, rows) -> np.ndarray:
+    """The given rows of a row-major device array with n columns."""
+    out = np.empty((len(rows), n))
+    lib = native.load()
+    for t, r in enumerate(rows):
+        native.check(lib.jlc_memcpy_d2h(out[t].ctypes.data, dc.ptr + 8 * int(r) * n, 8 * n))
+    return out
+
+
+def sample_error(sample: tuple, got: np.ndarray) -> dict:
+    """Max relative and componentwise error of the sampled C entries against their dd sums."""
+    hi, lo, absum = sample
     exact = hi + lo
-    got = c.reshape(m, n)[np.ix_(rows, cols)].reshape(-1)
+    got = got.reshape(-1)
     err = np.abs(got - hi - lo)
     nz = exact != 0.0
     rel = float(np.max(err[nz] / np.abs(exact[nz]))) if nz.any() else 0.0
@@ -561,6 +575,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--phi", type=float, default=0.5)
+    ap.add_argument("--phi-sweep", default="0.5,2,8",
+                    help="exponent spreads reported under phi_sweep (every form, 3 timed steps each); '' for none")
     ap.add_argument("--size", type=int, default=16384)
     ap.add_argument("--seed", type=int, default=20260603)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -608,65 +624,73 @@ def main() -> int:
         return bench_qr_batched(args)
 
     m = n = k = args.size
-    rng = np.random.default_rng(args.seed)
-    t0 = time.perf_counter()
-    a = make_operand(rng, m, k, args.phi)
-    b = make_operand(rng, k, n, args.phi)
-    log(f"generated inputs in {time.perf_counter() - t0:.1f}s")
-    da, db = native.DeviceArray.from_numpy(a), native.DeviceArray.from_numpy(b)
-    dc = native.DeviceArray(m * n)
+    da, db, dc = native.DeviceArray(m * k), native.DeviceArray(k * n), native.DeviceArray(m * n)
     stream = native.stream()
-
     flops = 2.0 * m * n * k
-    srng = np.random.default_rng(args.seed + 1)
-    rows = np.sort(srng.choice(m, 64, replace=False))
-    cols = np.sort(srng.choice(n, 64, replace=False))
-
-    def run_form(name: str, mode: int) -> dict:
-        def step():
-            return native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, dc.ptr, n, mode,
-                                    stream, stats=True)
-        st = step()
-        main = []
-        wall_ms, dev_ms = time_form(step, args.warmup, args.steps, stream, main)
-        form = {"form": name, "kernel": st.kernel_name, "slices": st.slices, "ms_per_step": round(wall_ms, 3),
-                "device_ms": round(dev_ms, 3), "main_kernel_ms": round(float(np.mean(main)), 3),
-                "tflops": round(flops / (wall_ms * 1e-3) / 1e12, 3)}
-        if st.slices:
-            crt = name == "split_crt"
-            pairs = st.slices if crt else st.slices * (st.slices + 1) // 2
-            ops = pairs * flops
-            peak = BF16_MFMA_PEAK_TFLOPS if name == "split_bf16" else INT8_MFMA_PEAK_TOPS
-            unit = "TFLOP/s" if name == "split_bf16" else "TOP/s"
-            what = (f"{pairs} modular INT8 products of 2MNK ops (bits {st.bits_a}/{st.bits_b}); time includes "
-                    f"residues and CRT reconstruction") if crt else \
-                f"{pairs} slice products of 2MNK ops; time includes slicing"
-            kms = form["main_kernel_ms"]
-            form["roofline"] = {"bound": "mfma", "achieved": round(ops / (kms * 1e-3) / 1e12, 1),
-                                "peak": peak, "unit": unit, "frac": round(ops / (kms * 1e-3) / 1e12 / peak, 4),
-                                "kernel_ms": kms, "note": what + "; achieved over the product kernel alone"}
-        log(f"{name}: {wall_ms:.2f} ms/step wall, {dev_ms:.2f} ms device, {form['tflops']:.2f} TFLOP/s"
-            f"{' slices=%d' % st.slices if st.slices else ''}")
-        if not args.no_error:
-            form["error"] = sampled_error(a, b, dc.to_numpy(), m, n, k, rows, cols)
-            log(f"{name} error vs dd oracle: {form['error']}")
-        return form
-
-    fp64 = run_form("fp64_mfma", native.MODE_FP64)
-    splits = []
     wanted = set(args.forms.split(",")) if args.forms else {"crt", "int8", "bf16"}
-    if not args.no_split:
-        if "crt" in wanted:
-            splits.append(run_form("split_crt", native.mode_slices(native.MODE_SPLIT_CRT, args.moduli)))
-        if "int8" in wanted:
-            splits.append(run_form("split_int8", native.mode_slices(native.MODE_SPLIT_INT8, args.slices)))
-        if "bf16" in wanted:
-            splits.append(run_form("split_bf16", native.mode_slices(native.MODE_SPLIT_BF16, 0)))
+    split_modes = [] if args.no_split else [
+        (name, mode) for key, name, mode in (
+            ("crt", "split_crt", native.mode_slices(native.MODE_SPLIT_CRT, args.moduli)),
+            ("int8", "split_int8", native.mode_slices(native.MODE_SPLIT_INT8, args.slices)),
+            ("bf16", "split_bf16", native.mode_slices(native.MODE_SPLIT_BF16, 0))) if key in wanted]
+
+    def measure_phi(phi: float, steps: int, warmup: int) -> tuple[dict, list]:
+        """Every form on the workload with spread phi: (FP64 form, split forms), each with its dd error."""
+        rng = np.random.default_rng(args.seed)
+        t0 = time.perf_counter()
+        a = make_operand(rng, m, k, phi)
+        b = make_operand(rng, k, n, phi)
+        da.copy_from(a)
+        db.copy_from(b)
+        srng = np.random.default_rng(args.seed + 1)
+        rows = np.sort(srng.choice(m, 64, replace=False))
+        cols = np.sort(srng.choice(n, 64, replace=False))
+        sample = None if args.no_error else dd_sample(a, b, m, n, k, rows, cols)
+        del a, b
+        log(f"phi={phi}: inputs generated and resident in {time.perf_counter() - t0:.1f}s")
+
+        def run_form(name: str, mode: int) -> dict:
+            def step():
+                return native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, dc.ptr, n,
+                                        mode, stream, stats=True)
+            st = step()
+            main = []
+            wall_ms, dev_ms = time_form(step, warmup, steps, stream, main)
+            form = {"form": name, "kernel": st.kernel_name, "slices": st.slices, "ms_per_step": round(wall_ms, 3),
+                    "device_ms": round(dev_ms, 3), "main_kernel_ms": round(float(np.mean(main)), 3),
+                    "tflops": round(flops / (wall_ms * 1e-3) / 1e12, 3)}
+            if st.kernel_name == "split_crt":
+                form["bits"] = [st.bits_a, st.bits_b]
+            if name != "fp64_mfma" and st.kernel_name != "mfma_f64":
+                crt = name == "split_crt"
+                pairs = st.slices if crt else st.slices * (st.slices + 1) // 2
+                ops = pairs * flops
+                peak = BF16_MFMA_PEAK_TFLOPS if name == "split_bf16" else INT8_MFMA_PEAK_TOPS
+                unit = "TFLOP/s" if name == "split_bf16" else "TOP/s"
+                what = (f"{pairs} modular INT8 products of 2MNK ops (bits {st.bits_a}/{st.bits_b}); time includes "
+                        f"residues and CRT reconstruction") if crt else \
+                    f"{pairs} slice products of 2MNK ops; time includes slicing"
+                kms = form["main_kernel_ms"]
+                form["roofline"] = {"bound": "mfma", "achieved": round(ops / (kms * 1e-3) / 1e12, 1),
+                                    "peak": peak, "unit": unit, "frac": round(ops / (kms * 1e-3) / 1e12 / peak, 4),
+                                    "kernel_ms": kms, "note": what + "; achieved over the product kernel alone"}
+            log(f"phi={phi} {name}: {wall_ms:.2f} ms/step wall, {dev_ms:.2f} ms device, {form['tflops']:.2f} "
+                f"TFLOP/s, kernel {st.kernel_name}{' slices=%d' % st.slices if st.slices else ''}")
+            if sample is not None:
+                form["error"] = sample_error(sample, fetch_rows(dc, n, rows)[:, cols])
+                log(f"phi={phi} {name} error vs dd oracle: {form['error']}")
+            return form
+
+        fp64 = run_form("fp64_mfma", native.MODE_FP64)
+        splits = [run_form(name, mode) for name, mode in split_modes]
+        for sf in splits:
+            if not split_ran(sf):
+                sf["fallback"] = "fp64"
+        return fp64, splits
+
+    fp64, splits = measure_phi(args.phi, args.steps, args.warmup)
     # value: best throughput among split forms that ran their own kernels and whose max relative error is
     # at most the FP64 MFMA form's; forms that fell back to the FP64 kernel are labelled as such.
-    for sf in splits:
-        if not split_ran(sf):
-            sf["fallback"] = "fp64"
     best = choose_value_form(fp64, splits, check_error=not args.no_error)
     # roofline of the value form's dominant kernel, timed alone with HIP events in the timed loop
     if best is fp64:
@@ -702,6 +726,21 @@ def main() -> int:
         "roofline": roofline,
         "speedup_vs_fp64": {f["form"]: round(f["tflops"] / fp64["tflops"], 3) for f in splits if split_ran(f)},
     }
+    sweep = [float(x) for x in args.phi_sweep.split(",") if x.strip()] if args.phi_sweep else []
+    if sweep:
+        out["phi_sweep"] = {}
+        for phi in sweep:
+            if phi == args.phi:
+                f64, spl = fp64, splits
+            else:
+                f64, spl = measure_phi(phi, min(args.steps, 3), 1)
+            entry = {"fp64_mfma": f64, "value_form": choose_value_form(f64, spl, not args.no_error)["form"]}
+            for sf in spl:
+                sf = dict(sf)
+                if split_ran(sf):
+                    sf["speedup_vs_fp64"] = round(sf["tflops"] / f64["tflops"], 3)
+                entry[sf["form"]] = sf
+            out["phi_sweep"][str(phi)] = entry
     if args.host_path:
         mode = {"fp64_mfma": native.MODE_FP64, "split_crt": native.MODE_SPLIT_CRT,
                 "split_int8": native.MODE_SPLIT_INT8, "split_bf16": native.MODE_SPLIT_BF16}[best["form"]]

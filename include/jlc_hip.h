@@ -57,7 +57,7 @@ extern "C" {
  * columns of op(B) scaled by powers of two at their 2-norms, modular INT8
  * products recombined by the Chinese remainder theorem (2..20 moduli, k below
  * 2^22). In automatic mode a spread that needs more bits than 20 moduli hold
- * (CRT), or more slices than the form has (9 INT8, 7 BF16), runs the FP64 MFMA
+ * (CRT), or more slices than the form has (9 INT8, 8 BF16), runs the FP64 MFMA
  * kernel instead (jlc_stats.kernel says which ran). */
 #define JLC_MODE_FP64 0
 #define JLC_MODE_SPLIT_INT8 1

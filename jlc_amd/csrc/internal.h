@@ -149,7 +149,7 @@ hipError_t launch_sumsq_partial(const double* x, int64_t ld, int rows, int cols,
                                 hipStream_t stream);
 hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_mn, hipStream_t stream);
 
-// BF16-slice variant (8-bit digits, slices 3..7).
+// BF16-slice variant (8-bit digits, slices 3..8).
 // CRT split form: N pairwise coprime moduli <= 256, residues in int8.
 constexpr int CRT_MAX_MODULI = 20;
 constexpr int CRT_MIN_MODULI = 2;
@@ -197,7 +197,7 @@ hipError_t launch_crt(int layout_a, int layout_b, const double* a, int64_t lda, 
                       const CrtPlan& p, double alpha, double beta, double* c, int64_t ldc, hipStream_t stream,
                       TileInfo* info);
 
-constexpr int SPLIT_BF16_MAX_SLICES = 7;
+constexpr int SPLIT_BF16_MAX_SLICES = 8;
 void split_bf16_plan_sizes(int64_t m, int64_t n, int64_t k, int slices, SplitPlan* p);
 int64_t split_bf16_chunk_k(int slices, int64_t k);
 hipError_t launch_split_bf16(int layout_a, int layout_b, const double* a, int64_t lda, const double* b, int64_t ldb,

@@ -257,6 +257,7 @@ hipError_t launch_split_bf16(int layout_a, int layout_b, const double* a, int64_
         case 5: err = run_bf16<5>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
         case 6: err = run_bf16<6>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
         case 7: err = run_bf16<7>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
+        case 8: err = run_bf16<8>(layout_a, layout_b, a, lda, b, ldb, p, alpha, beta, c, ldc, stream, info); break;
         default: return hipErrorInvalidValue;
     }
     if (info) {

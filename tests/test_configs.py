@@ -187,7 +187,7 @@ def test_config2d_matvec_32768_exact_integers():
 # Automatic picks per phi on the 16384^3 workload: (kernel, slices or moduli) for each split form;
 # ("mfma_f64", 0) is the FP64 kernel serving a spread the form cannot hold.
 CONFIG3_PICKS = {
-    0.5: {"split_crt": ("split_crt", 15), "split_int8": ("split_int8", 9), "split_bf16": ("mfma_f64", 0)},
+    0.5: {"split_crt": ("split_crt", 15), "split_int8": ("split_int8", 9), "split_bf16": ("split_bf16", 8)},
     2.0: {"split_crt": ("split_crt", 16), "split_int8": ("mfma_f64", 0), "split_bf16": ("mfma_f64", 0)},
     8.0: {"split_crt": ("mfma_f64", 0), "split_int8": ("mfma_f64", 0), "split_bf16": ("mfma_f64", 0)},
 }

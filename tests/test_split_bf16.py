@@ -41,7 +41,7 @@ def _case(rng, la, lb, m, n, k, S, alpha, beta, pad=0, off=0):
     assert same.all(), (la, lb, m, n, k, S, int((~same).sum()), float(np.nanmax(np.abs(c - exp))))
 
 
-@pytest.mark.parametrize("S", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("S", [3, 4, 5, 6, 7, 8])
 def test_bf16_bitwise_equal_to_restatement(S):
     rng = np.random.default_rng(200 + S)
     _case(rng, 0, 0, 64, 64, 48, S, 1.0, 0.0)
@@ -53,10 +53,11 @@ def test_bf16_layouts(la, lb):
     _case(np.random.default_rng(la * 2 + lb), la, lb, 77, 65, 41, 5, 2.5, -0.5, pad=1, off=3)
 
 
-def test_bf16_fp64_flush_chunks():
-    S = 7
+@pytest.mark.parametrize("S", [7, 8])
+def test_bf16_fp64_flush_chunks(S):
+    # S = 8: exact FP32 runs of 2 steps (2^24 / (8 * 255^2 * 16) = 2.02), INT32 flushes every 4128 k
     assert orc.split_bf16_chunk_k(S, 5000) < 5000
-    _case(np.random.default_rng(3), 0, 0, 40, 36, 5000, S, 1.0, 0.25)
+    _case(np.random.default_rng(3 + S), 0, 0, 40, 36, 5000, S, 1.0, 0.25)
 
 
 def test_bf16_accuracy_matches_int8_at_equal_bits():

@@ -81,6 +81,61 @@ def test_split_gemm_within_closed_form_bound(phi):
         prev = mx
 
 
+def _rigorous_bound(S, bits, k, ea, eb):
+    """Worst case of the slice forms' truncation, per unit of the row / column scales 2^ea, 2^eb: each
+    scaled entry (|x| < 1) is its S digits plus a residual below 2^-(bits S); the kept digit pairs
+    t + u <= S - 1 miss, for each t >= 1, digit t of one operand (below 2^bits) times the other's
+    digits after the first S - t, whose sum is the remaining fraction minus the residual and so below
+    2 * 2^-(bits (S - t)). Per term: 2^-(bits S) (2.01 + 2 (S - 1))."""
+    per = 2.0 ** (-bits * S) * (2.01 + 2.0 * (S - 1))
+    return k * per * np.ldexp(1.0, ea)[:, None] * np.ldexp(1.0, eb)[None, :]
+
+
+@pytest.mark.parametrize("phi", [0.5, 2.0])
+def test_bf16_digit_restatement_within_closed_form_bound(phi):
+    """The BF16 form's restatement (8-bit digits, exact FP32 runs, INT32 flushes and FP64 flushes every
+    chunk) stays inside the worst-case truncation bound for every S, across several FP64 flushes
+    (k = 9000 > 2 chunks at S = 8); an error in the digit width, the pair set or the recombination weights
+    can exceed it (one dropped diagonal costs up to 2^8 times the unit the bound is counted in)."""
+    rng = np.random.default_rng(int(phi * 10) + 5)
+    m, n, k = 16, 12, 9000
+    A = _operand(rng, m, k, phi)
+    B = _operand(rng, k, n, phi)
+    a, b = A.ravel().copy(), B.ravel().copy()
+    ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    hi, lo, ab = orc.dd_entries(0, 0, k, a, k, b, n, ii.ravel(), jj.ravel())
+    exact = (hi + lo).reshape(m, n)
+    ea = np.array([orc.split_exponent(A[i]) for i in range(m)])
+    eb = np.array([orc.split_exponent(B[:, j].copy()) for j in range(n)])
+    assert orc.split_bf16_chunk_k(8, k) < k // 2
+    for S in range(3, 9):
+        c = np.zeros(m * n)
+        orc.split_gemm_bf16(0, 0, m, n, k, 1.0, a, k, b, n, 0.0, c, n, S)
+        err = np.abs(c.reshape(m, n) - exact)
+        chunks = -(-k // orc.split_bf16_chunk_k(S, k))
+        bound = _rigorous_bound(S, 8, k, ea, eb) + 4 * (S + chunks) * U * (np.abs(exact) + ab.reshape(m, n))
+        assert np.all(err <= bound), (S, float(np.max(err / bound)))
+
+
+def test_int8_digit_restatement_within_rigorous_bound():
+    """The same worst-case bound with 7-bit digits, for the INT8 slice form's restatement."""
+    rng = np.random.default_rng(77)
+    m, n, k = 16, 12, 3000
+    A = _operand(rng, m, k, 1.0)
+    B = _operand(rng, k, n, 1.0)
+    a, b = A.ravel().copy(), B.ravel().copy()
+    ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    hi, lo, ab = orc.dd_entries(0, 0, k, a, k, b, n, ii.ravel(), jj.ravel())
+    exact = (hi + lo).reshape(m, n)
+    ea = np.array([orc.split_exponent(A[i]) for i in range(m)])
+    eb = np.array([orc.split_exponent(B[:, j].copy()) for j in range(n)])
+    for S in range(3, 10):
+        c = np.zeros(m * n)
+        orc.split_gemm(0, 0, m, n, k, 1.0, a, k, b, n, 0.0, c, n, S)
+        bound = _rigorous_bound(S, 7, k, ea, eb) + 4 * (S + 1) * U * (np.abs(exact) + ab.reshape(m, n))
+        assert np.all(np.abs(c.reshape(m, n) - exact) <= bound), S
+
+
 def _numpy_split(A, B, S, alpha, beta, C0, chunk_k):
     """Independent emulation: slices via numpy, diagonal sums by float64 matmul of
     small integers (exact: |sum| < 2^53), same recombination order."""
