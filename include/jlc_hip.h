@@ -221,6 +221,20 @@ int jlc_dorgqr_dev(int64_t m, int64_t n, const double* a, int64_t lda, const dou
 int jlc_dqr_errors_dev(int64_t m, int64_t n, const double* a_orig, int64_t lda_orig, const double* qr, int64_t lda,
                        const double* tau, int64_t nb, double* recon_err, double* orth_err);
 
+/* ---- tall-skinny QR (TallSkinnyQR.decompose, which CommunicationAvoidingQR.decompose delegates to;
+ * src/main/java/net/faulj/decomposition/qr/TallSkinnyQR.java:22-90, CommunicationAvoidingQR.java:18-20).
+ * k = min(m, n). m <= n: the thin Householder QR of A. m > n: row blocks of max(n, 64) rows, a thin
+ * QR of each (one batched launch while a block fits in LDS), a QR of the R factors stacked in block
+ * order, and each block's rows of Q as its block Q times its rows of the stack's Q (applied as
+ * reflectors, one batched launch). Outputs the explicit thin Q (m x k) and R (k x n, upper
+ * trapezoidal), with the reference's reflector convention.
+ * jlc_dtsqr_dev: column-major device arrays (a is not modified); jlc_dtsqr: row-major host arrays,
+ * the reference's Matrix storage (lda >= n, ldq >= k, ldr >= n). Synchronous. */
+int jlc_dtsqr_dev(int64_t m, int64_t n, const double* a, int64_t lda, double* q, int64_t ldq, double* r, int64_t ldr,
+                  void* stream, jlc_stats* stats);
+int jlc_dtsqr(int64_t m, int64_t n, const double* a, int64_t lda, double* q, int64_t ldq, double* r, int64_t ldr,
+              jlc_stats* stats);
+
 /* Batched small QR: each matrix (m*n <= 20220, n <= 256) factored by one
  * workgroup in LDS, unblocked, same reflector convention. */
 typedef struct jlc_qr_desc {

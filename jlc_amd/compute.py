@@ -19,6 +19,8 @@ reference's own tests:
   gemmStrided*
 * BlockedHessenbergQR.         eigen/qr/BlockedHessenbergQR.java:277-345 (compact-WY updates)
   applyBlockLeft / Right
+* TallSkinnyQR /                decomposition/qr/TallSkinnyQR.java:22-90, CommunicationAvoidingQR.java:18-20
+  CommunicationAvoidingQR
 * Matrix.multiply /             Matrix.java:1120-1122; BlockedMultiply.java:43-105, with
   BlockedMultiply.multiply      DispatchPolicy.selectForMultiply (:179-193, :254-279) and
                                 CudaGemm.multiply (CudaGemm.java:32-41) -> HipGemm.multiply
@@ -90,6 +92,16 @@ class Matrix:
 
     def to_numpy(self) -> np.ndarray:
         return self._data.reshape(self._rows, self._cols)
+
+    @staticmethod
+    def Identity(n: int) -> "Matrix":
+        """Matrix.Identity (Matrix.java:1007-1016): IllegalArgumentException for n <= 0."""
+        if n <= 0:
+            raise IllegalArgumentException("Size n must be positive")
+        return Matrix(n, n, np.eye(n).reshape(-1))
+
+    def copy(self) -> "Matrix":
+        return Matrix(self._rows, self._cols, self._data.copy())
 
     @staticmethod
     def zero(rows: int, cols: int) -> "Matrix":
@@ -540,6 +552,50 @@ class BlockedHessenbergQR:
         native.dgemm(ROW, ROW, n, panelWidth, panelWidth, 1.0, W, panelWidth, T, panelWidth, 0.0, W2, panelWidth)
         native.dgemm(ROW, COL, n, nCols, panelWidth, -1.0, W2, panelWidth, V, panelWidth, 1.0, a, n, mode,
                      b_off=colOffset * panelWidth, c_off=colStart)
+
+
+class QRResult:
+    """QRResult (src/main/java/net/faulj/decomposition/result/QRResult.java:102-135): A, Q, R."""
+
+    def __init__(self, A: Matrix, Q: Matrix, R: Matrix):
+        self._A, self._Q, self._R = A, Q, R
+
+    def getA(self) -> Matrix:
+        return self._A
+
+    def getQ(self) -> Matrix:
+        return self._Q
+
+    def getR(self) -> Matrix:
+        return self._R
+
+
+class TallSkinnyQR:
+    """TallSkinnyQR.decompose (src/main/java/net/faulj/decomposition/qr/TallSkinnyQR.java:22-90) on the
+    GPU through jlc_dtsqr: the reference's two-level tree (row blocks of max(n, 64) rows, a QR of the
+    stacked R factors), thin Q (m x min(m, n)) and R (min(m, n) x n)."""
+
+    DEFAULT_BLOCK_SIZE = 64
+
+    @staticmethod
+    def decompose(A: Matrix) -> QRResult:
+        if A is None:
+            raise IllegalArgumentException("Matrix must not be null")
+        m, n = A.getRowCount(), A.getColumnCount()
+        if m == 0 or n == 0:
+            return QRResult(A, Matrix.Identity(m), A.copy())  # Identity(0) raises, as in the reference
+        k = min(m, n)
+        Q, R = Matrix(m, k), Matrix(k, n)
+        native.dtsqr(m, n, np.ascontiguousarray(A.getRawData()), Q.getRawData(), R.getRawData())
+        return QRResult(A, Q, R)
+
+
+class CommunicationAvoidingQR:
+    """CommunicationAvoidingQR.decompose delegates to TallSkinnyQR (CommunicationAvoidingQR.java:18-20)."""
+
+    @staticmethod
+    def decompose(A: Matrix) -> QRResult:
+        return TallSkinnyQR.decompose(A)
 
 
 HIP_POLICY = DispatchPolicy.builder().enableHip(True).hipMinFlops(0).parallelism(1).build()

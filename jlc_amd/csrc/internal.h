@@ -148,6 +148,22 @@ hipError_t launch_fill_eye(double* x, int64_t ld, int rows, int cols, double dia
 hipError_t launch_sumsq_partial(const double* x, int64_t ld, int rows, int cols, double* part, int nparts,
                                 hipStream_t stream);
 hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_mn, hipStream_t stream);
+// Upper trapezoids of stacked row blocks into a column-major matrix (TSQR's stack of R factors, or R).
+hipError_t launch_copy_upper_blocks(const double* src, int64_t lds, int64_t stride, int per, int rows_total, int cols,
+                                    double* dst, int64_t ldd, hipStream_t stream);
+// Rows of a stacked matrix spread back to row blocks of `stride` rows (per rows each, the rest zero).
+hipError_t launch_spread_blocks(const double* src, int64_t lds, int per, int stride, int rows_total, int cols,
+                                int rows_src, double* dst, int64_t ldd, hipStream_t stream);
+// X <- Q X for many small factored blocks (m <= 256, nx <= 256, m * nx <= QR_APPLY_MAX_MNX).
+struct QrApplyDesc {
+    int32_t m, nx, k, pad;
+    int64_t lda, ldx;
+    const double* a;
+    const double* tau;
+    double* x;
+};
+constexpr int QR_APPLY_MAX_MNX = (163840 - 4096 - 256) / 8;
+hipError_t launch_qr_batched_apply_q(const QrApplyDesc* descs, int count, int max_mnx, hipStream_t stream);
 
 // BF16-slice variant (8-bit digits, slices 3..8).
 // CRT split form: N pairwise coprime moduli <= 256, residues in int8.

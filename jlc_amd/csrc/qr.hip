@@ -454,9 +454,108 @@ __global__ __launch_bounds__(256) void qr_batched_small(const QrBatchDesc* __res
     }
 }
 
+// Upper trapezoids of stacked row blocks: dst(s, j), s < rows_total, column-major (ldd), is row
+// i = s % per of block t = s / per of src (block t starting at row t * stride, column-major, lds),
+// kept on and above the diagonal (i <= j), zero below it. With one block this extracts R.
+__global__ __launch_bounds__(256) void copy_upper_blocks(const double* __restrict__ src, int64_t lds,
+                                                         int64_t stride, int per, int rows_total, int cols,
+                                                         double* __restrict__ dst, int64_t ldd) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)rows_total * cols) return;
+    const int j = (int)(idx / rows_total), s = (int)(idx % rows_total);
+    const int t = s / per, i = s - t * per;
+    dst[(int64_t)j * ldd + s] = i <= j ? src[(int64_t)j * lds + (int64_t)t * stride + i] : 0.0;
+}
+
+// TSQR's start of Q: dst(i, j) for row i of block t = i / stride (ii = i % stride) is row t * per + ii of
+// src when ii < per and that row exists (rows_src of them), else zero.
+__global__ __launch_bounds__(256) void spread_blocks(const double* __restrict__ src, int64_t lds, int per,
+                                                     int stride, int rows_total, int cols, int rows_src,
+                                                     double* __restrict__ dst, int64_t ldd) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)rows_total * cols) return;
+    const int j = (int)(idx / rows_total), i = (int)(idx % rows_total);
+    const int t = i / stride, ii = i - t * stride, sr = t * per + ii;
+    dst[(int64_t)j * ldd + i] = (ii < per && sr < rows_src) ? src[(int64_t)j * lds + sr] : 0.0;
+}
+
+// Batched X <- H_0 H_1 ... H_{k-1} X: one workgroup per descriptor holds its column-major m x nx block
+// of X in LDS (m <= 256, nx <= 256) and applies the block's reflectors (v_j below the diagonal of the
+// factored block, v_j[j] = 1, scalar tau_j) last to first: w = v_j^T X, X -= tau_j v_j w.
+__global__ __launch_bounds__(256) void qr_batched_apply_q(const QrApplyDesc* __restrict__ descs) {
+    extern __shared__ double x_s[];
+    __shared__ double vcol[256];
+    __shared__ double wrow[256];
+    const QrApplyDesc d = descs[blockIdx.x];
+    const int m = d.m, nx = d.nx;
+    for (int idx = threadIdx.x; idx < m * nx; idx += blockDim.x) {
+        const int c = idx / m, i = idx % m;
+        x_s[idx] = d.x[(int64_t)c * d.ldx + i];
+    }
+    const int sub = threadIdx.x & 7, slot = threadIdx.x >> 3;
+    for (int j = d.k - 1; j >= 0; j--) {
+        const double t = d.tau[j];
+        if (t == 0.0) continue;  // the same value in every thread of the workgroup
+        __syncthreads();
+        for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) vcol[i] = d.a[(int64_t)j * d.lda + i];
+        if (threadIdx.x == 0) vcol[j] = 1.0;
+        __syncthreads();
+        // w_c = sum_{i >= j} v_i x(i, c): eight threads per column, 32 columns at a time.
+        for (int cb = 0; cb < nx; cb += 32) {
+            const int c = cb + slot;
+            double sacc = 0.0;
+            if (c < nx) {
+                const double* xc = x_s + (int64_t)c * m;
+                for (int i = j + sub; i < m; i += 8) sacc += vcol[i] * xc[i];
+            }
+            sacc += __shfl_xor(sacc, 1, 64);
+            sacc += __shfl_xor(sacc, 2, 64);
+            sacc += __shfl_xor(sacc, 4, 64);
+            if (c < nx && sub == 0) wrow[c] = sacc;
+        }
+        __syncthreads();
+        const int mj = m - j;
+        for (int idx = threadIdx.x; idx < mj * nx; idx += blockDim.x) {
+            const int c = idx / mj, i = j + idx % mj;
+            x_s[(int64_t)c * m + i] -= t * vcol[i] * wrow[c];
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < m * nx; idx += blockDim.x) {
+        const int c = idx / m, i = idx % m;
+        d.x[(int64_t)c * d.ldx + i] = x_s[idx];
+    }
+}
+
 }  // namespace
 
 int qr_panel_chunk() { return PCHUNK; }
+
+hipError_t launch_copy_upper_blocks(const double* src, int64_t lds, int64_t stride, int per, int rows_total, int cols,
+                                    double* dst, int64_t ldd, hipStream_t stream) {
+    const int64_t total = (int64_t)rows_total * cols;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_upper_blocks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, src, lds, stride,
+                       per, rows_total, cols, dst, ldd);
+    return hipGetLastError();
+}
+
+hipError_t launch_spread_blocks(const double* src, int64_t lds, int per, int stride, int rows_total, int cols,
+                                int rows_src, double* dst, int64_t ldd, hipStream_t stream) {
+    const int64_t total = (int64_t)rows_total * cols;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(spread_blocks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, src, lds, per,
+                       stride, rows_total, cols, rows_src, dst, ldd);
+    return hipGetLastError();
+}
+
+hipError_t launch_qr_batched_apply_q(const QrApplyDesc* descs, int count, int max_mnx, hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    const size_t lds = (size_t)max_mnx * sizeof(double);
+    (void)hipFuncSetAttribute((const void*)qr_batched_apply_q, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(qr_batched_apply_q, dim3(count), dim3(256), lds, stream, descs);
+    return hipGetLastError();
+}
 
 hipError_t launch_qr_panel_fused(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part,
                                  double* dots, double* xbuf, hipStream_t stream) {

@@ -1567,6 +1567,48 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     return JLC_OK;
 }
 
+namespace {
+// X <- H_0 H_1 ... H_{kk-1} X for the reflectors below the diagonal of the column-major factored matrix
+// a (m rows, lda, tau), X column-major m x nx (ldx): panels of nb applied last to first as compact-WY
+// updates on the library GEMM (buildQ's backward accumulation, HouseholderQR.java:390-444). When X
+// starts as [I 0] (identity_start), rows k0.. of columns < k0 are still zero when panel k0 is applied,
+// so only columns k0.. are updated.
+int apply_q_panels(int64_t m, int64_t kk, const double* a, int64_t lda, const double* tau, int64_t nb, double* x,
+                   int64_t ldx, int64_t nx, bool identity_start, hipStream_t s) {
+    if (kk <= 0 || nx <= 0) return JLC_OK;
+    DevBuf vc, g, t, y1, y2;
+    hipError_t e = vc.alloc((size_t)nb * m * 8);
+    if (e == hipSuccess) e = g.alloc((size_t)nb * nb * 8);
+    if (e == hipSuccess) e = t.alloc((size_t)nb * nb * 8);
+    if (e == hipSuccess) e = y1.alloc((size_t)nx * nb * 8);
+    if (e == hipSuccess) e = y2.alloc((size_t)nx * nb * 8);
+    if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "Q workspace allocation failed: %s", hipGetErrorString(e));
+    jlc::TileInfo info;
+    int rc = JLC_OK;
+    const int64_t last = ((kk - 1) / nb) * nb;
+    for (int64_t k0 = last; k0 >= 0; k0 -= nb) {
+        const int64_t k1 = k0 + nb < kk ? k0 + nb : kk;
+        const int64_t nbk = k1 - k0, mk = m - k0;
+        const int64_t c0 = identity_start ? k0 : 0, nq = nx - c0;
+        if (nq <= 0) continue;
+        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
+        // X <- (I - V T V^T) X on rows k0..m-1:  X^T -= ((X^T V) T^T) V^T  (X^T row-major, ld = ldx).
+        double* xt = x + c0 * ldx + k0;
+        if ((rc = gemm_on(JLC_ROW, JLC_COL, nq, nbk, mk, 1.0, xt, ldx, vc.d(), mk, 0.0, y1.d(), nbk, JLC_MODE_FP64, s,
+                          &info)) != JLC_OK)
+            return rc;
+        if ((rc = gemm_on(JLC_ROW, JLC_COL, nq, nbk, nbk, 1.0, y1.d(), nbk, t.d(), nbk, 0.0, y2.d(), nbk,
+                          JLC_MODE_FP64, s, &info)) != JLC_OK)
+            return rc;
+        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nq, mk, nbk, -1.0, y2.d(), nbk, vc.d(), mk, 1.0, xt, ldx, JLC_MODE_FP64,
+                          s, &info)) != JLC_OK)
+            return rc;
+    }
+    JLC_HIP_CHECK(hipStreamSynchronize(s));  // the workspace is freed on return
+    return JLC_OK;
+}
+}  // namespace
+
 int jlc_dorgqr_dev(int64_t m, int64_t n, const double* a, int64_t lda, const double* tau, int64_t nb, double* q,
                    int64_t ldq, void* stream) {
     if (m < 0 || n < 0 || n > m || m > INT32_MAX) return fail(JLC_ERR_INVALID_ARG, "need 0 <= n <= m");
@@ -1576,34 +1618,144 @@ int jlc_dorgqr_dev(int64_t m, int64_t n, const double* a, int64_t lda, const dou
     int rc = ensure_init();
     if (rc != JLC_OK) return rc;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
-    DevBuf vc, g, t, y1, y2;
-    hipError_t e = vc.alloc((size_t)nb * m * 8);
-    if (e == hipSuccess) e = g.alloc((size_t)nb * nb * 8);
-    if (e == hipSuccess) e = t.alloc((size_t)nb * nb * 8);
-    if (e == hipSuccess) e = y1.alloc((size_t)n * nb * 8);
-    if (e == hipSuccess) e = y2.alloc((size_t)n * nb * 8);
-    if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "Q workspace allocation failed: %s", hipGetErrorString(e));
     // Q^T (n x m, row-major, ld = ldq) starts as [I 0].
     JLC_HIP_CHECK(jlc::launch_fill_eye(q, ldq, (int)n, (int)m, 1.0, s));
+    return apply_q_panels(m, n, a, lda, tau, nb, q, ldq, n, true, s);
+}
+
+int jlc_dtsqr_dev(int64_t m, int64_t n, const double* a, int64_t lda, double* q, int64_t ldq, double* r, int64_t ldr,
+                  void* stream, jlc_stats* stats) {
+    double t0 = now_ms();
+    if (m < 0 || n < 0 || m > INT32_MAX || n > INT32_MAX) return fail(JLC_ERR_INVALID_ARG, "bad dimensions");
+    const int64_t k = m < n ? m : n;
     jlc::TileInfo info;
-    const int64_t last = ((n - 1) / nb) * nb;
-    for (int64_t k0 = last; k0 >= 0; k0 -= nb) {
-        const int64_t k1 = k0 + nb < n ? k0 + nb : n;
-        const int64_t nbk = k1 - k0, mk = m - k0, nq = n - k0;
-        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
-        // Q <- (I - V T V^T) Q on rows k0..m-1, columns k0..n-1:  Q^T -= ((Q^T V) T^T) V^T.
-        double* qt = q + k0 * ldq + k0;
-        if ((rc = gemm_on(JLC_ROW, JLC_COL, nq, nbk, mk, 1.0, qt, ldq, vc.d(), mk, 0.0, y1.d(), nbk, JLC_MODE_FP64, s,
-                          &info)) != JLC_OK)
-            return rc;
-        if ((rc = gemm_on(JLC_ROW, JLC_COL, nq, nbk, nbk, 1.0, y1.d(), nbk, t.d(), nbk, 0.0, y2.d(), nbk,
-                          JLC_MODE_FP64, s, &info)) != JLC_OK)
-            return rc;
-        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nq, mk, nbk, -1.0, y2.d(), nbk, vc.d(), mk, 1.0, qt, ldq, JLC_MODE_FP64,
-                          s, &info)) != JLC_OK)
-            return rc;
+    if (k == 0) {
+        fill_stats(stats, info, JLC_KERNEL_NONE, 0.0, now_ms() - t0);
+        return JLC_OK;
     }
-    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    if (!a || !q || !r || lda < m || ldq < m || ldr < k)
+        return fail(JLC_ERR_INVALID_ARG, "a, q, r must not be null; need lda >= m, ldq >= m, ldr >= min(m, n)");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : t_ctx.stream;
+    if (stats) JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
+    const int64_t nbq = n < 128 ? n : 128;
+    DevBuf w, tb;
+    hipError_t e = w.alloc((size_t)m * n * 8);
+    if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "TSQR workspace: %s", hipGetErrorString(e));
+    JLC_HIP_CHECK(hipMemcpy2DAsync(w.d(), (size_t)m * 8, a, (size_t)lda * 8, (size_t)m * 8, (size_t)n,
+                                   hipMemcpyDeviceToDevice, s));
+    if (m <= n) {
+        // HouseholderQR.decomposeThin (TallSkinnyQR.java:34-36): Q m x m, R m x n.
+        if ((e = tb.alloc((size_t)k * 8)) != hipSuccess) return fail(JLC_ERR_ALLOC, "TSQR tau: %s", hipGetErrorString(e));
+        if ((rc = jlc_dgeqrf_dev(m, n, w.d(), m, tb.d(), nbq, JLC_MODE_FP64, s, nullptr)) != JLC_OK) return rc;
+        if ((rc = jlc_dorgqr_dev(m, k, w.d(), m, tb.d(), k < 128 ? k : 128, q, ldq, s)) != JLC_OK) return rc;
+        JLC_HIP_CHECK(jlc::launch_copy_upper_blocks(w.d(), m, 0, (int)k, (int)k, (int)n, r, ldr, s));
+    } else {
+        // Row blocks of max(n, 64) rows, a thin QR of each, a thin QR of the R factors stacked in block
+        // order, Q's block rows = the block's Q times its rows of the stack's Q (TallSkinnyQR.java:38-88).
+        const int64_t b = n > 64 ? n : 64, nblk = (m + b - 1) / b;
+        const int64_t rows_last = m - (nblk - 1) * b;
+        const int64_t srows = (nblk - 1) * n + (rows_last < n ? rows_last : n);
+        DevBuf stk, tau2, q2, dd;
+        e = tb.alloc((size_t)nblk * n * 8);
+        if (e == hipSuccess) e = stk.alloc((size_t)srows * n * 8);
+        if (e == hipSuccess) e = tau2.alloc((size_t)n * 8);
+        if (e == hipSuccess) e = q2.alloc((size_t)srows * n * 8);
+        if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "TSQR workspace: %s", hipGetErrorString(e));
+        const bool batched = b <= 256 && b * n <= jlc::QR_BATCH_MAX_MN && b * n <= jlc::QR_APPLY_MAX_MNX;
+        if (batched) {
+            std::vector<jlc_qr_desc> qd((size_t)nblk);
+            for (int64_t t = 0; t < nblk; t++) {
+                const int64_t rows = t + 1 < nblk ? b : rows_last;
+                qd[(size_t)t] = jlc_qr_desc{(int32_t)rows, (int32_t)n, m, w.d() + t * b, tb.d() + t * n};
+            }
+            if ((rc = jlc_dgeqrf_batched_dev(nblk, qd.data(), s, nullptr)) != JLC_OK) return rc;
+        } else {
+            for (int64_t t = 0; t < nblk; t++) {
+                const int64_t rows = t + 1 < nblk ? b : rows_last;
+                if ((rc = jlc_dgeqrf_dev(rows, n, w.d() + t * b, m, tb.d() + t * n, nbq, JLC_MODE_FP64, s, nullptr)) !=
+                    JLC_OK)
+                    return rc;
+            }
+        }
+        JLC_HIP_CHECK(jlc::launch_copy_upper_blocks(w.d(), m, b, (int)n, (int)srows, (int)n, stk.d(), srows, s));
+        if ((rc = jlc_dgeqrf_dev(srows, n, stk.d(), srows, tau2.d(), nbq, JLC_MODE_FP64, s, nullptr)) != JLC_OK)
+            return rc;
+        JLC_HIP_CHECK(jlc::launch_copy_upper_blocks(stk.d(), srows, 0, (int)n, (int)n, (int)n, r, ldr, s));
+        if ((rc = jlc_dorgqr_dev(srows, n, stk.d(), srows, tau2.d(), nbq, q2.d(), srows, s)) != JLC_OK) return rc;
+        // Q's rows of block t start as [Q2 rows t*n.. of the block's R; 0], then X <- Q_t X.
+        JLC_HIP_CHECK(jlc::launch_spread_blocks(q2.d(), srows, (int)n, (int)b, (int)m, (int)n, (int)srows, q, ldq, s));
+        if (batched) {
+            std::vector<jlc::QrApplyDesc> ad((size_t)nblk);
+            for (int64_t t = 0; t < nblk; t++) {
+                const int64_t rows = t + 1 < nblk ? b : rows_last;
+                ad[(size_t)t] = jlc::QrApplyDesc{(int32_t)rows, (int32_t)n, (int32_t)(rows < n ? rows : n), 0, m, ldq,
+                                                 w.d() + t * b, tb.d() + t * n, q + t * b};
+            }
+            if ((e = dd.alloc(ad.size() * sizeof(jlc::QrApplyDesc))) != hipSuccess)
+                return fail(JLC_ERR_ALLOC, "TSQR descriptors: %s", hipGetErrorString(e));
+            JLC_HIP_CHECK(hipMemcpyAsync(dd.p, ad.data(), ad.size() * sizeof(jlc::QrApplyDesc), hipMemcpyHostToDevice, s));
+            JLC_HIP_CHECK(jlc::launch_qr_batched_apply_q(static_cast<const jlc::QrApplyDesc*>(dd.p), (int)nblk,
+                                                         (int)(b * n), s));
+        } else {
+            for (int64_t t = 0; t < nblk; t++) {
+                const int64_t rows = t + 1 < nblk ? b : rows_last;
+                if ((rc = apply_q_panels(rows, rows < n ? rows : n, w.d() + t * b, m, tb.d() + t * n, nbq, q + t * b,
+                                         ldq, n, false, s)) != JLC_OK)
+                    return rc;
+            }
+        }
+    }
+    double kms = 0.0;
+    if (stats) {
+        JLC_HIP_CHECK(hipEventRecord(t_ctx.ev1, s));
+        JLC_HIP_CHECK(hipEventSynchronize(t_ctx.ev1));
+        float f = 0.0f;
+        JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
+        kms = f;
+    } else {
+        JLC_HIP_CHECK(hipStreamSynchronize(s));  // the workspace is freed on return
+    }
+    fill_stats(stats, info, JLC_KERNEL_QR, kms, now_ms() - t0);
+    return JLC_OK;
+}
+
+int jlc_dtsqr(int64_t m, int64_t n, const double* a, int64_t lda, double* q, int64_t ldq, double* r, int64_t ldr,
+              jlc_stats* stats) {
+    // Host row-major operands (the reference's Matrix storage): A (m x n, lda >= n), Q (m x k, ldq >= k),
+    // R (k x n, ldr >= n), k = min(m, n); transposed into column-major device copies and back.
+    double t0 = now_ms();
+    if (m < 0 || n < 0 || m > INT32_MAX || n > INT32_MAX) return fail(JLC_ERR_INVALID_ARG, "bad dimensions");
+    const int64_t k = m < n ? m : n;
+    jlc::TileInfo info;
+    if (k == 0) {
+        fill_stats(stats, info, JLC_KERNEL_NONE, 0.0, now_ms() - t0);
+        return JLC_OK;
+    }
+    if (!a || !q || !r || lda < n || ldq < k || ldr < n)
+        return fail(JLC_ERR_INVALID_ARG, "a, q, r must not be null; need lda >= n, ldq >= min(m, n), ldr >= n");
+    int rc = ensure_init();
+    if (rc != JLC_OK) return rc;
+    std::vector<double> h((size_t)(m * n));
+    for (int64_t i = 0; i < m; i++)
+        for (int64_t j = 0; j < n; j++) h[(size_t)(j * m + i)] = a[i * lda + j];
+    DevBuf da, dq, dr;
+    hipError_t e = da.alloc((size_t)m * n * 8);
+    if (e == hipSuccess) e = dq.alloc((size_t)m * k * 8);
+    if (e == hipSuccess) e = dr.alloc((size_t)k * n * 8);
+    if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "TSQR buffers: %s", hipGetErrorString(e));
+    JLC_HIP_CHECK(hipMemcpy(da.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    jlc_stats st{};
+    if ((rc = jlc_dtsqr_dev(m, n, da.d(), m, dq.d(), m, dr.d(), k, t_ctx.stream, &st)) != JLC_OK) return rc;
+    std::vector<double> hq((size_t)(m * k)), hr((size_t)(k * n));
+    JLC_HIP_CHECK(hipMemcpy(hq.data(), dq.p, hq.size() * 8, hipMemcpyDeviceToHost));
+    JLC_HIP_CHECK(hipMemcpy(hr.data(), dr.p, hr.size() * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < m; i++)
+        for (int64_t j = 0; j < k; j++) q[i * ldq + j] = hq[(size_t)(j * m + i)];
+    for (int64_t i = 0; i < k; i++)
+        for (int64_t j = 0; j < n; j++) r[i * ldr + j] = hr[(size_t)(j * k + i)];
+    fill_stats(stats, info, JLC_KERNEL_QR, st.kernel_ms, now_ms() - t0);
     return JLC_OK;
 }
 

@@ -136,6 +136,8 @@ _SIGNATURES = {
     "jlc_dgeqrf_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _i, _p, C.POINTER(Stats)]),
     "jlc_dorgqr_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p]),
     "jlc_dqr_errors_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, C.POINTER(_d), C.POINTER(_d)]),
+    "jlc_dtsqr_dev": (_i, [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p, C.POINTER(Stats)]),
+    "jlc_dtsqr": (_i, [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, C.POINTER(Stats)]),
     "jlc_qr_batched_plan_create": (_i, [_i64, _p, C.POINTER(_p)]),
     "jlc_qr_batched_plan_replay": (_i, [_p, _p, C.POINTER(Stats)]),
     "jlc_qr_batched_plan_destroy": (_i, [_p]),
@@ -421,6 +423,22 @@ def dqr_errors_dev(m: int, n: int, a_orig: int, lda_orig: int, qr: int, lda: int
     r, o = C.c_double(), C.c_double()
     check(load().jlc_dqr_errors_dev(m, n, a_orig, lda_orig, qr, lda, tau, nb, C.byref(r), C.byref(o)))
     return r.value, o.value
+
+
+def dtsqr_dev(m: int, n: int, a: int, lda: int, q: int, ldq: int, r: int, ldr: int,
+              stream_handle: int | None = None) -> Stats:
+    """Tall-skinny QR of the column-major device matrix a: thin Q (m x min(m,n)) and R (min(m,n) x n)."""
+    st = Stats()
+    check(load().jlc_dtsqr_dev(m, n, a, lda, q, ldq, r, ldr, stream_handle, C.byref(st)))
+    return st
+
+
+def dtsqr(m: int, n: int, a: np.ndarray, q: np.ndarray, r: np.ndarray) -> Stats:
+    """Tall-skinny QR of the row-major host matrix a (m x n) into row-major q (m x k) and r (k x n)."""
+    k = min(m, n)
+    st = Stats()
+    check(load().jlc_dtsqr(m, n, _ptr(a), max(1, n), _ptr(q), max(1, k), _ptr(r), max(1, n), C.byref(st)))
+    return st
 
 
 class QrBatchedPlan:

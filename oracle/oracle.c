@@ -496,6 +496,71 @@ void oracle_qr_form_q(int64_t m, int64_t n, const double* qr, int64_t lda, const
     }
 }
 
+/* Thin QR of the column-major m x n matrix a (lda): Q (m x k, ldq) and the upper-trapezoidal R
+ * (k x n, ldr), k = min(m, n) (HouseholderQR.decomposeThin, HouseholderQR.java:27, :52-76). */
+static void thin_qr(int64_t m, int64_t n, const double* a, int64_t lda, double* q, int64_t ldq, double* r,
+                    int64_t ldr) {
+    const int64_t k = m < n ? m : n;
+    double* w = (double*)malloc(sizeof(double) * (size_t)(m * n + 1));
+    double* tau = (double*)calloc((size_t)(k + 1), sizeof(double));
+    for (int64_t j = 0; j < n; j++) memcpy(w + j * m, a + j * lda, sizeof(double) * (size_t)m);
+    oracle_qr_factor(m, n, w, m, tau, 64);
+    oracle_qr_form_q(m, k, w, m, tau, q, ldq);
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t i = 0; i < k; i++) r[j * ldr + i] = i <= j ? w[j * m + i] : 0.0;
+    free(w);
+    free(tau);
+}
+
+/* TallSkinnyQR.decompose (src/main/java/net/faulj/decomposition/qr/TallSkinnyQR.java:22-90): m <= n is
+ * a thin QR of A; otherwise row blocks of max(n, 64) rows, a thin QR of each, a thin QR of the R
+ * factors stacked in block order, and each block's rows of Q as its Q times its rows of the stack's
+ * Q (a left-to-right sum). Column-major A (lda), Q (m x k, ldq), R (k x n, ldr), k = min(m, n). */
+void oracle_tsqr(int64_t m, int64_t n, const double* a, int64_t lda, double* q, int64_t ldq, double* r, int64_t ldr) {
+    if (m <= 0 || n <= 0) return;
+    if (m <= n) {
+        thin_qr(m, n, a, lda, q, ldq, r, ldr);
+        return;
+    }
+    const int64_t b = n > 64 ? n : 64;
+    const int64_t nblk = (m + b - 1) / b;
+    int64_t srows = 0;
+    for (int64_t t = 0; t < nblk; t++) {
+        const int64_t rows = m - t * b < b ? m - t * b : b;
+        srows += rows < n ? rows : n;
+    }
+    double* stack = (double*)calloc((size_t)(srows * n + 1), sizeof(double));
+    double* qblk = (double*)calloc((size_t)(m * n + 1), sizeof(double));  /* block t's Q at qblk + t*b*n */
+    double* blk = (double*)malloc(sizeof(double) * (size_t)(b * n + 1));
+    double* rblk = (double*)malloc(sizeof(double) * (size_t)(b * n + 1));
+    for (int64_t t = 0, off = 0; t < nblk; t++) {
+        const int64_t r0 = t * b, rows = m - r0 < b ? m - r0 : b, kt = rows < n ? rows : n;
+        for (int64_t j = 0; j < n; j++) memcpy(blk + j * rows, a + j * lda + r0, sizeof(double) * (size_t)rows);
+        thin_qr(rows, n, blk, rows, qblk + r0 * n, rows, rblk, kt);
+        for (int64_t j = 0; j < n; j++)
+            for (int64_t i = 0; i < kt; i++) stack[j * srows + off + i] = rblk[j * kt + i];
+        off += kt;
+    }
+    double* q2 = (double*)malloc(sizeof(double) * (size_t)(srows * n + 1));
+    thin_qr(srows, n, stack, srows, q2, srows, r, ldr);
+    for (int64_t t = 0, off = 0; t < nblk; t++) {
+        const int64_t r0 = t * b, rows = m - r0 < b ? m - r0 : b, kt = rows < n ? rows : n;
+        const double* qt = qblk + r0 * n;  /* rows x kt, column-major, ld rows */
+        for (int64_t j = 0; j < n; j++)
+            for (int64_t i = 0; i < rows; i++) {
+                double sum = 0.0;
+                for (int64_t p = 0; p < kt; p++) sum += qt[p * rows + i] * q2[j * srows + off + p];
+                q[j * ldq + r0 + i] = sum;
+            }
+        off += kt;
+    }
+    free(stack);
+    free(qblk);
+    free(blk);
+    free(rblk);
+    free(q2);
+}
+
 /* ------------------------------------------------------------------------- */
 /* Split-precision restatement (see oracle.h). Every FP64 step mirrors the
  * device code: frexp of the max |x|, ldexp scaling, v*128 / trunc / subtract

@@ -130,6 +130,8 @@ void oracle_wy_apply(int side, int trans_t, int64_t m, int64_t n, int64_t kb, co
                      const double* t, int64_t ldt, double* c, int64_t ldc, int threads);
 
 int oracle_qr_factor(int64_t m, int64_t n, double* a_colmajor, int64_t lda, double* tau, int64_t nb);
+/* TallSkinnyQR.decompose restated (oracle.c): column-major A, Q (m x min(m,n)), R (min(m,n) x n). */
+void oracle_tsqr(int64_t m, int64_t n, const double* a, int64_t lda, double* q, int64_t ldq, double* r, int64_t ldr);
 void oracle_qr_form_q(int64_t m, int64_t n, const double* qr_colmajor, int64_t lda, const double* tau,
                       double* q_colmajor, int64_t ldq);
 

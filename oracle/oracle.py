@@ -82,6 +82,7 @@ def _load():
                                     C.c_int, C.c_int, C.c_int, C.c_int]
     lib.oracle_qr_factor.argtypes = [_i64, _i64, _p, _i64, _p, _i64]
     lib.oracle_qr_form_q.argtypes = [_i64, _i64, _p, _i64, _p, _p, _i64]
+    lib.oracle_tsqr.argtypes = [_i64, _i64, _p, _i64, _p, _i64, _p, _i64]
     _lib = lib
     return lib
 
@@ -266,6 +267,15 @@ def qr_factor(a_colmajor: np.ndarray, m: int, n: int, lda: int | None = None, nb
     tau = np.zeros(min(m, n))
     _load().oracle_qr_factor(m, n, _ptr(a_colmajor), lda, _ptr(tau), nb)
     return tau
+
+
+def tsqr(a_colmajor: np.ndarray, m: int, n: int) -> tuple[np.ndarray, np.ndarray]:
+    """TallSkinnyQR.decompose restated: (Q column-major m x k, R column-major k x n), k = min(m, n)."""
+    k = min(m, n)
+    q = np.zeros(m * k)
+    r = np.zeros(k * n)
+    _load().oracle_tsqr(m, n, _ptr(np.ascontiguousarray(a_colmajor, dtype=np.float64)), m, _ptr(q), m, _ptr(r), k)
+    return q, r
 
 
 def qr_form_q(qr_colmajor: np.ndarray, tau: np.ndarray, m: int, n: int, lda: int | None = None) -> np.ndarray:
