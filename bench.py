@@ -558,9 +558,9 @@ def bench_qr_batched(args) -> int:
         # the matrices sit in LDS for the whole factorization: FP64 arithmetic against the FP64 peak
         "roofline": {"bound": "mfma", "achieved": round(flops / (ms * 1e-3) / 1e12, 4), "peak": FP64_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 5),
-                     "traffic": read_traffic("qr_batched_20000"), "kernel": "qr_batched_small",
+                     "traffic": read_traffic("qr_batched_20000"), "kernel": "qr_batched_wave",
                      "kernel_ms": round(ms, 3),
-                     "note": "one workgroup per matrix, a column step per barrier sequence: latency-bound"},
+                     "note": "one wavefront per matrix in LDS: LDS-bandwidth and latency bound"},
         "cpu_baseline": {"value": round(2000 / (cpu_ms * 1e-3), 1), "unit": "matrices/s", "cores": 1,
                          "kind": "port", "sample": f"first 2000 matrices, oracle HouseholderQR restatement, "
                                                    f"1 thread ({cpu_ms / 1e3:.2f} s)"},

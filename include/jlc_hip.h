@@ -236,7 +236,9 @@ int jlc_dtsqr(int64_t m, int64_t n, const double* a, int64_t lda, double* q, int
               jlc_stats* stats);
 
 /* Batched small QR: each matrix (m*n <= 20220, n <= 256) factored by one
- * workgroup in LDS, unblocked, same reflector convention. */
+ * wavefront in LDS, unblocked, same reflector convention (the HouseholderQR
+ * column sequence, src/main/java/net/faulj/decomposition/qr/HouseholderQR.java:219-291).
+ * No counterpart in the reference, which factors one matrix per call. */
 typedef struct jlc_qr_desc {
     int32_t m, n;
     int64_t lda;
@@ -254,7 +256,8 @@ int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream
  * hip_min_flops (offload gate, the HIP analogue of GemmDispatch.CUDA_THRESHOLD,
  * src/main/java/net/faulj/compute/GemmDispatch.java:32), splitk_tiles_per_cu,
  * splitk_min_kslice, grouped_waves_per_simd, grouped_xcds (tile lists dealt per XCD;
- * 1 = plain order), crt_bits_offset, crt_bits_slope, crt_waves, crt_group, crt_xcd.
+ * 1 = plain order), crt_bits_offset, crt_bits_slope, crt_waves, crt_group, crt_xcd,
+ * qr_wave_max_lds (batched QR size classes up to this many LDS doubles run one wavefront per matrix).
  * Text file, one "key value" per line.
  * On first use the library loads $JLC_TUNE_TABLE, or else the table shipped
  * beside it (<libdir>/../tuning/mi355x.tune); jlc_tune_load replaces it later. */

@@ -39,6 +39,11 @@ struct TuneTable {
     int crt_waves = 8;                // CRT product kernel: 4 (128 x 128 per wavefront) or 8 (128 x 64)
     int crt_group = 16;               // CRT product kernel: tile rows per column strip of the tile order
     int crt_xcd = 0;                  // CRT product kernel: 1 = keep neighbouring tiles on one XCD
+    int qr_wave_max_lds = 2560;       // batched QR: size classes up to this many LDS doubles run one
+                                      // wavefront per matrix, larger ones qr_block_threads threads
+    int qr_block_threads = 256;       // batched QR: threads per matrix above qr_wave_max_lds (256/512/1024)
+    int qr_reg_max_m = 192;           // batched QR: matrices with m <= this (at most 192), n <= 128 held
+                                      // in registers; larger ones in LDS
 };
 TuneTable tune_snapshot();
 
@@ -147,7 +152,18 @@ hipError_t launch_extract_r(const double* at, int64_t lda, int n, double* r, hip
 hipError_t launch_fill_eye(double* x, int64_t ld, int rows, int cols, double diag, hipStream_t stream);
 hipError_t launch_sumsq_partial(const double* x, int64_t ld, int rows, int cols, double* part, int nparts,
                                 hipStream_t stream);
-hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_mn, hipStream_t stream);
+// threads = 64 (one wavefront per matrix), 256, 512 or 1024; max_lds = the largest
+// qr_batched_lds_entries of the launch (doubles).
+hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_lds, int threads,
+                                   hipStream_t stream);
+// Batched QR with each matrix in registers: m <= 8 * rows_per_thread (4, 8, ..., 24), n <= max_n <= 128
+// for every descriptor of the launch.
+hipError_t launch_qr_batched_reg(const QrBatchDesc* descs, int count, int rows_per_thread, int max_n,
+                                 hipStream_t stream);
+// LDS doubles one matrix of the batched QR takes: leading dimension m | 1 (odd: conflict-free
+// column-parallel reads) while that fits QR_ODD_LD_MAX entries, else m; times n.
+int qr_batched_lds_entries(int m, int n);
+constexpr int QR_ODD_LD_MAX = (163840 - 2048 - 16 * 8) / 8;  // 160 KiB less the largest kernel's static LDS
 // Upper trapezoids of stacked row blocks into a column-major matrix (TSQR's stack of R factors, or R).
 hipError_t launch_copy_upper_blocks(const double* src, int64_t lds, int64_t stride, int per, int rows_total, int cols,
                                     double* dst, int64_t ldd, hipStream_t stream);

@@ -365,45 +365,49 @@ __global__ __launch_bounds__(256) void sumsq_partial(const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
-// Batched small QR: one workgroup factors one column-major m x n matrix held
-// in LDS (m*n <= QR_BATCH_MAX_MN doubles, n <= 256), unblocked, same reflector convention.
-__global__ __launch_bounds__(256) void qr_batched_small(const QrBatchDesc* __restrict__ descs) {
+// Batched small QR: one wavefront factors one column-major m x n matrix held in LDS (n <= 256), the
+// reference's unblocked Householder sequence (HouseholderQR.computeHouseholder, HouseholderQR.java:
+// 219-291, applied column by column). The leading dimension in LDS is m | 1 when that fits: an odd
+// stride puts the 64 lanes' column-parallel reads (lane = column) in distinct banks. A column step is
+// two shuffle reductions (max, then the scaled sum of squares), the column scaled by 1 / v0, the dot
+// products w_j = v^T a_j with one lane per trailing column (two lanes, halves of the rows, when 32 or
+// fewer columns remain) and the rank-1 update with one lane per row; with one wavefront per workgroup
+// the barriers between them cost nothing but the ordering of LDS accesses.
+__device__ __forceinline__ int qr_wave_ld(int m, int n) {
+    return (int64_t)(m | 1) * n <= QR_ODD_LD_MAX ? (m | 1) : m;
+}
+
+__global__ __launch_bounds__(64) void qr_batched_wave(const QrBatchDesc* __restrict__ descs) {
     extern __shared__ double a_s[];
-    __shared__ double red[4];
-    __shared__ double wcol[256];
+    __shared__ double wbuf[256];
     const QrBatchDesc d = descs[blockIdx.x];
-    const int m = d.m, n = d.n;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int idx = threadIdx.x; idx < m * n; idx += blockDim.x) {
+    const int m = d.m, n = d.n, ld = qr_wave_ld(m, n);
+    const int lane = threadIdx.x;
+    for (int idx = lane; idx < m * n; idx += 64) {
         const int j = idx / m, i = idx % m;
-        a_s[idx] = d.a[(int64_t)j * d.lda + i];
+        a_s[j * ld + i] = d.a[(int64_t)j * d.lda + i];
     }
     __syncthreads();
     const int kmax = m < n ? m : n;
     for (int k = 0; k < kmax; k++) {
-        double* col = a_s + (int64_t)k * m;
+        double* col = a_s + k * ld;
         const double x0 = col[k];
         double mx = 0.0;
-        for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) mx = fmax(mx, fabs(col[i]));
-        mx = block_max(mx, red);
-        const double M = fmax(mx, fabs(x0));
+        for (int i = k + 1 + lane; i < m; i += 64) mx = fmax(mx, fabs(col[i]));
+        const double M = fmax(wave_max(mx), fabs(x0));
         double t = 0.0, beta = x0, v0inv = 0.0;
-        bool negate = false;
         if (k < m - 1 && M >= 2.2250738585072014e-308) {
             double sq = 0.0;
-            for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) {
+            for (int i = k + 1 + lane; i < m; i += 64) {
                 const double y = col[i] / M;
                 sq += y * y;
             }
-            sq = block_sum(sq, red);
-            const double xnorm = sqrt(sq);
+            const double xnorm = sqrt(wave_sum(sq));
             const double x0n = x0 / M;
             if (xnorm == 0.0) {
-                if (x0n < 0.0) {
+                if (x0n < 0.0) {  // H = I - 2 e_k e_k^T negates the column's pivot
                     t = 2.0;
                     beta = -x0;
-                    negate = true;
                 }
             } else {
                 const double nbeta = -copysign(hypot(x0n, xnorm), x0n);
@@ -412,45 +416,311 @@ __global__ __launch_bounds__(256) void qr_batched_small(const QrBatchDesc* __res
                 v0inv = 1.0 / ((x0n - nbeta) * M);
             }
         }
-        (void)negate;
-        __syncthreads();
         if (v0inv != 0.0)
-            for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) col[i] *= v0inv;
+            for (int i = k + 1 + lane; i < m; i += 64) col[i] *= v0inv;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            col[k] = beta;
-            d.tau[k] = t;
-        }
+        if (lane == 0) d.tau[k] = t;
         if (t != 0.0) {
-            // w_j = a(k,j) + sum_{i>k} v_i a(i,j); a(:,j) -= t * v * w_j.
-            // Eight threads per column (32 columns at a time) take interleaved rows
-            // and three lane swaps add their partial sums.
-            const int sub = threadIdx.x & 7, slot = threadIdx.x >> 3;
-            for (int jb = k + 1; jb < n; jb += 32) {
-                const int j = jb + slot;
+            // w_j = a(k, j) + sum_{i > k} v_i a(i, j) for the trailing columns j > k.
+            const int nc = n - k - 1;
+            if (nc <= 32) {
+                const int j = k + 1 + (lane & 31), half = lane >> 5;
                 double s = 0.0;
                 if (j < n) {
-                    const double* cj = a_s + (int64_t)j * m;
-                    for (int i = k + 1 + sub; i < m; i += 8) s += col[i] * cj[i];
+                    const double* cj = a_s + j * ld;
+                    for (int i = k + 1 + half; i < m; i += 2) s += col[i] * cj[i];
                 }
-                s += __shfl_xor(s, 1, 64);
-                s += __shfl_xor(s, 2, 64);
-                s += __shfl_xor(s, 4, 64);
-                if (j < n && sub == 0) wcol[j - k - 1] = s + a_s[(int64_t)j * m + k];
+                s += __shfl_xor(s, 32, 64);
+                if (j < n && half == 0) wbuf[j - k - 1] = s + a_s[j * ld + k];
+            } else {
+                for (int j = k + 1 + lane; j < n; j += 64) {
+                    const double* cj = a_s + j * ld;
+                    double s0 = 0.0, s1 = 0.0;
+                    int i = k + 1;
+                    for (; i + 1 < m; i += 2) {
+                        s0 += col[i] * cj[i];
+                        s1 += col[i + 1] * cj[i + 1];
+                    }
+                    if (i < m) s0 += col[i] * cj[i];
+                    wbuf[j - k - 1] = (s0 + s1) + cj[k];
+                }
             }
             __syncthreads();
-            for (int j = k + 1 + w; j < n; j += 4) {
-                double* cj = a_s + (int64_t)j * m;
-                const double s = t * wcol[j - k - 1];
-                if (lane == 0) cj[k] -= s;
-                for (int i = k + 1 + lane; i < m; i += 64) cj[i] -= s * col[i];
+            // a(i, j) -= t v_i w_j (v_k = 1), one lane per row.
+            for (int i = k + lane; i < m; i += 64) {
+                const double tv = t * (i == k ? 1.0 : col[i]);
+                for (int j = k + 1; j < n; j++) a_s[j * ld + i] -= tv * wbuf[j - k - 1];
             }
         }
         __syncthreads();
+        if (lane == 0) col[k] = beta;
     }
-    for (int idx = threadIdx.x; idx < m * n; idx += blockDim.x) {
+    __syncthreads();
+    for (int idx = lane; idx < m * n; idx += 64) {
         const int j = idx / m, i = idx % m;
-        d.a[(int64_t)j * d.lda + i] = a_s[idx];
+        d.a[(int64_t)j * d.lda + i] = a_s[j * ld + i];
+    }
+}
+
+// (max |x|, sum (x / max)^2) of two parts of a column, merged (a max of 0 is an empty part).
+__device__ __forceinline__ void merge_norm(double& m1, double& s1, double m2, double s2) {
+    if (m2 > m1) {
+        const double r = m1 / m2;
+        s1 = s2 + s1 * r * r;
+        m1 = m2;
+    } else if (m2 > 0.0) {
+        const double r = m2 / m1;
+        s1 += s2 * r * r;
+    }
+}
+
+// One wavefront's (max |x_i|, sum (x_i / max)^2) over rows i in [lo, m) of a column, in every lane
+// (the column is not written while it runs).
+__device__ __forceinline__ void wave_norm(const double* col, int lo, int m, int lane, double& mx, double& sq) {
+    mx = 0.0;
+    sq = 0.0;
+    for (int i = lo + lane; i < m; i += 64) merge_norm(mx, sq, fabs(col[i]), 1.0);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double m2 = __shfl_xor(mx, off, 64), s2 = __shfl_xor(sq, off, 64);
+        merge_norm(mx, sq, m2, s2);
+    }
+}
+
+// Batched small QR, NT / 64 wavefronts per matrix (the larger size classes): one workgroup factors one
+// column-major m x n matrix held in LDS (m*n <= QR_BATCH_MAX_MN doubles, n <= 256, the same odd leading
+// dimension as qr_batched_wave when it fits), the same reflector per column as the reference
+// (max-scaled norm, beta = -sign(x0) ||x||, tau = (beta - x0) / beta, v = x / (x0 - beta)), with two
+// barriers per column: the dot products w_j = a(k,j) + (1 / v0) sum_{i>k} x_i a(i,j) on the unscaled
+// column (eight threads per trailing column, NT / 8 columns at a time), then the rank-1 update (one
+// wavefront per column). Wavefront 0 updates column k + 1 and leaves its norm statistics (one pass,
+// merged by shuffles) for step k + 1, so no step reduces across wavefronts; column k is scaled to v
+// during step k + 1, when nothing reads it.
+template <int NT>
+__global__ __launch_bounds__(NT) void qr_batched_block(const QrBatchDesc* __restrict__ descs) {
+    constexpr int NW = NT / 64, SLOTS = NT / 8;
+    extern __shared__ double a_s[];
+    __shared__ double stat[2];
+    __shared__ double wcol[256];
+    const QrBatchDesc d = descs[blockIdx.x];
+    const int m = d.m, n = d.n, ld = qr_wave_ld(m, n);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int idx = threadIdx.x; idx < m * n; idx += NT) {
+        const int j = idx / m, i = idx % m;
+        a_s[j * ld + i] = d.a[(int64_t)j * d.lda + i];
+    }
+    __syncthreads();
+    if (w == 0) {
+        double mx, sq;
+        wave_norm(a_s, 1, m, lane, mx, sq);
+        if (lane == 0) {
+            stat[0] = mx;
+            stat[1] = sq;
+        }
+    }
+    __syncthreads();
+    const int kmax = m < n ? m : n;
+    double prev_scale = 0.0;  // 1 / v0 of the previous column, applied to it during this step
+    for (int k = 0; k < kmax; k++) {
+        double* col = a_s + k * ld;
+        const double x0 = col[k];
+        const double mx = stat[0], sqm = stat[1];
+        const double M = fmax(mx, fabs(x0));
+        double t = 0.0, beta = x0, v0inv = 0.0;
+        if (k < m - 1 && M >= 2.2250738585072014e-308) {
+            const double r = mx / M;
+            const double xnorm = sqrt(sqm * r * r);
+            const double x0n = x0 / M;
+            if (xnorm == 0.0) {
+                if (x0n < 0.0) {  // H = I - 2 e_k e_k^T negates the column's pivot
+                    t = 2.0;
+                    beta = -x0;
+                }
+            } else {
+                const double nbeta = -copysign(hypot(x0n, xnorm), x0n);
+                beta = nbeta * M;
+                t = (nbeta - x0n) / nbeta;
+                v0inv = 1.0 / ((x0n - nbeta) * M);
+            }
+        }
+        if (k > 0 && prev_scale != 0.0) {  // column k - 1 becomes v (no one reads it in this step)
+            double* pc = a_s + (k - 1) * ld;
+            for (int i = k + threadIdx.x; i < m; i += NT) pc[i] *= prev_scale;
+        }
+        if (threadIdx.x == 0) d.tau[k] = t;
+        if (t != 0.0) {
+            const int sub = threadIdx.x & 7, slot = threadIdx.x >> 3;
+            for (int jb = k + 1; jb < n; jb += SLOTS) {
+                const int j = jb + slot;
+                double s0 = 0.0, s1 = 0.0;
+                if (j < n) {
+                    const double* cj = a_s + j * ld;
+                    int i = k + 1 + sub;
+                    for (; i + 8 < m; i += 16) {
+                        s0 += col[i] * cj[i];
+                        s1 += col[i + 8] * cj[i + 8];
+                    }
+                    if (i < m) s0 += col[i] * cj[i];
+                }
+                double sd = s0 + s1;
+                sd += __shfl_xor(sd, 1, 64);
+                sd += __shfl_xor(sd, 2, 64);
+                sd += __shfl_xor(sd, 4, 64);
+                if (j < n && sub == 0) wcol[j - k - 1] = t * (a_s[j * ld + k] + v0inv * sd);
+            }
+        }
+        __syncthreads();
+        // a(k,j) -= t w_j and a(i,j) -= t w_j x_i / v0 for i > k; wavefront 0 owns column k + 1 and
+        // leaves its statistics over rows k + 2 .. m - 1 for the next step.
+        for (int j = k + 1 + w; j < n; j += NW) {
+            double* cj = a_s + j * ld;
+            const double tw = t != 0.0 ? wcol[j - k - 1] : 0.0, twv = tw * v0inv;
+            if (t != 0.0 && lane == 0) cj[k] -= tw;
+            if (j == k + 1) {
+                // Each lane accumulates the rows it updates itself (rows k + 2 .. m - 1), in registers.
+                double nmx = 0.0, nsq = 0.0;
+                for (int i = k + 1 + lane; i < m; i += 64) {
+                    const double v = t != 0.0 ? cj[i] - twv * col[i] : cj[i];
+                    if (t != 0.0) cj[i] = v;
+                    if (i >= k + 2) merge_norm(nmx, nsq, fabs(v), 1.0);
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double m2 = __shfl_xor(nmx, off, 64), s2 = __shfl_xor(nsq, off, 64);
+                    merge_norm(nmx, nsq, m2, s2);
+                }
+                if (lane == 0) {
+                    stat[0] = nmx;
+                    stat[1] = nsq;
+                }
+            } else if (t != 0.0) {
+                for (int i = k + 1 + lane; i < m; i += 64) cj[i] -= twv * col[i];
+            }
+        }
+        if (threadIdx.x == 0) col[k] = beta;
+        prev_scale = v0inv;
+        __syncthreads();
+    }
+    if (kmax > 0 && prev_scale != 0.0) {
+        double* pc = a_s + (kmax - 1) * ld;
+        for (int i = kmax + threadIdx.x; i < m; i += NT) pc[i] *= prev_scale;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < m * n; idx += NT) {
+        const int j = idx / m, i = idx % m;
+        d.a[(int64_t)j * d.lda + i] = a_s[j * ld + i];
+    }
+}
+
+// Batched small QR with the matrix in registers (m <= 8 R, n <= 128): eight threads own each column
+// (thread t: column t / 8, rows t % 8 + 8 r for r < R), so a column step moves no matrix data through
+// LDS. The reflector of column k is formed by its eight owners (max |x| and the sum of (x / max)^2 by
+// tree reductions and shuffles inside the group, x0 by a shuffle from its owner), which write the
+// whole v (zero above row k, 1 at row k) and tau into a double-buffered LDS vector; after one barrier
+// every thread reads v at its rows unconditionally, forms its column's dot product by three shuffles
+// and updates its column in registers. The owners of column k + 1 then form reflector k + 1 before the
+// next barrier: one barrier per column.
+template <int R>
+__global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __restrict__ descs) {
+    __shared__ double vbuf[2][8 * R];
+    __shared__ double tbuf[2];
+    const QrBatchDesc d = descs[blockIdx.x];
+    const int m = d.m, n = d.n, kmax = m < n ? m : n;
+    const int c = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    const bool own = c < n;
+    double a[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = sub + 8 * r;
+        a[r] = (own && i < m) ? d.a[(int64_t)c * d.lda + i] : 0.0;
+    }
+    // Reflector of this thread's column at row k, formed by the column's eight owners together.
+    auto reflect = [&](int k, int buf) {
+        double mxr[4] = {0.0, 0.0, 0.0, 0.0};
+        double own_x0 = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = sub + 8 * r;
+            if (i == k) own_x0 = a[r];
+            mxr[r & 3] = fmax(mxr[r & 3], i > k ? fabs(a[r]) : 0.0);  // rows >= m hold zeros
+        }
+        double mx = fmax(fmax(mxr[0], mxr[1]), fmax(mxr[2], mxr[3]));
+        mx = fmax(mx, __shfl_xor(mx, 1, 64));
+        mx = fmax(mx, __shfl_xor(mx, 2, 64));
+        mx = fmax(mx, __shfl_xor(mx, 4, 64));
+        const double x0 = __shfl(own_x0, k & 7, 8);  // from the owner of row k, bits as they are
+        const double M = fmax(mx, fabs(x0));
+        double t = 0.0, beta = x0, v0inv = 0.0;
+        if (k < m - 1 && M >= 2.2250738585072014e-308) {
+            // x / M as x * (1 / M) while 1 / M is a normal number (M below 2^1000), else by division.
+            const bool huge = M > 0x1p1000;
+            const double inv = 1.0 / M;
+            double sqr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int i = sub + 8 * r;
+                const double y = i > k ? (huge ? a[r] / M : a[r] * inv) : 0.0;
+                sqr[r & 3] += y * y;
+            }
+            double sq = (sqr[0] + sqr[1]) + (sqr[2] + sqr[3]);
+            sq += __shfl_xor(sq, 1, 64);
+            sq += __shfl_xor(sq, 2, 64);
+            sq += __shfl_xor(sq, 4, 64);
+            const double xnorm = sqrt(sq);
+            const double x0n = huge ? x0 / M : x0 * inv;
+            if (xnorm == 0.0) {
+                if (x0n < 0.0) {  // H = I - 2 e_k e_k^T negates the column's pivot
+                    t = 2.0;
+                    beta = -x0;
+                }
+            } else {
+                const double nbeta = -copysign(hypot(x0n, xnorm), x0n);
+                beta = nbeta * M;
+                t = (nbeta - x0n) / nbeta;
+                v0inv = 1.0 / ((x0n - nbeta) * M);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = sub + 8 * r;
+            if (i == k) {
+                a[r] = beta;
+            } else if (i > k && v0inv != 0.0) {
+                a[r] *= v0inv;  // rows >= m stay zero
+            }
+            vbuf[buf][i] = i == k ? 1.0 : (i > k && t != 0.0 ? a[r] : 0.0);
+        }
+        if (sub == 0) {
+            tbuf[buf] = t;
+            d.tau[k] = t;
+        }
+    };
+    if (c == 0 && kmax > 0) reflect(0, 0);
+    __syncthreads();
+    for (int k = 0; k < kmax; k++) {
+        const int buf = k & 1;
+        const double t = tbuf[buf];
+        if (t != 0.0 && own && c > k) {
+            double dr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < R; r++) dr[r & 3] += vbuf[buf][sub + 8 * r] * a[r];
+            double dot = (dr[0] + dr[1]) + (dr[2] + dr[3]);
+            dot += __shfl_xor(dot, 1, 64);
+            dot += __shfl_xor(dot, 2, 64);
+            dot += __shfl_xor(dot, 4, 64);
+            const double tw = t * dot;
+#pragma unroll
+            for (int r = 0; r < R; r++) a[r] -= tw * vbuf[buf][sub + 8 * r];
+        }
+        if (c == k + 1 && k + 1 < kmax) reflect(k + 1, buf ^ 1);
+        __syncthreads();
+    }
+    if (own) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = sub + 8 * r;
+            if (i < m) d.a[(int64_t)c * d.lda + i] = a[r];
+        }
     }
 }
 
@@ -611,11 +881,52 @@ hipError_t launch_sumsq_partial(const double* x, int64_t ld, int rows, int cols,
     return hipGetLastError();
 }
 
-hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_mn, hipStream_t stream) {
-    const size_t lds = (size_t)max_mn * sizeof(double);
-    (void)hipFuncSetAttribute((const void*)qr_batched_small, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(qr_batched_small, dim3(count), dim3(256), lds, stream, descs);
+int qr_batched_lds_entries(int m, int n) {
+    return ((int64_t)(m | 1) * n <= QR_ODD_LD_MAX ? (m | 1) : m) * n;
+}
+
+template <int NT>
+static hipError_t launch_block(const QrBatchDesc* descs, int count, size_t lds, hipStream_t stream) {
+    (void)hipFuncSetAttribute((const void*)qr_batched_block<NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(qr_batched_block<NT>, dim3(count), dim3(NT), lds, stream, descs);
     return hipGetLastError();
+}
+
+hipError_t launch_qr_batched_reg(const QrBatchDesc* descs, int count, int rows_per_thread, int max_n,
+                                 hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    if (max_n < 1 || max_n > 128) return hipErrorInvalidValue;
+    const int threads = ((max_n * 8 + 63) / 64) * 64;
+    switch (rows_per_thread) {
+        case 4: hipLaunchKernelGGL(qr_batched_reg<4>, dim3(count), dim3(threads), 0, stream, descs); break;
+        case 8: hipLaunchKernelGGL(qr_batched_reg<8>, dim3(count), dim3(threads), 0, stream, descs); break;
+        case 12: hipLaunchKernelGGL(qr_batched_reg<12>, dim3(count), dim3(threads), 0, stream, descs); break;
+        case 16: hipLaunchKernelGGL(qr_batched_reg<16>, dim3(count), dim3(threads), 0, stream, descs); break;
+        case 20: hipLaunchKernelGGL(qr_batched_reg<20>, dim3(count), dim3(threads), 0, stream, descs); break;
+        case 24: hipLaunchKernelGGL(qr_batched_reg<24>, dim3(count), dim3(threads), 0, stream, descs); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_qr_batched_small(const QrBatchDesc* descs, int count, int max_lds, int threads,
+                                   hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    const size_t lds = (size_t)max_lds * sizeof(double);
+    // Only the 256-thread kernel's static LDS (2080 bytes) leaves room past QR_ODD_LD_MAX entries.
+    if (max_lds > QR_ODD_LD_MAX && threads != 256) threads = 256;
+    switch (threads) {
+        case 64:
+            (void)hipFuncSetAttribute((const void*)qr_batched_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds);
+            hipLaunchKernelGGL(qr_batched_wave, dim3(count), dim3(64), lds, stream, descs);
+            return hipGetLastError();
+        case 256: return launch_block<256>(descs, count, lds, stream);
+        case 512: return launch_block<512>(descs, count, lds, stream);
+        case 1024: return launch_block<1024>(descs, count, lds, stream);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace jlc

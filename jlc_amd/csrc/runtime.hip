@@ -768,6 +768,9 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"crt_waves", nullptr, &jlc::TuneTable::crt_waves},
         {"crt_group", nullptr, &jlc::TuneTable::crt_group},
         {"crt_xcd", nullptr, &jlc::TuneTable::crt_xcd},
+        {"qr_wave_max_lds", nullptr, &jlc::TuneTable::qr_wave_max_lds},
+        {"qr_block_threads", nullptr, &jlc::TuneTable::qr_block_threads},
+        {"qr_reg_max_m", nullptr, &jlc::TuneTable::qr_reg_max_m},
     };
     for (const F& f : fields) {
         if (std::strcmp(f.name, key) != 0) continue;
@@ -826,6 +829,9 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "crt_waves %d\n", t.crt_waves);
     std::fprintf(f, "crt_group %d\n", t.crt_group);
     std::fprintf(f, "crt_xcd %d\n", t.crt_xcd);
+    std::fprintf(f, "qr_wave_max_lds %d\n", t.qr_wave_max_lds);
+    std::fprintf(f, "qr_block_threads %d\n", t.qr_block_threads);
+    std::fprintf(f, "qr_reg_max_m %d\n", t.qr_reg_max_m);
     std::fclose(f);
     return JLC_OK;
 }
@@ -1814,26 +1820,35 @@ int jlc_dqr_errors_dev(int64_t m, int64_t n, const double* a_orig, int64_t lda_o
 static_assert(sizeof(jlc_qr_desc) == sizeof(jlc::QrBatchDesc), "QR descriptor layouts must match");
 
 // Matrices are grouped into size classes by the LDS their tile needs, one
-// launch per class, so small matrices run several workgroups per CU instead of
+// launch per class, so small matrices run several wavefronts per CU instead of
 // being held to the occupancy of the largest matrix of the batch.
 struct jlc_qr_batched_plan {
-    static constexpr int NCLASS = 4;
+    static constexpr int NCLASS = 7;
+    static constexpr int NREG = 6;  // register-kernel buckets: m <= 32, 64, ..., 192 (rows per thread 4 .. 24)
+    int reg_off[NREG] = {}, reg_cnt[NREG] = {}, reg_n[NREG] = {};
     int count = 0;
     int max_mn = 0;
     int cls_off[NCLASS] = {}, cls_cnt[NCLASS] = {}, cls_mn[NCLASS] = {};
+    int cls_threads[NCLASS] = {};  // threads per matrix: 64 (one wavefront), 256, 512 or 1024
     jlc::QrBatchDesc* d_descs = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
 namespace {
-// LDS entries (doubles) of each class: 8, 4, 2 and 1 workgroups per CU.
-constexpr int QR_CLASS_MN[jlc_qr_batched_plan::NCLASS] = {2304, 4864, 9984, jlc::QR_BATCH_MAX_MN};
+// LDS entries (doubles) of each class: 16, 8, 5, 4, 3, 2 and 1 wavefronts per CU by LDS.
+constexpr int QR_CLASS_MN[jlc_qr_batched_plan::NCLASS] = {1280, 2560, 4096, 5120, 6826, 10240, jlc::QR_BATCH_MAX_MN};
 
 hipError_t launch_qr_plan(const jlc_qr_batched_plan* p, hipStream_t s) {
+    for (int b = 0; b < jlc_qr_batched_plan::NREG; b++) {
+        if (!p->reg_cnt[b]) continue;
+        hipError_t e = jlc::launch_qr_batched_reg(p->d_descs + p->reg_off[b], p->reg_cnt[b], 4 * (b + 1), p->reg_n[b], s);
+        if (e != hipSuccess) return e;
+    }
     for (int c = 0; c < jlc_qr_batched_plan::NCLASS; c++) {
         if (!p->cls_cnt[c]) continue;
-        hipError_t e = jlc::launch_qr_batched_small(p->d_descs + p->cls_off[c], p->cls_cnt[c], p->cls_mn[c], s);
+        hipError_t e = jlc::launch_qr_batched_small(p->d_descs + p->cls_off[c], p->cls_cnt[c], p->cls_mn[c],
+                                                    p->cls_threads[c], s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1859,15 +1874,32 @@ int jlc_qr_batched_plan_create(int64_t count, const jlc_qr_desc* descs, jlc_qr_b
     jlc_qr_batched_plan* p = new jlc_qr_batched_plan;
     p->count = (int)count;
     p->max_mn = max_mn;
-    // Stable order by size class; each class launches with its own largest tile.
+    // Stable order by size class; each class launches with its own largest tile. Matrices that fit the
+    // register kernel (m <= qr_reg_max_m <= 192, n <= 128) go first, bucketed by rows per thread.
     std::vector<jlc_qr_desc> sorted;
     sorted.reserve((size_t)count);
-    for (int c = 0, off = 0; c < jlc_qr_batched_plan::NCLASS; c++) {
+    const int reg_max_m = std::min(192, jlc::tune_snapshot().qr_reg_max_m);
+    auto reg_bucket = [&](const jlc_qr_desc& d) { return d.m <= reg_max_m && d.n <= 128 ? (d.m - 1) / 32 : -1; };
+    for (int b = 0; b < jlc_qr_batched_plan::NREG; b++) {
+        p->reg_off[b] = (int)sorted.size();
+        int mx = 0;
+        for (int64_t i = 0; i < count; i++)
+            if (reg_bucket(descs[i]) == b) {
+                sorted.push_back(descs[i]);
+                mx = std::max(mx, descs[i].n);
+            }
+        p->reg_cnt[b] = (int)sorted.size() - p->reg_off[b];
+        p->reg_n[b] = mx;
+    }
+    for (int c = 0, off = (int)sorted.size(); c < jlc_qr_batched_plan::NCLASS; c++) {
         const int lo = c ? QR_CLASS_MN[c - 1] : 0, hi = QR_CLASS_MN[c];
+        // Small classes run one wavefront per matrix, the large ones qr_block_threads threads (the
+        // boundary and the width are the tuning table's qr_wave_max_lds and qr_block_threads).
+        const jlc::TuneTable tt = jlc::tune_snapshot();
         int mx = 0;
         for (int64_t i = 0; i < count; i++) {
-            const int mn = descs[i].m * descs[i].n;
-            if (mn > lo && mn <= hi) {
+            const int mn = jlc::qr_batched_lds_entries(descs[i].m, descs[i].n);
+            if (reg_bucket(descs[i]) < 0 && mn > lo && mn <= hi) {
                 sorted.push_back(descs[i]);
                 mx = std::max(mx, mn);
             }
@@ -1875,6 +1907,7 @@ int jlc_qr_batched_plan_create(int64_t count, const jlc_qr_desc* descs, jlc_qr_b
         p->cls_off[c] = off;
         p->cls_cnt[c] = (int)sorted.size() - off;
         p->cls_mn[c] = mx;
+        p->cls_threads[c] = hi <= tt.qr_wave_max_lds ? 64 : tt.qr_block_threads;
         off = (int)sorted.size();
     }
     hipError_t e = hipMalloc(&p->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::QrBatchDesc));

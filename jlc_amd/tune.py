@@ -8,6 +8,11 @@ Tuned entries:
   (which must not regress).
 * grouped_waves_per_simd: width of the persistent grouped launch, timed on a
   batch of uneven problems.
+* qr_wave_max_lds / qr_block_threads: the batched QR's size-class boundary (in LDS doubles per
+  matrix) up to which a matrix is factored by one wavefront (cheap barriers, more matrices per CU),
+  and the threads per matrix above it (256, 512 or 1024: more lanes on each column step); and
+  qr_reg_max_m, the largest m (<= 192, with n <= 128) held in registers instead of LDS; timed on
+  a batch with the config's shape distribution (m uniform on [16, 192], n = m / 2).
 * hip_min_flops: the offload gate of the host-side dispatch (the HIP analogue
   of GemmDispatch.CUDA_THRESHOLD, src/main/java/net/faulj/compute/GemmDispatch.java:32).
   The end-to-end host-pointer GEMM (staging included) is timed per size and
@@ -88,6 +93,37 @@ def tune_grouped(quick: bool) -> dict:
     return {"chosen": {"grouped_waves_per_simd": best}, "ms": {str(k): round(v, 4) for k, v in res.items()}}
 
 
+def tune_qr_batched(quick: bool) -> dict:
+    rng = np.random.default_rng(5)
+    count = 5000 if quick else 20000
+    ms_ = rng.integers(16, 193, count)
+    mats = [rng.standard_normal(int(m) * max(1, int(m) // 2)) for m in ms_]
+    flat = np.concatenate(mats)
+    offs = np.concatenate([[0], np.cumsum([x.size for x in mats])[:-1]])
+    toffs = np.concatenate([[0], np.cumsum([max(1, int(m) // 2) for m in ms_])[:-1]])
+    da = native.DeviceArray.from_numpy(flat)
+    dt = native.DeviceArray(int(toffs[-1] + max(1, int(ms_[-1]) // 2)))
+    descs = [native.QrDesc(int(m), max(1, int(m) // 2), int(m), da.at(int(o)), dt.at(int(t)))
+             for m, o, t in zip(ms_, offs, toffs)]
+    res = {}
+    for reg in (0, 32, 64, 96, 128, 160, 192):
+        for threads in (512, 1024):
+            for bound in (0, 2560, 4096, 5120):
+                native.tune_set("qr_reg_max_m", reg)
+                native.tune_set("qr_wave_max_lds", bound)
+                native.tune_set("qr_block_threads", threads)
+                plan = native.QrBatchedPlan(descs)
+                res[(reg, bound, threads)] = _time_dev(lambda: plan.replay(native.stream()), 3)
+                plan.destroy()
+                da.copy_from(flat)
+    best = min(res, key=res.get)
+    native.tune_set("qr_reg_max_m", best[0])
+    native.tune_set("qr_wave_max_lds", best[1])
+    native.tune_set("qr_block_threads", best[2])
+    return {"chosen": {"qr_reg_max_m": best[0], "qr_wave_max_lds": best[1], "qr_block_threads": best[2]},
+            "count": count, "ms": {f"{r},{b},{t}": round(v, 3) for (r, b, t), v in res.items()}}
+
+
 def tune_offload(cpu_gflops: float, quick: bool) -> dict:
     sizes = [32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024]
     if quick:
@@ -121,15 +157,23 @@ def main(argv=None) -> int:
     ap.add_argument("--out", default=str(native.TUNE_TABLE))
     ap.add_argument("--cpu-gflops", type=float, default=100.0)
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of splitk,grouped,qr,offload (default all); the "
+                                               "other entries keep the loaded table's values")
     args = ap.parse_args(argv)
+    only = set(args.only.split(",")) if args.only else {"splitk", "grouped", "qr", "offload"}
     rc, count, name = native.probe()
     if rc != native.JLC_OK:
         print(f"no usable device: {native.last_error()}", file=sys.stderr)
         return 1
     report = {"device": name}
-    report["splitk"] = tune_splitk(args.quick)
-    report["grouped"] = tune_grouped(args.quick)
-    report["offload"] = tune_offload(args.cpu_gflops, args.quick)
+    if "splitk" in only:
+        report["splitk"] = tune_splitk(args.quick)
+    if "grouped" in only:
+        report["grouped"] = tune_grouped(args.quick)
+    if "qr" in only:
+        report["qr_batched"] = tune_qr_batched(args.quick)
+    if "offload" in only:
+        report["offload"] = tune_offload(args.cpu_gflops, args.quick)
     out = Path(args.out)
     out.parent.mkdir(parents=True, exist_ok=True)
     native.tune_write(str(out))
