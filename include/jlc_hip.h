@@ -256,7 +256,9 @@ int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream
  * hip_min_flops (offload gate, the HIP analogue of GemmDispatch.CUDA_THRESHOLD,
  * src/main/java/net/faulj/compute/GemmDispatch.java:32), splitk_tiles_per_cu,
  * splitk_min_kslice, grouped_waves_per_simd, grouped_xcds (tile lists dealt per XCD;
- * 1 = plain order), crt_bits_offset, crt_bits_slope, crt_waves, crt_group, crt_xcd,
+ * 1 = plain order), grouped_small_tile (grouped FP64 problems with m or n up to this run 16-row or
+ * 16-column tiles; 0 = always 32 x 32), grouped_chunk_tiles (tiles of one problem dealt to one XCD
+ * together), crt_bits_offset, crt_bits_slope, crt_waves, crt_group, crt_xcd,
  * qr_wave_max_lds (batched QR size classes up to this many LDS doubles run one wavefront per matrix).
  * Text file, one "key value" per line.
  * On first use the library loads $JLC_TUNE_TABLE, or else the table shipped

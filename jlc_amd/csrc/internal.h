@@ -39,6 +39,8 @@ struct TuneTable {
     int crt_waves = 8;                // CRT product kernel: 4 (128 x 128 per wavefront) or 8 (128 x 64)
     int crt_group = 16;               // CRT product kernel: tile rows per column strip of the tile order
     int crt_xcd = 0;                  // CRT product kernel: 1 = keep neighbouring tiles on one XCD
+    int grouped_small_tile = 16;      // grouped FP64: 16-row / 16-column tiles for m / n up to this (0: off)
+    int grouped_chunk_tiles = 64;     // grouped: tiles of one problem dealt to one XCD together (whole tile rows)
     int qr_wave_max_lds = 2560;       // batched QR: size classes up to this many LDS doubles run one
                                       // wavefront per matrix, larger ones qr_block_threads threads
     int qr_block_threads = 256;       // batched QR: threads per matrix above qr_wave_max_lds (256/512/1024)
@@ -119,7 +121,9 @@ struct GroupedDesc {
 int grouped_grid(int device_cus, int waves_per_simd);
 constexpr int GROUPED_WAVES_PER_GROUP = 4;  // wavefronts per grouped workgroup
 constexpr uint64_t GROUPED_SKIP = ~0ull;     // tile-list entry that names no tile
-// tiles: (problem << 32) | (tile_row << 16) | tile_col, 32 x 32 tiles; wavefront w of the
+constexpr int GROUPED_SHAPE_SHIFT = 30;
+// tiles: (problem << 32) | (shape << 30) | (tile_row << 15) | tile_col, where shape bit 1 selects 16-row
+// and bit 0 16-column tiles (else 32; the split form always uses 32 x 32); wavefront w of the
 // grid takes entries w, w + W, w + 2W, ... (W = grid * GROUPED_WAVES_PER_GROUP) and skips
 // GROUPED_SKIP entries.
 hipError_t launch_grouped_f64(const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,

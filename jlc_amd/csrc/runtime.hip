@@ -763,6 +763,8 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"splitk_min_kslice", nullptr, &jlc::TuneTable::splitk_min_kslice},
         {"grouped_waves_per_simd", nullptr, &jlc::TuneTable::grouped_waves_per_simd},
         {"grouped_xcds", nullptr, &jlc::TuneTable::grouped_xcds},
+        {"grouped_small_tile", nullptr, &jlc::TuneTable::grouped_small_tile},
+        {"grouped_chunk_tiles", nullptr, &jlc::TuneTable::grouped_chunk_tiles},
         {"crt_bits_offset", &jlc::TuneTable::crt_bits_offset, nullptr},
         {"crt_bits_slope", &jlc::TuneTable::crt_bits_slope, nullptr},
         {"crt_waves", nullptr, &jlc::TuneTable::crt_waves},
@@ -824,6 +826,8 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "splitk_min_kslice %d\n", t.splitk_min_kslice);
     std::fprintf(f, "grouped_waves_per_simd %d\n", t.grouped_waves_per_simd);
     std::fprintf(f, "grouped_xcds %d\n", t.grouped_xcds);
+    std::fprintf(f, "grouped_small_tile %d\n", t.grouped_small_tile);
+    std::fprintf(f, "grouped_chunk_tiles %d\n", t.grouped_chunk_tiles);
     std::fprintf(f, "crt_bits_offset %.17g\n", t.crt_bits_offset);
     std::fprintf(f, "crt_bits_slope %.17g\n", t.crt_bits_slope);
     std::fprintf(f, "crt_waves %d\n", t.crt_waves);
@@ -1300,7 +1304,8 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     } else {
         return fail(JLC_ERR_UNSUPPORTED, "grouped launches serve JLC_MODE_FP64 and JLC_MODE_SPLIT_INT8");
     }
-    // Problems longest K first (stable); each cut into chunks of whole tile rows of at most 16 tiles.
+    // Problems longest K first (stable); each cut into chunks of whole tile rows of at most
+    // grouped_chunk_tiles tiles (at least one tile row).
     for (int64_t p = 0; p < count; p++) {
         const jlc_gemm_desc& d = descs[p];
         int rc = validate_gemm(d.layout_a, d.layout_b, d.m, d.n, d.k, d.alpha, d.a, d.lda, d.b, d.ldb, d.c, d.ldc,
@@ -1309,7 +1314,7 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
             std::string msg = t_error;
             return fail(rc, "descriptor %lld: %s", (long long)p, msg.c_str());
         }
-        if (d.m > 32 * 65535 || d.n > 32 * 65535) return fail(JLC_ERR_INVALID_ARG, "descriptor %lld too large", (long long)p);
+        if (d.m > 32 * 32767 || d.n > 32 * 32767) return fail(JLC_ERR_INVALID_ARG, "descriptor %lld too large", (long long)p);
     }
     std::vector<int64_t> probs((size_t)count);
     for (int64_t p = 0; p < count; p++) probs[(size_t)p] = p;
@@ -1328,18 +1333,27 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     int X = std::max(1, jlc::tune_snapshot().grouped_xcds);
     if (grid % X != 0) X = 1;
     const int64_t Wx = W / X;
+    // The FP64 form cuts a problem with m (n) up to grouped_small_tile into 16-row (16-column) tiles, so a
+    // small problem does not pad its MFMA work to 32; the split form keeps 32 x 32 tiles.
+    const int small = form == JLC_MODE_FP64 ? jlc::tune_snapshot().grouped_small_tile : 0;
+    const int chunk = std::max(1, jlc::tune_snapshot().grouped_chunk_tiles);
     std::vector<std::vector<uint64_t>> seq((size_t)X);
     std::vector<double> work((size_t)X, 0.0);
     for (int64_t p : probs) {
         const jlc_gemm_desc& d = descs[p];
-        const int tm = (int)((d.m + 31) / 32), tn = (int)((d.n + 31) / 32);
-        const int rows = std::max(1, 16 / std::max(1, tn));
+        const bool rows16 = d.m <= small, cols16 = d.n <= small;
+        const int TM = rows16 ? 16 : 32, TN = cols16 ? 16 : 32;
+        const uint64_t shape = (uint64_t)((rows16 ? 2 : 0) | (cols16 ? 1 : 0)) << jlc::GROUPED_SHAPE_SHIFT;
+        const int tm = (int)((d.m + TM - 1) / TM), tn = (int)((d.n + TN - 1) / TN);
+        const int rows = std::max(1, chunk / std::max(1, tn));
+        const double blocks = (double)(TM / 16) * (TN / 16);  // 16 x 16 MFMA blocks per tile
         for (int r0 = 0; r0 < tm; r0 += rows) {
             const int r1 = std::min(tm, r0 + rows);
             const size_t x = (size_t)(std::min_element(work.begin(), work.end()) - work.begin());
-            work[x] += (double)(r1 - r0) * tn * std::max<int64_t>(d.k, 16);
+            work[x] += (double)(r1 - r0) * tn * blocks * std::max<int64_t>(d.k, 16);
             for (int i = r0; i < r1; i++)
-                for (int j = 0; j < tn; j++) seq[x].push_back(((uint64_t)p << 32) | ((uint64_t)i << 16) | (uint64_t)j);
+                for (int j = 0; j < tn; j++)
+                    seq[x].push_back(((uint64_t)p << 32) | shape | ((uint64_t)i << 15) | (uint64_t)j);
         }
     }
     size_t rounds = 0;

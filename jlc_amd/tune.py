@@ -6,8 +6,9 @@ Tuned entries:
   a long K (the Gram matrix and W = C^T V of blocked QR, tall-skinny products),
   K is cut into slices; candidates are timed on such shapes and on square ones
   (which must not regress).
-* grouped_waves_per_simd: width of the persistent grouped launch, timed on a
-  batch of uneven problems.
+* grouped_waves_per_simd: width of the persistent grouped launch; grouped_small_tile, the m (n)
+  up to which a problem runs 16-row (16-column) MFMA tiles instead of 32; grouped_chunk_tiles,
+  how many tiles of one problem go to one XCD together; all timed on a batch of uneven problems.
 * qr_wave_max_lds / qr_block_threads: the batched QR's size-class boundary (in LDS doubles per
   matrix) up to which a matrix is factored by one wavefront (cheap barriers, more matrices per CU),
   and the threads per matrix above it (256, 512 or 1024: more lanes on each column step); and
@@ -82,15 +83,38 @@ def tune_grouped(quick: bool) -> dict:
     dims, alpha, beta = batched_shapes(5000 if quick else 20000, seed=3)
     w = BatchedWorkload(dims, alpha, beta, seed=4)
     descs = w.descs()
+
+    def timed() -> float:
+        plan = native.GroupedPlan(descs)
+        ms = _time_dev(lambda: plan.run(native.stream()), 5)
+        plan.destroy()
+        return ms
+
     res = {}
     for wps in (0, 1, 2, 3):
         native.tune_set("grouped_waves_per_simd", wps)
-        plan = native.GroupedPlan(descs)
-        res[wps] = _time_dev(lambda: plan.run(native.stream()), 5)
-        plan.destroy()
+        res[wps] = timed()
     best = min(res, key=res.get)
     native.tune_set("grouped_waves_per_simd", best)
-    return {"chosen": {"grouped_waves_per_simd": best}, "ms": {str(k): round(v, 4) for k, v in res.items()}}
+    # Tile shape per size class: problems with m (n) up to the bound run 16-row (16-column) tiles.
+    shapes = {}
+    for small in (0, 16, 48, 80):
+        native.tune_set("grouped_small_tile", small)
+        shapes[small] = timed()
+    best_small = min(shapes, key=shapes.get)
+    native.tune_set("grouped_small_tile", best_small)
+    # Tiles of one problem dealt to one XCD together.
+    chunks = {}
+    for chunk in (8, 16, 64):
+        native.tune_set("grouped_chunk_tiles", chunk)
+        chunks[chunk] = timed()
+    best_chunk = min(chunks, key=chunks.get)
+    native.tune_set("grouped_chunk_tiles", best_chunk)
+    return {"chosen": {"grouped_waves_per_simd": best, "grouped_small_tile": best_small,
+                       "grouped_chunk_tiles": best_chunk},
+            "ms": {str(k): round(v, 4) for k, v in res.items()},
+            "small_tile_ms": {str(k): round(v, 4) for k, v in shapes.items()},
+            "chunk_ms": {str(k): round(v, 4) for k, v in chunks.items()}}
 
 
 def tune_qr_batched(quick: bool) -> dict:

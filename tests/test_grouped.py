@@ -131,6 +131,30 @@ def test_tile_placement_does_not_change_results(xcds, wps):
         _check_problem(w, i, ref, "placement")
 
 
+@pytest.mark.parametrize("small", [0, 8, 16, 300])
+def test_tile_shapes_do_not_change_results(small):
+    # Problems with m (n) up to grouped_small_tile run 16-row (16-column) tiles; every 16 x 16 block of C
+    # is the same MFMA chain over k in either tile shape, so the bits match the all-32 x 32 run, and the
+    # mixed-shape lists (0 / 16-only / all shapes) are checked against the oracle too.
+    dims, alpha, beta = batched_shapes(900, seed=19, lo=1, hi=70)
+    rng = np.random.default_rng(20)
+    layouts = rng.integers(0, 2, size=(len(dims), 2))
+    old = native.tune_get("grouped_small_tile")
+    try:
+        native.tune_set("grouped_small_tile", 0)
+        ref_w = BatchedWorkload(dims, alpha, beta, seed=21, layouts=layouts)
+        native.dgemm_grouped_dev(ref_w.descs())
+        ref = ref_w.dc.to_numpy()
+        native.tune_set("grouped_small_tile", small)
+        w = BatchedWorkload(dims, alpha, beta, seed=21, layouts=layouts)
+        native.dgemm_grouped_dev(w.descs())
+        assert np.array_equal(w.dc.to_numpy(), ref)
+    finally:
+        native.tune_set("grouped_small_tile", old)
+    for i in range(0, w.count, 7):
+        _check_problem(w, i, ref, f"small tiles {small}")
+
+
 def _check_split_problem(w, i, c_now, slices, tag):
     m, n, k = (int(x) for x in w.dims[i])
     la, lb = (int(x) for x in w.layouts[i])
