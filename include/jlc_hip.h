@@ -190,11 +190,14 @@ typedef struct jlc_grouped_plan jlc_grouped_plan;
  * problem's tiles dealt to one XCD) and uploads both to the device. mode is
  * JLC_MODE_FP64, or JLC_MODE_SPLIT_INT8 with an explicit slice count in [3, 9]
  * (JLC_MODE_SLICES): every problem then computed exactly as jlc_dgemm's INT8
- * slice form computes it with that count (no-counterpart extension: the
+ * slice form computes it with that count, its operands sliced once per run into
+ * a plan-owned workspace by a pack launch ahead of the product launch (S KiB per
+ * 32 x 32 block of op(A) and op(B); no-counterpart extension: the
  * reference has no batched GEMM entry point; its TINY / SMALL kernels,
  * SpecializedKernels.java:79-158, serve one problem per call). */
 int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode, jlc_grouped_plan** plan);
-/* One persistent launch over every tile of the plan. */
+/* One persistent launch over every tile of the plan (the slice form: the pack
+ * launch, then the product launch). */
 int jlc_grouped_plan_run(jlc_grouped_plan* plan, void* stream, jlc_stats* stats);
 /* Captures the launch into a HIP graph (once) and replays it on stream. */
 int jlc_grouped_plan_replay(jlc_grouped_plan* plan, void* stream, jlc_stats* stats);

@@ -234,92 +234,318 @@ __global__ __launch_bounds__(64 * WAVES_PER_GROUP) __attribute__((amdgpu_waves_p
 // ---------------------------------------------------------------------------
 // Split-INT8 form of the grouped GEMM: every problem computed as the
 // single-GEMM INT8 slice form computes it (split_int8.hip; the same exponents,
-// digits, diagonal sums, K chunks, recombination and alpha / beta steps), one
-// wavefront per 32 x 32 tile and no workspace, so a plan stays one graph node.
-//  Pass 1 reads the tile's 32 rows of op(A) and 32 columns of op(B) over all K
-//  for their exponents (frexp of max |x|; Inf / NaN marks the row or column).
-//  Pass 2 slices each 32-deep block on the fly: lane l holds row / column l%32
-//  and k in [16 (l/32), +16) of the block, the MFMA fragment order of
-//  v_mfma_i32_32x32x32_i8, each value cut into S signed 7-bit digits packed 16
-//  to four registers; the S(S+1)/2 slice products accumulate per diagonal.
+// digits, diagonal sums, K chunks, recombination and alpha / beta steps), in
+// two launches over a plan-owned workspace.
+//  Pack, in two launches: one wavefront per 32-row block of op(A) or 32-column
+//  block of op(B) of one problem (a job of the plan's job list) reads the block
+//  over all K for the row / column exponents (frexp of max |x|; Inf / NaN marks
+//  the row or column); then one wavefront per (job, 32-deep k block) slices the
+//  block: lane l holds row / column l%32 and k in [16 (l/32), +16), the MFMA
+//  fragment order of v_mfma_i32_32x32x32_i8, each value cut into S signed
+//  7-bit digits packed 16 to four registers, and the S fragments (1 KiB each)
+//  of the block are stored side by side. Rows / columns past m / n are zeros.
+//  Product: one wavefront per 32 x 32 tile (the persistent tile list); per k
+//  block it loads the S fragments of its A and B blocks (the next block's
+//  while the current one is multiplied) and accumulates the S(S+1)/2 slice
+//  products per diagonal. Each element is sliced once, not once per tile.
 constexpr int SPLIT_NONFINITE = -100000;
 
-template <int S>
-__device__ __forceinline__ void slice16(gcptr p, int64_t ks, int k0, int K, int e, v4i (&out)[S]) {
-    unsigned q[S][4];
+// Lane's 16 values of one 32-deep k block: k in [k0, k0 + 16) of its row / column (clamped to K - 1;
+// the slicing masks k >= K).
+__device__ __forceinline__ void load16(gcptr p, int64_t ks, int k0, int K, double (&x)[16]) {
 #pragma unroll
-    for (int t = 0; t < S; t++) q[t][0] = q[t][1] = q[t][2] = q[t][3] = 0u;
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int k = k0 + j;
-        const double x = p[(int64_t)min(k, K - 1) * ks];
-        double v = (k < K && e != SPLIT_NONFINITE) ? ldexp(x, -e) : 0.0;
-#pragma unroll
-        for (int t = 0; t < S; t++) {
-            v *= 128.0;
-            const double dg = trunc(v);
-            v -= dg;
-            q[t][j >> 2] |= ((unsigned)(int)dg & 255u) << (8 * (j & 3));
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < S; t++) out[t] = v4i{(int)q[t][0], (int)q[t][1], (int)q[t][2], (int)q[t][3]};
+    for (int j = 0; j < 16; j++) x[j] = p[(int64_t)min(k0 + j, K - 1) * ks];
 }
 
-// Frexp exponent of max |x| over the lane's 16-value share of every 32-deep block, combined with the
-// other half-wave's share: the exponent of the whole row / column, or SPLIT_NONFINITE.
-__device__ __forceinline__ int split_exponent(gcptr p, int64_t ks, int K, int h) {
-    double mx = 0.0;
-    int bad = 0;
-    for (int k0 = 16 * h; k0 < K; k0 += 32) {
+// S signed 7-bit digits of each of the 16 values scaled by 2^-e, 16 to four registers per slice:
+// v <- v * 128, digit = trunc(v), v <- v - digit, S times, as split_int8.hip's pack does. Digit t is
+// floor(|v| 2^(7t+7)) mod 128 with the sign of v, taken here from the 28-bit integers
+// trunc(|v| 2^28), trunc(frac(|v| 2^28) 2^28), ... (every step exact in FP64).
+template <int S>
+__device__ __forceinline__ void slice16(const double (&x)[16], int k0, int K, int e, bool live, v4i (&out)[S]) {
+    constexpr int NCH = (S + 3) / 4;  // 28-bit chunks of four digits
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
+    for (int w = 0; w < 4; w++) {  // four values per register
+        int dg[S][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int j = 4 * w + i;
             const int k = k0 + j;
-            const double v = fabs(p[(int64_t)min(k, K - 1) * ks]);
-            if (k < K) {
-                bad |= !(v <= 1.7976931348623157e308);
-                mx = fmax(mx, v);
+            const double v = (live && k < K && e != SPLIT_NONFINITE) ? ldexp(x[j], -e) : 0.0;
+            const int neg = v < 0.0 ? -1 : 0;
+            double y = fabs(v);
+            int ch[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                y *= 268435456.0;  // 2^28
+                const double hc = trunc(y);
+                y -= hc;
+                ch[c] = (int)hc;
+            }
+#pragma unroll
+            for (int t = 0; t < S; t++) {
+                const int d = (int)__builtin_amdgcn_ubfe((unsigned)ch[t >> 2], 21 - 7 * (t & 3), 7);
+                dg[t][i] = (d ^ neg) - neg;
             }
         }
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            const unsigned lo = __builtin_amdgcn_perm((unsigned)dg[t][1], (unsigned)dg[t][0], 0x0c0c0400u);
+            const unsigned hi = __builtin_amdgcn_perm((unsigned)dg[t][3], (unsigned)dg[t][2], 0x0c0c0400u);
+            out[t][w] = (int)__builtin_amdgcn_perm(hi, lo, 0x05040100u);
+        }
     }
-    mx = fmax(mx, __shfl_xor(mx, 32, 64));
-    bad |= __shfl_xor(bad, 32, 64);
-    int e = 0;
-    if (mx > 0.0) frexp(mx, &e);
-    return bad ? SPLIT_NONFINITE : e;
+}
+
+__device__ __forceinline__ void stat16(const double (&x)[16], int k0, int K, double& mx, int& bad) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const double v = fabs(x[j]);
+        if (k0 + j < K) {
+            bad |= !(v <= 1.7976931348623157e308);
+            mx = fmax(mx, v);
+        }
+    }
+}
+
+// Row r of op(A) (is_b false) or column r of op(B) of a problem as a pointer and k stride; r is
+// clamped to the last real one.
+__device__ __forceinline__ void operand_line(const GroupedDesc& d, bool is_b, int r, gcptr& ptr, int64_t& ks) {
+    if (!is_b) {
+        const bool row = d.layout_a == JLC_LAYOUT_ROW;
+        ptr = (gcptr)d.a + (row ? (int64_t)r * d.lda : (int64_t)r);
+        ks = row ? 1 : d.lda;
+    } else {
+        const bool row = d.layout_b == JLC_LAYOUT_ROW;
+        ptr = (gcptr)d.b + (row ? (int64_t)r : (int64_t)r * d.ldb);
+        ks = row ? d.ldb : 1;
+    }
+}
+
+// Pack, launch 1: one wavefront per job (a 32-row block of op(A) or 32-column block of op(B)) writes
+// the exponents of its 32 rows / columns. Two register sets in turn keep the next k block's loads in
+// flight while the current one is scanned (every load unconditional, the block index clamped).
+__global__ __launch_bounds__(64 * WAVES_PER_GROUP) void grouped_split_expo(const GroupedDesc* __restrict__ descs,
+                                                                           const GroupedPack* __restrict__ packs,
+                                                                           const uint64_t* __restrict__ jobs,
+                                                                           int64_t njobs, int* __restrict__ expo) {
+    const int lane = threadIdx.x & 63;
+    const int64_t job = (int64_t)blockIdx.x * WAVES_PER_GROUP + (threadIdx.x >> 6);
+    if (job >= njobs) return;
+    const uint64_t e = jobs[job];
+    const int p = __builtin_amdgcn_readfirstlane((int)(e >> 32));
+    const bool is_b = (e >> 31) & 1u;
+    const int blk = (int)(e & 0x7fffffffu);
+    const GroupedDesc d = descs[p];
+    const int K = d.k, kblocks = (K + 31) / 32;
+    const int r32 = lane & 31, h = lane >> 5;
+    const int R = is_b ? d.n : d.m;
+    const int r = blk * 32 + r32;
+    gcptr ptr;
+    int64_t ks;
+    operand_line(d, is_b, min(r, R - 1), ptr, ks);
+    double mx = 0.0;
+    int bad = 0;
+    if (ks == 1) {
+        // k-contiguous lines: load j of a k block covers lines 2j and 2j + 1 of the block (lane l: line
+        // 2j + l / 32, k = l % 32), two 256-byte runs, two blocks in flight; each lane keeps a maximum
+        // per line (Inf / NaN count as +Inf), then a butterfly over the 32 lanes of each half leaves
+        // lane l with line 2 (8 b4 + 4 b3 + 2 b2 + b1) + l / 32 (b: the bits of l % 32) reduced.
+        const int kl = lane & 31;
+        const gcptr base = (gcptr)(is_b ? d.b : d.a);
+        const int64_t ld = is_b ? d.ldb : d.lda;  // line stride of a k-contiguous operand
+        double m16[16], v0[16], v1[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) m16[j] = 0.0;
+        auto fetch = [&](double (&v)[16], int k0) {
+            const int k = min(k0 + kl, K - 1);
+#pragma unroll
+            for (int j = 0; j < 16; j++) v[j] = base[(int64_t)min(blk * 32 + 2 * j + h, R - 1) * ld + k];
+        };
+        auto scan = [&](const double (&v)[16], int k0) {
+            const bool in = k0 + kl < K;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                double a = fabs(v[j]);
+                a = a <= 1.7976931348623157e308 ? a : __builtin_inf();
+                m16[j] = in ? fmax(m16[j], a) : m16[j];
+            }
+        };
+        fetch(v0, 0);
+        for (int kb = 0; kb < kblocks; kb += 2) {
+            fetch(v1, 32 * min(kb + 1, kblocks - 1));
+            scan(v0, 32 * kb);
+            if (kb + 1 >= kblocks) break;
+            fetch(v0, 32 * min(kb + 2, kblocks - 1));
+            scan(v1, 32 * (kb + 1));
+        }
+        double m8[8], m4[4], m2[2];
+        const bool b4 = kl & 16, b3 = kl & 8, b2 = kl & 4, b1 = kl & 2;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const double send = b4 ? m16[i] : m16[i + 8];
+            m8[i] = fmax(b4 ? m16[i + 8] : m16[i], __shfl_xor(send, 16, 64));
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double send = b3 ? m8[i] : m8[i + 4];
+            m4[i] = fmax(b3 ? m8[i + 4] : m8[i], __shfl_xor(send, 8, 64));
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const double send = b2 ? m4[i] : m4[i + 2];
+            m2[i] = fmax(b2 ? m4[i + 2] : m4[i], __shfl_xor(send, 4, 64));
+        }
+        double m1 = fmax(b1 ? m2[1] : m2[0], __shfl_xor(b1 ? m2[0] : m2[1], 2, 64));
+        m1 = fmax(m1, __shfl_xor(m1, 1, 64));
+        const int line = 2 * ((b4 ? 8 : 0) + (b3 ? 4 : 0) + (b2 ? 2 : 0) + (b1 ? 1 : 0)) + h;
+        int ex = 0;
+        if (m1 > 0.0) frexp(m1, &ex);
+        if (!(m1 <= 1.7976931348623157e308)) ex = SPLIT_NONFINITE;
+        if (blk * 32 + line >= R) ex = 0;
+        const GroupedPack pk = packs[p];
+        if ((kl & 1) == 0) expo[(is_b ? pk.eb_off : pk.ea_off) + blk * 32 + line] = ex;
+        return;
+    } else {
+        double x0[16], x1[16];
+        load16(ptr, ks, 16 * h, K, x0);
+        for (int kb = 0; kb < kblocks; kb += 2) {
+            load16(ptr, ks, 32 * min(kb + 1, kblocks - 1) + 16 * h, K, x1);
+            stat16(x0, 32 * kb + 16 * h, K, mx, bad);
+            if (kb + 1 >= kblocks) break;
+            load16(ptr, ks, 32 * min(kb + 2, kblocks - 1) + 16 * h, K, x0);
+            stat16(x1, 32 * (kb + 1) + 16 * h, K, mx, bad);
+        }
+        mx = fmax(mx, __shfl_xor(mx, 32, 64));
+        bad |= __shfl_xor(bad, 32, 64);
+    }
+    int ex = 0;
+    if (mx > 0.0) frexp(mx, &ex);
+    if (bad) ex = SPLIT_NONFINITE;
+    if (r >= R) ex = 0;
+    const GroupedPack pk = packs[p];
+    if (h == 0) expo[(is_b ? pk.eb_off : pk.ea_off) + r] = ex;
+}
+
+// Pack, launch 2: one wavefront per (job, 32-deep k block), listed as (job << 24) | k block, slices
+// the block with the exponents launch 1 wrote and stores its S fragments side by side. Rows /
+// columns past m / n are stored as zeros.
+template <int S>
+__global__ __launch_bounds__(64 * WAVES_PER_GROUP) void grouped_split_slice(const GroupedDesc* __restrict__ descs,
+                                                                            const GroupedPack* __restrict__ packs,
+                                                                            const uint64_t* __restrict__ jobs,
+                                                                            const uint64_t* __restrict__ sjobs,
+                                                                            int64_t nsjobs, v4i* __restrict__ frag,
+                                                                            const int* __restrict__ expo) {
+    const int lane = threadIdx.x & 63;
+    const int64_t sj = (int64_t)blockIdx.x * WAVES_PER_GROUP + (threadIdx.x >> 6);
+    if (sj >= nsjobs) return;
+    const uint64_t se = sjobs[sj];
+    const int kb = (int)(se & 0xffffffu);
+    const uint64_t e = jobs[se >> 24];
+    const int p = __builtin_amdgcn_readfirstlane((int)(e >> 32));
+    const bool is_b = (e >> 31) & 1u;
+    const int blk = (int)(e & 0x7fffffffu);
+    const GroupedDesc d = descs[p];
+    const GroupedPack pk = packs[p];
+    const int K = d.k, kblocks = (K + 31) / 32;
+    const int r32 = lane & 31, h = lane >> 5;
+    const int R = is_b ? d.n : d.m;
+    const int r = blk * 32 + r32;
+    gcptr ptr;
+    int64_t ks;
+    operand_line(d, is_b, min(r, R - 1), ptr, ks);
+    double x[16];
+    if (ks == 1) {
+        // k-contiguous lines: load j covers lines 2j and 2j + 1 of the block as two 256-byte runs
+        // (lane l: line 2j + l / 32, k = l % 32); the block is turned to fragment order through LDS.
+        __shared__ double tile[WAVES_PER_GROUP][32][33];
+        double (*tw)[33] = tile[threadIdx.x >> 6];
+        const int kl = lane & 31;
+        const int k = min(32 * kb + kl, K - 1);
+        const gcptr base = (gcptr)(is_b ? d.b : d.a);
+        const int64_t ld = is_b ? d.ldb : d.lda;  // line stride of a k-contiguous operand
+        double v[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = base[(int64_t)min(blk * 32 + 2 * j + h, R - 1) * ld + k];
+#pragma unroll
+        for (int j = 0; j < 16; j++) tw[2 * j + h][kl] = v[j];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int i = 0; i < 16; i++) x[i] = tw[r32][16 * h + i];
+    } else {
+        load16(ptr, ks, 32 * kb + 16 * h, K, x);
+    }
+    const int ex = expo[(is_b ? pk.eb_off : pk.ea_off) + r];
+    v4i f[S];
+    slice16<S>(x, 32 * kb + 16 * h, K, ex, r < R, f);
+    v4i* out = frag + ((is_b ? pk.b_off : pk.a_off) + ((int64_t)blk * kblocks + kb) * S) * 64 + lane;
+#pragma unroll
+    for (int t = 0; t < S; t++) out[t * 64] = f[t];
 }
 
 template <int S>
-__device__ __forceinline__ void split_tile(const GroupedDesc& d, int i0, int j0, int lane) {
-    const int K = d.k;
+__device__ __forceinline__ void split_tile(const GroupedDesc& d, const GroupedPack& pk, int tm, int tn, int lane,
+                                           const v4i* __restrict__ frag, const int* __restrict__ expo) {
+    const int K = d.k, kblocks = (K + 31) / 32;
     const int r32 = lane & 31, h = lane >> 5;
-    const bool a_row = d.layout_a == JLC_LAYOUT_ROW, b_row = d.layout_b == JLC_LAYOUT_ROW;
-    // Rows past m and columns past n read the last real one (their outputs are never stored).
-    const int arow = min(i0 + r32, d.m - 1), bcol = min(j0 + r32, d.n - 1);
-    const gcptr pa = (gcptr)d.a + (a_row ? (int64_t)arow * d.lda : (int64_t)arow);
-    const gcptr pb = (gcptr)d.b + (b_row ? (int64_t)bcol : (int64_t)bcol * d.ldb);
-    const int64_t ask = a_row ? 1 : d.lda, bsk = b_row ? d.ldb : 1;
-    const int ea = split_exponent(pa, ask, K, h);
-    const int eb = split_exponent(pb, bsk, K, h);
+    const int i0 = tm * 32, j0 = tn * 32;
+    const int ea = expo[pk.ea_off + i0 + r32];
+    const int eb = expo[pk.eb_off + j0 + r32];
+    const v4i* fa = frag + (pk.a_off + (int64_t)tm * kblocks * S) * 64 + lane;
+    const v4i* fb = frag + (pk.b_off + (int64_t)tn * kblocks * S) * 64 + lane;
     // INT32 diagonal sums stay exact for `chunk` 32-deep blocks: S * chunk * 32 * 127^2 < 2^31.
     const int chunk = (int)(2147483647LL / ((long long)S * 32 * 127 * 127));
-    const int kblocks = (K + 31) / 32;
     const gptr c = (gptr)d.c;
     const int col = j0 + r32;
+    // C (beta != 0) is read ahead of the products, all 16 entries of the lane at once, instead of one
+    // dependent load per stored entry.
+    double cold[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int row = i0 + 8 * (r >> 2) + 4 * h + (r & 3);
+        cold[r] = (d.beta != 0.0 && row < d.m && col < d.n) ? c[(int64_t)row * d.ldc + col] : 0.0;
+    }
+    v4i a0[S], b0[S], a1[S], b1[S];
+    auto load = [&](v4i (&xa)[S], v4i (&xb)[S], int kb) {
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            xa[t] = fa[((int64_t)kb * S + t) * 64];
+            xb[t] = fb[((int64_t)kb * S + t) * 64];
+        }
+    };
     for (int c0 = 0; c0 < kblocks; c0 += chunk) {
         v16i acc[S];
 #pragma unroll
         for (int dd = 0; dd < S; dd++) acc[dd] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const int c1 = min(kblocks, c0 + chunk);
-        for (int kb = c0; kb < c1; kb++) {
-            v4i af[S], bf[S];
-            slice16<S>(pa, ask, 32 * kb + 16 * h, K, ea, af);
-            slice16<S>(pb, bsk, 32 * kb + 16 * h, K, eb, bf);
+        auto mma = [&](const v4i (&af)[S], const v4i (&bf)[S]) {
 #pragma unroll
             for (int t = 0; t < S; t++)
 #pragma unroll
                 for (int u = S - 1 - t; u >= 0; u--)
                     acc[t + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[t], bf[u], acc[t + u], 0, 0, 0);
+        };
+        const int c1 = min(kblocks, c0 + chunk);
+        // Two register sets in turn (one for S >= 7, whose accumulators leave no room: the next block
+        // is then loaded into the set as the multiplies release it); every load is unconditional (the
+        // block index clamped to the chunk's last), so the wait before each multiply covers only that
+        // set's loads.
+        load(a0, b0, c0);
+        if (S >= 7) {
+            for (int kb = c0; kb < c1; kb++) {
+                mma(a0, b0);
+                if (kb + 1 < c1) load(a0, b0, kb + 1);
+            }
+        } else {
+            for (int kb = c0; kb < c1; kb += 2) {
+                load(a1, b1, min(kb + 1, c1 - 1));
+                mma(a0, b0);
+                if (kb + 1 >= c1) break;
+                load(a0, b0, min(kb + 2, c1 - 1));
+                mma(a1, b1);
+            }
         }
         // Accumulator element r of lane l holds C(8*(r/4) + 4*(l/32) + r%4, l%32) of the tile.
 #pragma unroll
@@ -334,7 +560,7 @@ __device__ __forceinline__ void split_tile(const GroupedDesc& d, int i0, int j0,
                 const double v = (ei == SPLIT_NONFINITE || eb == SPLIT_NONFINITE) ? __builtin_nan("")
                                                                                   : ldexp(sum, ei + eb);
                 const gptr pc = c + (int64_t)row * d.ldc + col;
-                if (c0 == 0) *pc = (d.beta == 0.0) ? d.alpha * v : fma(d.alpha, v, d.beta * *pc);
+                if (c0 == 0) *pc = (d.beta == 0.0) ? d.alpha * v : fma(d.alpha, v, d.beta * cold[r]);
                 else *pc = fma(d.alpha, v, *pc);
             }
         }
@@ -357,8 +583,10 @@ __device__ __forceinline__ void scale_tile(const GroupedDesc& d, int i0, int j0,
 
 template <int S>
 __global__ __launch_bounds__(64 * WAVES_PER_GROUP) void grouped_split_i8(const GroupedDesc* __restrict__ descs,
+                                                                         const GroupedPack* __restrict__ packs,
                                                                          const uint64_t* __restrict__ tiles,
-                                                                         int ntiles) {
+                                                                         int ntiles, const v4i* __restrict__ frag,
+                                                                         const int* __restrict__ expo) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_GROUP + (threadIdx.x >> 6));
     const int nwaves = gridDim.x * WAVES_PER_GROUP;
@@ -371,9 +599,8 @@ __global__ __launch_bounds__(64 * WAVES_PER_GROUP) void grouped_split_i8(const G
         const int p = __builtin_amdgcn_readfirstlane((int)(e >> 32));
         const int tm = (int)((e >> 15) & 0x7fff), tn = (int)(e & 0x7fff);
         const GroupedDesc d = descs[p];
-        const int i0 = tm * 32, j0 = tn * 32;
-        if (d.alpha == 0.0 || d.k <= 0) scale_tile(d, i0, j0, lane);
-        else split_tile<S>(d, i0, j0, lane);
+        if (d.alpha == 0.0 || d.k <= 0) scale_tile(d, tm * 32, tn * 32, lane);
+        else split_tile<S>(d, packs[p], tm, tn, lane, frag, expo);
     }
 }
 
@@ -416,18 +643,31 @@ int grouped_split_grid(int slices, int device_cus, int waves_per_simd) {
     }
 }
 
-hipError_t launch_grouped_split_i8(int slices, const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
+hipError_t launch_grouped_split_i8(int slices, const GroupedDesc* descs, const GroupedPack* packs,
+                                   const uint64_t* jobs, int64_t njobs, const uint64_t* sjobs, int64_t nsjobs,
+                                   const uint64_t* tiles, int ntiles, int grid, void* frag, int* expo,
                                    hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
-    const dim3 g(grid), b(64 * WAVES_PER_GROUP);
+    const dim3 b(64 * WAVES_PER_GROUP);
+    const dim3 eg((unsigned)((njobs + WAVES_PER_GROUP - 1) / WAVES_PER_GROUP));
+    const dim3 sg((unsigned)((nsjobs + WAVES_PER_GROUP - 1) / WAVES_PER_GROUP)), g(grid);
+    v4i* fr = static_cast<v4i*>(frag);
+    if (njobs > 0) {
+        hipLaunchKernelGGL(grouped_split_expo, eg, b, 0, stream, descs, packs, jobs, njobs, expo);
+        const hipError_t err = hipGetLastError();
+        if (err != hipSuccess) return err;
+    }
     switch (slices) {
-        case 3: hipLaunchKernelGGL(grouped_split_i8<3>, g, b, 0, stream, descs, tiles, ntiles); break;
-        case 4: hipLaunchKernelGGL(grouped_split_i8<4>, g, b, 0, stream, descs, tiles, ntiles); break;
-        case 5: hipLaunchKernelGGL(grouped_split_i8<5>, g, b, 0, stream, descs, tiles, ntiles); break;
-        case 6: hipLaunchKernelGGL(grouped_split_i8<6>, g, b, 0, stream, descs, tiles, ntiles); break;
-        case 7: hipLaunchKernelGGL(grouped_split_i8<7>, g, b, 0, stream, descs, tiles, ntiles); break;
-        case 8: hipLaunchKernelGGL(grouped_split_i8<8>, g, b, 0, stream, descs, tiles, ntiles); break;
-        case 9: hipLaunchKernelGGL(grouped_split_i8<9>, g, b, 0, stream, descs, tiles, ntiles); break;
+#define JLC_GROUPED_SPLIT_CASE(S_)                                                                                \
+    case S_:                                                                                                      \
+        if (nsjobs > 0)                                                                                           \
+            hipLaunchKernelGGL(grouped_split_slice<S_>, sg, b, 0, stream, descs, packs, jobs, sjobs, nsjobs, fr,  \
+                               expo);                                                                             \
+        hipLaunchKernelGGL(grouped_split_i8<S_>, g, b, 0, stream, descs, packs, tiles, ntiles, fr, expo);         \
+        break;
+        JLC_GROUPED_SPLIT_CASE(3) JLC_GROUPED_SPLIT_CASE(4) JLC_GROUPED_SPLIT_CASE(5) JLC_GROUPED_SPLIT_CASE(6)
+        JLC_GROUPED_SPLIT_CASE(7) JLC_GROUPED_SPLIT_CASE(8) JLC_GROUPED_SPLIT_CASE(9)
+#undef JLC_GROUPED_SPLIT_CASE
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

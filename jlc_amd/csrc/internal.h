@@ -131,7 +131,19 @@ hipError_t launch_grouped_f64(const GroupedDesc* descs, const uint64_t* tiles, i
 // Split-INT8 form of the grouped launch (slices in [SPLIT_MIN_SLICES, SPLIT_MAX_SLICES]): the grid that
 // fills the device for that kernel, and the launch over the same tile list.
 int grouped_split_grid(int slices, int device_cus, int waves_per_simd);
-hipError_t launch_grouped_split_i8(int slices, const GroupedDesc* descs, const uint64_t* tiles, int ntiles, int grid,
+// Where the INT8 slice form keeps one problem's packed operands in the plan's workspace: fragments of
+// 32-row block rb of op(A), k block kb, slice t at 1 KiB block a_off + (rb * kblocks + kb) * S + t
+// (kblocks = ceil(k / 32)), of op(B)'s 32-column blocks likewise from b_off; row / column exponents
+// at expo[ea_off + i] / expo[eb_off + j] (rows and columns padded to 32).
+struct GroupedPack {
+    int64_t a_off, b_off, ea_off, eb_off;
+};
+// Pack jobs: (problem << 32) | (operand << 31) | block, operand 0 = a 32-row block of op(A), 1 = a
+// 32-column block of op(B); slice jobs: (job index << 24) | 32-deep k block of that job. The exponent
+// launch (one wavefront per job) and the slicing launch (one per slice job) precede the product.
+hipError_t launch_grouped_split_i8(int slices, const GroupedDesc* descs, const GroupedPack* packs,
+                                   const uint64_t* jobs, int64_t njobs, const uint64_t* sjobs, int64_t nsjobs,
+                                   const uint64_t* tiles, int ntiles, int grid, void* frag, int* expo,
                                    hipStream_t stream);
 
 // ---- QR

@@ -1263,6 +1263,14 @@ struct jlc_grouped_plan {
     int slices = 0;  // 0: FP64 MFMA tiles; 3..9: the INT8 slice form with that many slices
     jlc::GroupedDesc* d_descs = nullptr;
     uint64_t* d_tiles = nullptr;
+    // INT8 slice form: per-problem workspace offsets, pack jobs, packed fragments and exponents.
+    jlc::GroupedPack* d_packs = nullptr;
+    uint64_t* d_jobs = nullptr;
+    int64_t njobs = 0;
+    uint64_t* d_sjobs = nullptr;
+    int64_t nsjobs = 0;
+    void* d_frag = nullptr;
+    int* d_expo = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1282,7 +1290,7 @@ static int stats_for_stream(hipStream_t s, jlc_stats* stats, double t0, int slic
     info.tile_n = 32;
     info.tile_k = slices ? 32 : 16;
     info.slices = slices;
-    info.launches = 1;
+    info.launches = slices ? 3 : 1;  // the slice form packs its operands in two launches of their own
     fill_stats(stats, info, JLC_KERNEL_GROUPED, f, now_ms() - t0);
     return JLC_OK;
 }
@@ -1370,18 +1378,65 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
             used = std::max(used, t + 1);
         }
     tiles.resize(used);
+    // The INT8 slice form packs every operand block once per run: the workspace holds, per problem with
+    // alpha != 0 and k > 0, S fragments of 1 KiB per (32-row block of op(A) or 32-column block of op(B),
+    // 32-deep k block), and one exponent per padded row / column.
+    std::vector<jlc::GroupedPack> packs;
+    std::vector<uint64_t> jobs, sjobs;
+    int64_t frag_blocks = 0, expo_count = 0;
+    if (form != JLC_MODE_FP64) {
+        packs.assign((size_t)count, jlc::GroupedPack{-1, -1, -1, -1});
+        for (int64_t p = 0; p < count; p++) {
+            const jlc_gemm_desc& d = descs[p];
+            if (d.alpha == 0.0 || d.k <= 0 || d.m <= 0 || d.n <= 0) continue;
+            const int64_t rb = (d.m + 31) / 32, cb = (d.n + 31) / 32, kb = (d.k + 31) / 32;
+            jlc::GroupedPack& pk = packs[(size_t)p];
+            pk.a_off = frag_blocks;
+            frag_blocks += rb * kb * slices;
+            pk.b_off = frag_blocks;
+            frag_blocks += cb * kb * slices;
+            pk.ea_off = expo_count;
+            expo_count += rb * 32;
+            pk.eb_off = expo_count;
+            expo_count += cb * 32;
+            if (kb >= ((int64_t)1 << 24)) return fail(JLC_ERR_INVALID_ARG, "descriptor %lld: k too large for the "
+                                                                         "grouped slice form", (long long)p);
+            for (int64_t i = 0; i < rb + cb; i++) {
+                const uint64_t job = ((uint64_t)p << 32) | (i < rb ? (uint64_t)i : (1ull << 31) | (uint64_t)(i - rb));
+                for (int64_t q = 0; q < kb; q++) sjobs.push_back(((uint64_t)jobs.size() << 24) | (uint64_t)q);
+                jobs.push_back(job);
+            }
+        }
+        if ((int64_t)sjobs.size() > (int64_t)INT32_MAX * jlc::GROUPED_WAVES_PER_GROUP)
+            return fail(JLC_ERR_INVALID_ARG, "too many operand blocks");
+    }
     jlc_grouped_plan* pl = new jlc_grouped_plan;
     pl->device = dev;
     pl->count = count;
     pl->ntiles = (int)tiles.size();
     pl->grid = grid;
     pl->slices = form == JLC_MODE_FP64 ? 0 : slices;
+    pl->njobs = (int64_t)jobs.size();
+    pl->nsjobs = (int64_t)sjobs.size();
     hipError_t e = hipMalloc(&pl->d_descs, std::max<size_t>(1, (size_t)count) * sizeof(jlc::GroupedDesc));
     if (e == hipSuccess) e = hipMalloc(&pl->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(uint64_t));
     if (e == hipSuccess && count)
         e = hipMemcpy(pl->d_descs, descs, (size_t)count * sizeof(jlc::GroupedDesc), hipMemcpyHostToDevice);
     if (e == hipSuccess && !tiles.empty())
         e = hipMemcpy(pl->d_tiles, tiles.data(), tiles.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && form != JLC_MODE_FP64) {
+        e = hipMalloc(&pl->d_packs, std::max<size_t>(1, packs.size()) * sizeof(jlc::GroupedPack));
+        if (e == hipSuccess) e = hipMalloc(&pl->d_jobs, std::max<size_t>(1, jobs.size()) * sizeof(uint64_t));
+        if (e == hipSuccess) e = hipMalloc(&pl->d_frag, std::max<size_t>(1, (size_t)frag_blocks) * 1024);
+        if (e == hipSuccess) e = hipMalloc(&pl->d_expo, std::max<size_t>(1, (size_t)expo_count) * sizeof(int));
+        if (e == hipSuccess && !packs.empty())
+            e = hipMemcpy(pl->d_packs, packs.data(), packs.size() * sizeof(jlc::GroupedPack), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !jobs.empty())
+            e = hipMemcpy(pl->d_jobs, jobs.data(), jobs.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&pl->d_sjobs, std::max<size_t>(1, sjobs.size()) * sizeof(uint64_t));
+        if (e == hipSuccess && !sjobs.empty())
+            e = hipMemcpy(pl->d_sjobs, sjobs.data(), sjobs.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         jlc_grouped_plan_destroy(pl);
         return fail(JLC_ERR_ALLOC, "grouped plan upload failed: %s", hipGetErrorString(e));
@@ -1391,8 +1446,9 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
 }
 
 static hipError_t launch_plan(const jlc_grouped_plan* plan, hipStream_t s) {
-    return plan->slices ? jlc::launch_grouped_split_i8(plan->slices, plan->d_descs, plan->d_tiles, plan->ntiles,
-                                                       plan->grid, s)
+    return plan->slices ? jlc::launch_grouped_split_i8(plan->slices, plan->d_descs, plan->d_packs, plan->d_jobs,
+                                                       plan->njobs, plan->d_sjobs, plan->nsjobs, plan->d_tiles,
+                                                       plan->ntiles, plan->grid, plan->d_frag, plan->d_expo, s)
                         : jlc::launch_grouped_f64(plan->d_descs, plan->d_tiles, plan->ntiles, plan->grid, s);
 }
 
@@ -1443,6 +1499,11 @@ int jlc_grouped_plan_destroy(jlc_grouped_plan* plan) {
         if (plan->graph) (void)hipGraphDestroy(plan->graph);
         if (plan->d_descs) (void)hipFree(plan->d_descs);
         if (plan->d_tiles) (void)hipFree(plan->d_tiles);
+        if (plan->d_packs) (void)hipFree(plan->d_packs);
+        if (plan->d_jobs) (void)hipFree(plan->d_jobs);
+        if (plan->d_sjobs) (void)hipFree(plan->d_sjobs);
+        if (plan->d_frag) (void)hipFree(plan->d_frag);
+        if (plan->d_expo) (void)hipFree(plan->d_expo);
     }
     delete plan;
     return JLC_OK;

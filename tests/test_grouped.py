@@ -170,7 +170,7 @@ def _check_split_problem(w, i, c_now, slices, tag):
     assert same.all(), (tag, i, m, n, k, la, lb, int((~same).sum()))
 
 
-@pytest.mark.parametrize("slices", [3, 5, 7, 9])
+@pytest.mark.parametrize("slices", [3, 5, 7, 8, 9])
 def test_grouped_split_int8_bitwise_against_restatement(slices):
     # Each problem must carry the single-GEMM INT8 slice form's bits: same
     # exponents, digits, diagonal sums and FP64 recombination.
