@@ -256,20 +256,56 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, double alpha, con
 
     // v_mfma_f64_16x16x4 accumulator layout: element r of lane l holds C(l/16 + 4*r, l%16)
     // of its 16 x 16 block.
+    if (beta == 0.0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < 4; i++) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < 4; j++) {
+                const int col = n0 + wn * 64 + j * 16 + c16;
+                if (MASKED && col >= N) continue;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = m0 + wm * 64 + i * 16 + g + 4 * r;
+                    if (!MASKED || row < M) C[(int64_t)row * ldc + col] = alpha * acc[i][j][r];
+                }
+            }
+        }
+        return;
+    }
+    // beta != 0: C is read eight entries (two 16 x 16 fragments' share of the lane) at a time, the next
+    // group's loads issued before the current group's stores, so the reads overlap one another instead of
+    // each waiting behind the previous entry's store (a short K loop would otherwise spend most of its
+    // time here).
+    double cv[2][2][4];
+    auto load_c = [&](int q, double (&x)[2][4]) {
+        const int i = q >> 1;
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+            const int col = n0 + wn * 64 + ((q & 1) * 2 + jj) * 16 + c16;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wm * 64 + i * 16 + g + 4 * r;
+                x[jj][r] = (!MASKED || (row < M && col < N)) ? C[(int64_t)row * ldc + col] : 0.0;
+            }
+        }
+    };
+    load_c(0, cv[0]);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (q < 7) load_c(q + 1, cv[(q + 1) & 1]);
+        const int i = q >> 1;
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+            const int j = (q & 1) * 2 + jj;
             const int col = n0 + wn * 64 + j * 16 + c16;
             if (MASKED && col >= N) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = m0 + wm * 64 + i * 16 + g + 4 * r;
                 if (!MASKED || row < M) {
-                    double* p = C + (int64_t)row * ldc + col;
                     double v = alpha * acc[i][j][r];
-                    if (beta != 0.0) v += beta * *p;
-                    *p = v;
+                    v += beta * cv[q & 1][jj][r];
+                    C[(int64_t)row * ldc + col] = v;
                 }
             }
         }
