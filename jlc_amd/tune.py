@@ -2,10 +2,10 @@
 and writes the table (jlc_amd/tuning/mi355x.tune) that dispatch reads.
 
 Tuned entries:
-* splitk_tiles_per_cu / splitk_min_kslice: when a GEMM has few output tiles and
-  a long K (the Gram matrix and W = C^T V of blocked QR, tall-skinny products),
-  K is cut into slices; candidates are timed on such shapes and on square ones
-  (which must not regress).
+* splitk_tiles_per_cu / splitk_min_kslice: when a GEMM has few output tiles for
+  its K (the Gram matrix and W = C^T V of blocked QR, tall-skinny products, and
+  small products that leave most CUs idle), K is cut into slices; candidates are
+  timed on such shapes and on square ones (which must not regress).
 * grouped_waves_per_simd: width of the persistent grouped launch; grouped_small_tile, the m (n)
   up to which a problem runs 16-row (16-column) MFMA tiles instead of 32; grouped_chunk_tiles,
   how many tiles of one problem go to one XCD together; all timed on a batch of uneven problems.
@@ -49,10 +49,16 @@ def _time_dev(fn, iters: int) -> float:
 
 
 def tune_splitk(quick: bool) -> dict:
-    shapes = [(128, 128, 32768), (3968, 128, 32768), (1024, 128, 16384), (2048, 2048, 16384), (8192, 8192, 4096)]
+    # tall-skinny and long-K products (the Gram matrix and W = C^T V of blocked QR), square ones that must
+    # not regress, and small ones where a few 128 x 128 tiles leave most CUs idle
+    shapes = [(128, 128, 32768), (3968, 128, 32768), (1024, 128, 16384), (2048, 2048, 16384), (8192, 8192, 4096),
+              (256, 256, 256), (512, 512, 512), (1024, 1024, 1024), (512, 512, 4096)]
     if quick:
-        shapes = shapes[:3]
-    cands = [(0, 1024), (1, 1024), (2, 1024), (4, 1024), (2, 512), (4, 512), (2, 2048), (8, 512)]
+        shapes = shapes[:3] + shapes[5:7]
+    # Slices at least 128 deep: a slice of depth d writes and re-reads its partial C, 16 / (2d) bytes per
+    # flop, which at d = 64 would ask 8 TB/s of a product running at the FP64 peak.
+    cands = [(0, 1024), (4, 1024), (4, 512), (2, 512), (1, 512), (4, 256), (2, 256), (1, 256), (4, 128), (2, 128),
+             (1, 128)]
     rng = np.random.default_rng(0)
     bufs = {}
     for (m, n, k) in shapes:

@@ -5,14 +5,16 @@ For each phi and split form the test pins which kernel ran and with how many sli
 a form which ran its own kernels has a max relative error against double-double dot products, on a
 128 x 128 sample, at most the FP64 MFMA kernel's on the same inputs, so that automatic mode never hands
 back a less accurate result than FP64. A form that cannot hold the spread runs the FP64 kernel and
-returns its result bit for bit. The CRT pick is also checked against its defining rule: the fewest
-moduli that hold the chosen norm bits, with the leftover capacity spent on bits.
+returns its result bit for bit. The FP64 reference is the unsplit kernel (tests/fp64_ref.py). The
+CRT pick is also checked against its defining rule: the fewest moduli that hold the chosen norm bits,
+with the leftover capacity spent on bits.
 """
 import numpy as np
 import pytest
 
 from jlc_amd import native
 from oracle import oracle as orc
+from tests.fp64_ref import unsplit_fp64
 
 pytestmark = pytest.mark.gpu
 
@@ -44,7 +46,8 @@ def test_chooser_picks_and_accuracy(phi):
     nz = exact != 0.0
 
     def run(mode):
-        st = native.dgemm_dev(native.ROW, native.ROW, M, N, K, 1.0, da.ptr, K, db.ptr, N, 0.0, dc.ptr, N, mode)
+        with unsplit_fp64():
+            st = native.dgemm_dev(native.ROW, native.ROW, M, N, K, 1.0, da.ptr, K, db.ptr, N, 0.0, dc.ptr, N, mode)
         c = dc.to_numpy()
         got = c.reshape(M, N)[ii, jj]
         return st, c, float(np.max(np.abs(got[nz] - exact[nz]) / np.abs(exact[nz])))
