@@ -25,7 +25,13 @@
  * reads A or B (C = beta*C). This is the behaviour of the reference's
  * MICROKERNEL / SMALL / MATVEC regimes (PackingUtils.scaleCPanel,
  * src/main/java/net/faulj/compute/PackingUtils.java:159-183;
- * OptimizedBLAS3.java:190-192; SpecializedKernels.java:32-49).
+ * OptimizedBLAS3.java:190-192; SpecializedKernels.java:32-49). It differs from
+ * the reference's OptimizedBLAS3 TINY CPU path, which keeps a NaN-seeded C at
+ * beta == 0 (src/test/java/net/faulj/compute/GemmNumericEdgeTest.java:44-47):
+ * the host-side dispatch only reaches this library above its offload gate
+ * (hip_min_flops, 14,155,776 flops in the shipped table), far above the TINY
+ * regime (under 512 flops, GemmDispatch.java:30-33), so calls routed by the
+ * default gate keep the reference's TINY semantics on the CPU.
  */
 #ifndef JLC_HIP_H
 #define JLC_HIP_H

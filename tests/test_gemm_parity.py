@@ -183,7 +183,13 @@ def test_offheap_facade_and_output_sync():
 
 
 def test_numeric_edge_known_answers():
-    """GemmNumericEdgeTest (GemmNumericEdgeTest.java:32-95) known answers, BLAS semantics."""
+    """GemmNumericEdgeTest (GemmNumericEdgeTest.java:32-95) known answers, BLAS semantics.
+
+    HIP_POLICY forces the GPU path even for these 2 x 2 and 1 x 1 calls, where beta = 0 clears a
+    NaN-seeded C. The reference's OptimizedBLAS3 TINY CPU path keeps that NaN
+    (GemmNumericEdgeTest.java:44-47); the expectation here is safe only because the shipped offload
+    gate (hip_min_flops 14,155,776) never sends TINY sizes (under 512 flops) to the GPU, so under the
+    default policy such calls stay on the CPU path with the reference's semantics."""
     cases = json.loads((Path(__file__).parent / "golden" / "gemm_kat.json").read_text())["cases"]
     for cs in cases:
         conv = lambda v: float(v)
