@@ -136,3 +136,18 @@ def test_select_for_multiply_order():
     assert nosimd.selectForMultiply(100, 100, 100) is Algorithm.NAIVE
     assert nosimd.selectForMultiply(300, 100, 100) is Algorithm.BLOCKED
     assert nosimd.selectForMultiply(10, 10, 10) is Algorithm.NAIVE
+
+
+def test_shipped_offload_gate_keeps_tiny_calls_off_the_gpu():
+    """The library's BLAS beta = 0 semantics clear a NaN-seeded C, where the reference's TINY CPU path
+    keeps it (GemmNumericEdgeTest.java:44-47); the two never meet while the shipped offload gate sits
+    above the TINY regime's 512 flops (GemmDispatch.java:30-33)."""
+    from pathlib import Path
+    table = Path(__file__).resolve().parent.parent / "jlc_amd" / "tuning" / "mi355x.tune"
+    gate = None
+    for line in table.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 2 and parts[0] == "hip_min_flops":
+            gate = float(parts[1])
+    assert gate is not None and gate > 512
+    assert GemmDispatch.selectKernel(2, 2, 2, False, 1) is K.TINY and 2 * 2 * 2 * 2 < gate
