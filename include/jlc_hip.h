@@ -130,8 +130,12 @@ int jlc_dgemm_dev(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, d
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Same arguments and results as jlc_dgemm_dev with mode JLC_MODE_FP64 (the
- * same kernels, bit for bit); stream NULL is the null stream; always
- * asynchronous. The split-K partial products that jlc_dgemm_dev keeps in a
+ * same kernels, bit for bit, with one exception: a shape with more output
+ * tiles than run at once whose last round jlc_dgemm_dev fills by splitting K,
+ * splitk_quant, is only split here when the partials fit the array below, and
+ * otherwise runs unsplit, its sums then in another order); stream NULL is the
+ * null stream; always asynchronous. The split-K partial products that
+ * jlc_dgemm_dev keeps in a
  * per-thread buffer go to a 256 MiB device array the library holds on each
  * device for the life of the process (its address is resolved once per device
  * by jlc_hip_init, which this call runs for device 0 if nothing has). A shape
@@ -266,7 +270,8 @@ int jlc_dgeqrf_batched_dev(int64_t count, const jlc_qr_desc* descs, void* stream
  * src/main/java/net/faulj/compute/GemmDispatch.java:32), splitk_tiles_per_cu,
  * splitk_min_kslice, grouped_waves_per_simd, grouped_xcds (tile lists dealt per XCD;
  * 1 = plain order), grouped_small_tile (grouped FP64 problems with m or n up to this run 16-row or
- * 16-column tiles; 0 = always 32 x 32), grouped_chunk_tiles (tiles of one problem dealt to one XCD
+ * 16-column tiles; 0 = always 32 x 32), splitk_quant (1: split K by 2-4 when the
+ * last round of a many-tile grid would run mostly empty), grouped_chunk_tiles (tiles of one problem dealt to one XCD
  * together), crt_bits_offset, crt_bits_slope, crt_waves, crt_group, crt_xcd,
  * qr_wave_max_lds (batched QR size classes up to this many LDS doubles run one wavefront per matrix).
  * Text file, one "key value" per line.

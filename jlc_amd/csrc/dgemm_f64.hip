@@ -403,6 +403,26 @@ int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus, int til
     return s < 2 ? 1 : (int)s;
 }
 
+int dgemm_quant_slices(int64_t m, int64_t n, int64_t k, int device_cus, int64_t max_partial_bytes) {
+    // Many tiles, but a last round of resident tiles (two workgroups per CU) that is mostly empty: cutting
+    // K into s <= 4 slices fills the rounds at the cost of writing and re-reading s partial products. The
+    // time in units of one tile's 16-deep step (2 * 128 * 128 * 16 flops at 1/512 of 67 TFLOP/s, 4.0 us):
+    //   ceil(tiles * s / slots) * ceil(k / (16 s))  +  (s + 2) * m * n * 8 bytes at 5 TB/s (4e-7 steps per
+    //   element and pass).
+    const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+    const int64_t slots = 2 * (int64_t)device_cus;
+    if (device_cus <= 0 || tiles <= slots || k < 512) return 1;
+    auto cost = [&](int s) {
+        const double rounds = (double)((tiles * s + slots - 1) / slots);
+        const double steps = (double)((k + 16 * s - 1) / (16 * s));
+        return rounds * steps + (s > 1 ? (s + 2) * (double)m * (double)n * 4e-7 : 0.0);
+    };
+    int best = 1;
+    for (int s = 2; s <= 4; s++)
+        if ((int64_t)s * m * n * (int64_t)sizeof(double) <= max_partial_bytes && cost(s) < 0.97 * cost(best)) best = s;
+    return best;
+}
+
 int64_t dgemm_splitk_kslice(int64_t k, int slices) {
     int64_t ks = (k + slices - 1) / slices;
     return (ks + BK - 1) / BK * BK;

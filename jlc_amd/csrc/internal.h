@@ -39,6 +39,7 @@ struct TuneTable {
     int crt_waves = 8;                // CRT product kernel: 4 (128 x 128 per wavefront) or 8 (128 x 64)
     int crt_group = 16;               // CRT product kernel: tile rows per column strip of the tile order
     int crt_xcd = 0;                  // CRT product kernel: 1 = keep neighbouring tiles on one XCD
+    int splitk_quant = 1;             // split K against wave quantization of many-tile grids (0 = off)
     int grouped_small_tile = 16;      // grouped FP64: 16-row / 16-column tiles for m / n up to this (0: off)
     int grouped_chunk_tiles = 64;     // grouped: tiles of one problem dealt to one XCD together (whole tile rows)
     int qr_wave_max_lds = 2560;       // batched QR: size classes up to this many LDS doubles run one
@@ -57,6 +58,9 @@ hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, in
 // Split-K variant: K cut into `slices` pieces whose partial products go to
 // work (slices * m * n doubles) and are summed in order into C.
 int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus, int tiles_per_cu, int min_kslice);
+// Split factor (1..4) against wave quantization for grids of more tiles than are resident at once, with
+// the partial products bounded by max_partial_bytes; 1 when the split does not pay.
+int dgemm_quant_slices(int64_t m, int64_t n, int64_t k, int device_cus, int64_t max_partial_bytes);
 hipError_t launch_dgemm_f64_splitk(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k, double alpha,
                                    const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                                    double* c, int64_t ldc, int slices, double* work, hipStream_t stream,

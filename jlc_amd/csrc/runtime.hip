@@ -550,6 +550,9 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
 // split-K partial products go to the thread's grow-only workspace (ordered across streams by ws), or,
 // when `fixed` is given, to that caller-owned buffer of fixed_bytes; a call whose partials do not fit
 // it fails before anything is launched.
+// Largest workspace a wave-quantization split may claim (the grow-only per-thread buffer keeps it).
+constexpr size_t QUANT_SPLIT_MAX_BYTES = (size_t)1 << 30;
+
 int run_fp64(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
              const double* b, int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t stream,
              jlc::TileInfo* info, int* kernel, WorkspaceUse* ws, double* fixed, size_t fixed_bytes) {
@@ -561,7 +564,10 @@ int run_fp64(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, cons
     }
     *kernel = JLC_KERNEL_MFMA_F64;
     const jlc::TuneTable tt = jlc::tune_snapshot();
-    const int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load(), tt.splitk_tiles_per_cu, tt.splitk_min_kslice);
+    int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load(), tt.splitk_tiles_per_cu, tt.splitk_min_kslice);
+    if (slices == 1 && tt.splitk_quant)
+        slices = jlc::dgemm_quant_slices(m, n, k, g_cus.load(),
+                                         fixed ? (int64_t)fixed_bytes : (int64_t)QUANT_SPLIT_MAX_BYTES);
     if (slices > 1) {
         const size_t need = (size_t)slices * (size_t)(m * n) * sizeof(double);
         double* work = fixed;
@@ -763,6 +769,7 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"splitk_min_kslice", nullptr, &jlc::TuneTable::splitk_min_kslice},
         {"grouped_waves_per_simd", nullptr, &jlc::TuneTable::grouped_waves_per_simd},
         {"grouped_xcds", nullptr, &jlc::TuneTable::grouped_xcds},
+        {"splitk_quant", nullptr, &jlc::TuneTable::splitk_quant},
         {"grouped_small_tile", nullptr, &jlc::TuneTable::grouped_small_tile},
         {"grouped_chunk_tiles", nullptr, &jlc::TuneTable::grouped_chunk_tiles},
         {"crt_bits_offset", &jlc::TuneTable::crt_bits_offset, nullptr},
@@ -826,6 +833,7 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "splitk_min_kslice %d\n", t.splitk_min_kslice);
     std::fprintf(f, "grouped_waves_per_simd %d\n", t.grouped_waves_per_simd);
     std::fprintf(f, "grouped_xcds %d\n", t.grouped_xcds);
+    std::fprintf(f, "splitk_quant %d\n", t.splitk_quant);
     std::fprintf(f, "grouped_small_tile %d\n", t.grouped_small_tile);
     std::fprintf(f, "grouped_chunk_tiles %d\n", t.grouped_chunk_tiles);
     std::fprintf(f, "crt_bits_offset %.17g\n", t.crt_bits_offset);
