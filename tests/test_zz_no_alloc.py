@@ -87,3 +87,29 @@ def test_noalloc_rejects_partials_beyond_the_scratch():
         assert "scratch" in native.last_error()
     finally:
         native.tune_set("splitk_tiles_per_cu", old)
+
+
+def test_noalloc_skips_the_quantization_split_its_scratch_cannot_hold():
+    """6000^3: jlc_dgemm_dev fills the grid's last round by splitting K three ways (partials 864 MB);
+    the no-alloc call, whose scratch is 256 MiB, runs the same product unsplit instead of failing.
+    The two results are the same product summed in another order: within the FP64 bound of each other,
+    and the unsplit one equal bit for bit to jlc_dgemm_dev with the split switched off."""
+    rng = np.random.default_rng(6000)
+    m = n = k = 6000
+    A, B = rng.uniform(-1, 1, m * k), rng.uniform(-1, 1, k * n)
+    da, db = native.DeviceArray.from_numpy(A), native.DeviceArray.from_numpy(B)
+    split, got, plain = native.DeviceArray(m * n), native.DeviceArray(m * n), native.DeviceArray(m * n)
+    native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, split.ptr, n)
+    native.check(_noalloc(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, got.ptr, n, None))
+    native.synchronize()
+    old = native.tune_get("splitk_quant")
+    try:
+        native.tune_set("splitk_quant", 0)
+        native.dgemm_dev(native.ROW, native.ROW, m, n, k, 1.0, da.ptr, k, db.ptr, n, 0.0, plain.ptr, n)
+    finally:
+        native.tune_set("splitk_quant", old)
+    g, p, s = got.to_numpy(), plain.to_numpy(), split.to_numpy()
+    assert np.array_equal(g, p)
+    assert not np.array_equal(g, s)  # the default call did split
+    bound = 2 * k * 2.0 ** -53 * np.max(np.abs(A)) * np.max(np.abs(B)) * k
+    assert np.max(np.abs(g - s)) <= bound
