@@ -722,13 +722,6 @@ __global__ __launch_bounds__(256) void crt_combine(const v4i* __restrict__ Cr, i
 #pragma unroll
     for (int t = 0; t < NM; t++) vv[t] = Cr[t * c_plane + (int64_t)blk * 64 + lane];
     const int fj = eb[col];
-    // C (beta != 0) read for all 16 outputs of the lane before any is stored.
-    double cold[16];
-#pragma unroll
-    for (int b = 0; b < 16; b++) {
-        const int row = rbase + 8 * (b >> 2) + (b & 3);
-        cold[b] = (beta != 0.0 && row < M) ? C[(int64_t)row * ldc + col] : 0.0;
-    }
 #pragma unroll
     for (int b = 0; b < 16; b++) {
         const int row = rbase + 8 * (b >> 2) + (b & 3);
@@ -739,7 +732,8 @@ __global__ __launch_bounds__(256) void crt_combine(const v4i* __restrict__ Cr, i
         const int ei = ea[row];
         const double v = (ei == NONFINITE || fj == NONFINITE) ? __builtin_nan("")
                                                               : ldexp(crt_value<NM>(r, T), ei + fj - pa - pb);
-        C[(int64_t)row * ldc + col] = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * cold[b]);
+        double* p = C + (int64_t)row * ldc + col;
+        *p = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * *p);
     }
 }
 
