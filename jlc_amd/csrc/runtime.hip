@@ -1357,7 +1357,8 @@ int jlc_grouped_plan_create(int64_t count, const jlc_gemm_desc* descs, int mode,
     std::vector<double> work((size_t)X, 0.0);
     for (int64_t p : probs) {
         const jlc_gemm_desc& d = descs[p];
-        const bool rows16 = d.m <= small, cols16 = d.n <= small;
+        // (tile indices hold 15 bits: 16-wide tiles only up to 16 * 32767)
+        const bool rows16 = d.m <= small && d.m <= 16 * 32767, cols16 = d.n <= small && d.n <= 16 * 32767;
         const int TM = rows16 ? 16 : 32, TN = cols16 ? 16 : 32;
         const uint64_t shape = (uint64_t)((rows16 ? 2 : 0) | (cols16 ? 1 : 0)) << jlc::GROUPED_SHAPE_SHIFT;
         const int tm = (int)((d.m + TM - 1) / TM), tn = (int)((d.n + TN - 1) / TN);
