@@ -223,7 +223,12 @@ int jlc_dgemm_grouped_dev(int64_t count, const jlc_gemm_desc* descs, int mode, v
  * (:219-291) stores them. Panels of nb <= 128 columns are factored in FP64 on
  * the GPU; the compact-WY trailing updates (:167-209) run on the GEMM entry
  * points in `mode` (JLC_MODE_FP64, JLC_MODE_SPLIT_INT8[, slices] or
- * JLC_MODE_SPLIT_CRT[, moduli]); T is FP64. */
+ * JLC_MODE_SPLIT_CRT[, moduli]); T is FP64. A panel of at least the tuning
+ * table's qr_cqr_min_rows rows and twice its width is factored by CholeskyQR2
+ * with Householder reconstruction (the same reflectors, tau and R in exact
+ * arithmetic; a panel whose Cholesky breaks down or loses orthogonality runs
+ * the column kernels instead); stats->tile_n counts the panels that took that
+ * path, stats->tile_k the panels factored column by column. */
 int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, int64_t nb, int mode, void* stream,
                    jlc_stats* stats);
 /* Thin Q (column-major m x n, ldq) from the factors (buildQ, HouseholderQR.java:390-444), n <= m. */

@@ -47,6 +47,8 @@ struct TuneTable {
     int qr_block_threads = 256;       // batched QR: threads per matrix above qr_wave_max_lds (256/512/1024)
     int qr_reg_max_m = 192;           // batched QR: matrices with m <= this (at most 192), n <= 128 held
                                       // in registers; larger ones in LDS
+    int qr_cqr_min_rows = 512;        // blocked QR: panels of at least this many rows (and 2 * width)
+                                      // by CholeskyQR2 + reconstruction (0 = column kernels only)
 };
 TuneTable tune_snapshot();
 
@@ -166,6 +168,18 @@ int qr_panel_chunk();
 // (G = ceil((m - k0) / qr_panel_chunk())); k1 - k0 <= 128.
 hipError_t launch_qr_panel_fused(double* at, int64_t lda, int m, int k0, int k1, double* tau, double* part,
                                  double* dots, double* xbuf, hipStream_t stream);
+// CholeskyQR2 panel with Householder reconstruction (qr_cqr.hip), b <= 128:
+// pass 1: R1 = chol(G), X1 = R1^-1, *flag = 1 on breakdown (else 0);
+// pass 2: R2 = chol(G), r_out = R2 * r1, X2 = R2^-1, *flag |= 2 on breakdown or max|G - I| > 2^-8.
+hipError_t launch_cqr_chol(const double* g, int b, int pass, const double* r1, double* r_out, double* x_out,
+                           int* flag, hipStream_t stream);
+// Modified LU of Q's top b x b block (column-major, ldq): Y1 and D R into the panel's top block ap (lda),
+// tau, ui = U^-1 (row-major b x b), spec (b ints) = candidates for the tau = 0 case.
+hipError_t launch_cqr_lu(const double* q, int64_t ldq, int b, const double* r, double* ap, int64_t lda, double* tau,
+                         double* ui, int* spec, hipStream_t stream);
+// Columns flagged in spec whose Y2 part (rows b..mp-1 of ap) is exactly zero: tau = 0, R row negated.
+hipError_t launch_cqr_special(double* ap, int64_t lda, int mp, int b, double* tau, const int* spec,
+                              hipStream_t stream);
 hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb, double* vc, hipStream_t stream);
 hipError_t launch_build_t(const double* g, int nb, const double* tau, double* T, hipStream_t stream);
 hipError_t launch_extract_r(const double* at, int64_t lda, int n, double* r, hipStream_t stream);

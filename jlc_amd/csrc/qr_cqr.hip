@@ -1,0 +1,437 @@
+// Tall QR panels by CholeskyQR2 with Householder reconstruction.
+//
+// The column-at-a-time panel (qr.hip) needs one grid-wide reduction per column, so a 128-wide panel
+// of a tall matrix is 129 dependent launches, each re-reading the panel from memory. For a panel
+// A_p (m_p x b, m_p >= 2b) this path instead forms, with a handful of launches:
+//
+//   CholeskyQR2:   G1 = A_p^T A_p, R1 = chol(G1), Q1 = A_p R1^-1,
+//                  G2 = Q1^T Q1,   R2 = chol(G2), Q  = Q1 R2^-1,  R = R2 R1
+//   reconstruction (Ballard, Demmel, Grigori, Jacquelin, Nguyen, Solomonik, "Reconstructing
+//   Householder vectors from Tall-Skinny QR", IPDPS 2014): the LU factorisation without pivoting
+//   Q_top - D = Y1 U, with the sign D_ii = -sign of the i-th pivot taken as the elimination reaches
+//   it, and Y2 = Q_bottom U^-1. Then H = I - Y T Y^T with Y = [Y1; Y2] unit lower trapezoidal is the
+//   product of the b reflectors Householder QR of A_p forms, R_H = D R is its R factor and
+//   tau_i = -U_ii D_ii (T_ii of T = -U D Y1^-T).
+//
+// With R's diagonal positive, Householder QR of A_p and of Q = A_p R^-1 make the same reflectors, and
+// the sign rule above is the reference's beta = -sign(x0) * ||x|| (HouseholderQR.computeHouseholder,
+// src/main/java/net/faulj/decomposition/qr/HouseholderQR.java:219-291), so in exact arithmetic the
+// factors are the reference's. Its one discontinuous case, ||x(1:)|| == 0 with x0 >= 0 (tau = 0,
+// R(k,k) = x0 instead of tau = 2, R(k,k) = -x0), shows as a column of Y that is exactly zero below the
+// diagonal; cqr_special gives such a column tau = 0 and R's row its other sign.
+//
+// CholeskyQR2 needs kappa(A_p) below ~u^-1/2: the first Cholesky reports a pivot that is not positive
+// and finite, the second one an orthogonality defect max|Q1^T Q1 - I| above 2^-8 (which bounds
+// ||Q1^T Q1 - I||_2 by 1/2 for b <= 128, where the second pass is exact to O(u)). Either sets a flag
+// the host reads before anything is written to the matrix; the panel then runs the column kernels.
+//
+// Kernels run one workgroup of 1024 threads on a b x b (b <= 128) matrix held in registers: thread t
+// owns column k = t % 128 and rows 16g + q (g = t / 128, q < 16), so a thread's rows are the same
+// across a wavefront and contiguous (LDS reads of row-indexed values are 16-byte broadcasts). Each
+// elimination or substitution step broadcasts one row and/or column through a double-buffered LDS
+// vector: one barrier per step.
+#include <math.h>
+
+#include "internal.h"
+
+namespace jlc {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int CQ_NT = 512;           // 8 wavefronts
+constexpr int CQ_W = CQ_NT / 64;
+constexpr int SLD = 129;             // LDS matrix: 128 x SLD doubles, row-major (odd: conflict-free columns)
+constexpr int CQ_DYN = (128 * SLD + 8 * 256) * 8;  // dynamic LDS: the matrix and 8 diagonal-block inverses
+
+__device__ __forceinline__ d4 mfma(d4 acc, double a, double b) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+
+// v_mfma_f64_16x16x4 fragments: lane l (i16 = l % 16, g = l / 16) supplies A(i16, k) and B(k, i16) with
+// k = g + 4 * step for the four steps of a 16-deep product; accumulator register r of lane l holds
+// C(g + 4r, i16).
+// acc += sa * A * B, A(i, k) at a[i * lda + k], B(k, j) at b[k * ldb + j]; b_upper reads B's entries
+// below its diagonal as zero.
+__device__ __forceinline__ d4 tile_mma(d4 acc, const double* a, int lda, const double* b, int ldb, double sa,
+                                       bool b_upper = false) {
+    const int l = threadIdx.x & 63, i16 = l & 15, g = l >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const double bv = b[(g + 4 * s) * ldb + i16];
+        acc = mfma(acc, sa * a[i16 * lda + g + 4 * s], (b_upper && g + 4 * s > i16) ? 0.0 : bv);
+    }
+    return acc;
+}
+
+// acc += sa * A^T * B, A(k, i) at a[k * lda + i].
+__device__ __forceinline__ d4 tile_mma_tn(d4 acc, const double* a, int lda, const double* b, int ldb, double sa) {
+    const int l = threadIdx.x & 63, i16 = l & 15, g = l >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; s++) acc = mfma(acc, sa * a[(g + 4 * s) * lda + i16], b[(g + 4 * s) * ldb + i16]);
+    return acc;
+}
+
+__device__ __forceinline__ d4 tile_load(const double* c, int ldc) {
+    const int l = threadIdx.x & 63, i16 = l & 15, g = l >> 4;
+    d4 v;
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = c[(g + 4 * r) * ldc + i16];
+    return v;
+}
+
+// upper_only: entries below the tile's diagonal are left as they are.
+__device__ __forceinline__ void tile_store(double* c, int ldc, d4 v, bool upper_only) {
+    const int l = threadIdx.x & 63, i16 = l & 15, g = l >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        if (!upper_only || g + 4 * r <= i16) c[(g + 4 * r) * ldc + i16] = v[r];
+}
+
+// In place, X = U^-1 for the upper-triangular nblk*16 square U in S (entries below the diagonal are not
+// read, and only the diagonal tiles' upper triangles are written, so a strictly lower part survives).
+// dinv[i] = 1 / U(i, i). First the nblk diagonal 16 x 16 blocks are inverted side by side (16 lanes per
+// block, each lane one column by back substitution) into dblk; then block rows from the bottom up:
+// X(ib, cb) = -U(ib, ib)^-1 sum_{ib < lb <= cb} U(ib, lb) X(lb, cb) on FP64 MFMA, one tile per wavefront.
+__device__ void upper_inverse_blocked(double* S, double* dblk, const double* dinv, int nblk) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    if (w < 2) {
+        const int blk = 4 * w + (l >> 4), c = l & 15;
+        if (blk < nblk) {
+            const double* U = S + 16 * blk * SLD + 16 * blk;
+            double x[16];
+#pragma unroll
+            for (int r = 15; r >= 0; r--) {
+                double v = r == c ? 1.0 : 0.0;
+#pragma unroll
+                for (int q = r + 1; q < 16; q++) v = fma(-U[r * SLD + q], x[q], v);
+                x[r] = v * dinv[16 * blk + r];
+            }
+#pragma unroll
+            for (int r = 0; r < 16; r++) dblk[blk * 256 + r * 16 + c] = x[r];
+        }
+    }
+    __syncthreads();
+    if (w == 0) tile_store(S + 16 * (nblk - 1) * (SLD + 1), SLD, tile_load(dblk + (nblk - 1) * 256, 16), true);
+    for (int ib = nblk - 2; ib >= 0; ib--) {
+        const int cb = ib + 1 + w;
+        d4 res = {0.0, 0.0, 0.0, 0.0};
+        if (cb < nblk) {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            for (int lb = ib + 1; lb <= cb; lb++)  // X(cb, cb)'s strictly lower part may hold other data
+                acc = tile_mma(acc, S + 16 * ib * SLD + 16 * lb, SLD, S + 16 * lb * SLD + 16 * cb, SLD, 1.0, lb == cb);
+            // res = -Dinv(ib) * acc: acc's register s is the B fragment of product step s.
+            const int i16 = l & 15, g = l >> 4;
+            const double* di = dblk + ib * 256;
+#pragma unroll
+            for (int s = 0; s < 4; s++) res = mfma(res, -di[i16 * 16 + g + 4 * s], acc[s]);
+        }
+        __syncthreads();  // row block ib of U fully read
+        if (cb < nblk) tile_store(S + 16 * ib * SLD + 16 * cb, SLD, res, false);
+        if (w == CQ_W - 1) tile_store(S + 16 * ib * (SLD + 1), SLD, tile_load(dblk + ib * 256, 16), true);
+        __syncthreads();
+    }
+}
+
+// pass 1: R1 = chol(G) (upper, R1^T R1 = G), X1 = R1^-1, both b x b row-major; flag = 1 on a pivot that
+// is not positive and finite (a plain store: pass 1 starts the panel's flag).
+// pass 2: R2 = chol(G) as above plus max|G - I| > 2^-8 (flag |= 2); r_out = R2 * R1 (R1 row-major from
+// r1) stored column-major, X2 = R2^-1 row-major.
+// G is b x b row-major (symmetric; only its upper triangle is read); padded with the identity to a
+// multiple of 16. Right-looking blocked Cholesky: for each 16-row block, 16 steps (one barrier each)
+// factor the block row across all columns, then the trailing upper tiles take the block's rank-16
+// update on FP64 MFMA.
+__global__ __launch_bounds__(CQ_NT) void cqr_chol(const double* __restrict__ gm, int b, int pass,
+                                                  const double* __restrict__ r1, double* __restrict__ r_out,
+                                                  double* __restrict__ x_out, int* __restrict__ flag) {
+    __shared__ double rowb[2][128];
+    __shared__ double dinv[128];
+    __shared__ int bad;
+    extern __shared__ double dyn[];
+    double* S = dyn;
+    double* dblk = dyn + 128 * SLD;
+    const int t = threadIdx.x, w = t >> 6;
+    const int nb16 = (b + 15) & ~15, nblk = nb16 >> 4;
+    const int c = t & 127, h = t >> 7;  // panel role: column c, rows h + 4i of the block
+    if (t == 0) bad = 0;
+    double dev = 0.0;
+    for (int r = h; r < nb16; r += 4) {
+        if (c >= nb16) break;
+        double v;
+        if (r < b && c < b) {
+            v = r <= c ? gm[r * b + c] : 0.0;
+            if (r <= c) dev = fmax(dev, fabs(v - (r == c ? 1.0 : 0.0)));
+        } else {
+            v = r == c ? 1.0 : 0.0;
+        }
+        S[r * SLD + c] = v;
+    }
+    __syncthreads();
+    if (pass == 2 && !(dev <= 0x1p-8)) bad = 1;
+    for (int jb = 0; jb < nblk; jb++) {
+        const int j0 = 16 * jb;
+        const bool act = c < nb16;
+        double pv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) pv[i] = act ? S[(j0 + h + 4 * i) * SLD + c] : 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) {
+            const int j = j0 + jj, p = jj & 1;
+            const bool own = h == (jj & 3);
+            if (own) rowb[p][c] = pv[jj >> 2];
+            __syncthreads();
+            const double d = rowb[p][j];
+            if (t == 0 && !(d > 0.0 && d <= 1.7976931348623157e308)) bad = 1;
+            const double rk = rowb[p][c], id = 1.0 / d;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = j0 + h + 4 * i;
+                if (4 * i + h > jj) {  // uniform in a wavefront
+                    const double rr = rowb[p][r];
+                    if (c >= r) pv[i] = fma(-(rr * id), rk, pv[i]);
+                }
+            }
+            if (own) {
+                const double sd = sqrt(d);
+                pv[jj >> 2] = c >= j ? rk / sd : 0.0;
+                if (c == j) dinv[j] = 1.0 / pv[jj >> 2];
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) S[(j0 + h + 4 * i) * SLD + c] = pv[i];
+        }
+        __syncthreads();
+        // Trailing upper tiles (kb, lb), jb < kb <= lb: S(kb, lb) -= R(jb, kb)^T R(jb, lb).
+        const int nt = nblk - jb - 1;
+        for (int idx = w; idx < nt * (nt + 1) / 2; idx += CQ_W) {
+            int kb = jb + 1, rem = idx;
+            while (rem >= nblk - kb) {
+                rem -= nblk - kb;
+                kb++;
+            }
+            const int lb = kb + rem;
+            double* ct = S + 16 * kb * SLD + 16 * lb;
+            d4 acc = tile_load(ct, SLD);
+            acc = tile_mma_tn(acc, S + j0 * SLD + 16 * kb, SLD, S + j0 * SLD + 16 * lb, SLD, -1.0);
+            tile_store(ct, SLD, acc, false);
+        }
+        __syncthreads();
+    }
+    if (pass == 1) {
+        for (int r = h; r < b; r += 4)
+            if (c < b) r_out[r * b + c] = S[r * SLD + c];
+    } else {
+        // R = R2 R1 tile by tile (upper tiles; R1 from global, zero past b), stored column-major.
+        const int l = t & 63, i16 = l & 15, g = l >> 4;
+        for (int idx = w; idx < nblk * (nblk + 1) / 2; idx += CQ_W) {
+            int ib = 0, rem = idx;
+            while (rem >= nblk - ib) {
+                rem -= nblk - ib;
+                ib++;
+            }
+            const int cb = ib + rem;
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            for (int lb = ib; lb <= cb; lb++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const int kr = 16 * lb + g + 4 * s, col = 16 * cb + i16;
+                    const double bv = (kr < b && col < b) ? r1[kr * b + col] : 0.0;
+                    acc = mfma(acc, S[(16 * ib + i16) * SLD + 16 * lb + g + 4 * s], bv);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = 16 * ib + g + 4 * r, col = 16 * cb + i16;
+                if (row < b && col < b) r_out[col * b + row] = acc[r];
+            }
+        }
+        // Zeros below the diagonal of the column-major R.
+        for (int r = h; r < b; r += 4)
+            if (c < r) r_out[c * b + r] = 0.0;
+    }
+    __syncthreads();
+    upper_inverse_blocked(S, dblk, dinv, nblk);
+    for (int r = h; r < b; r += 4)
+        if (c < b) x_out[r * b + c] = r <= c ? S[r * SLD + c] : 0.0;
+    if (t == 0) {
+        if (pass == 1) *flag = bad ? 1 : 0;
+        else if (bad) *flag |= 2;
+    }
+}
+
+// Modified LU of the top b x b block of Q (column-major, ld ldq): Q_top - D = Y1 U, D_ii = -sign(c_i)
+// of the pivot c_i the elimination reaches (copysign, so -0.0 counts as negative as in the
+// reference's copySign), U_ii = c_i - D_ii (|U_ii| >= 1, no breakdown). Writes into the panel's top
+// block (column-major, lda): Y1 below the diagonal, R_H = D R (R column-major b x b) on and above it;
+// tau_i = -U_ii D_ii; ui = U^-1 (row-major); spec_i = 1 where Y1's column i is exactly zero below the
+// diagonal and c_i >= 0 (a candidate for the reference's tau = 0 case; cqr_special checks Y2).
+// Right-looking blocked: for each 16-column block, 16 steps (one barrier each) factor the block column,
+// each column of the block row is then solved by one thread, and the trailing tiles take the rank-16
+// update on FP64 MFMA; U^-1 as in cqr_chol. Padded with the identity to a multiple of 16.
+__global__ __launch_bounds__(CQ_NT) void cqr_lu(const double* __restrict__ qm, int64_t ldq, int b,
+                                                const double* __restrict__ rr, double* __restrict__ ap,
+                                                int64_t lda, double* __restrict__ tau, double* __restrict__ ui,
+                                                int* __restrict__ spec) {
+    __shared__ double rowb[2][16];
+    __shared__ double colb[2][128];
+    __shared__ double dinv[128];
+    __shared__ double dsg[128];
+    __shared__ double cpv[128];
+    __shared__ double upv[128];
+    __shared__ int nz[128];
+    extern __shared__ double dyn[];
+    double* S = dyn;
+    double* dblk = dyn + 128 * SLD;
+    const int t = threadIdx.x, w = t >> 6;
+    const int nb16 = (b + 15) & ~15, nblk = nb16 >> 4;
+    // Coalesced load of Q's top block (column contiguous in memory), identity past b.
+    {
+        const int r = t & 127;
+        for (int c = t >> 7; c < nb16; c += 4)
+            if (r < nb16) S[r * SLD + c] = (r < b && c < b) ? qm[(int64_t)c * ldq + r] : (r == c ? 1.0 : 0.0);
+    }
+    if (t < 128) nz[t] = 0;
+    __syncthreads();
+    const int cc = t & 15, r0 = t >> 4;  // block-column role: column j0 + cc, rows j0 + r0 + 32e
+    for (int jb = 0; jb < nblk; jb++) {
+        const int j0 = 16 * jb;
+        double pv[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int r = j0 + r0 + 32 * e;
+            pv[e] = r < nb16 ? S[r * SLD + j0 + cc] : 0.0;
+        }
+#pragma unroll
+        for (int ii = 0; ii < 16; ii++) {
+            const int i = j0 + ii, p = ii & 1;
+            if (r0 == ii) rowb[p][cc] = pv[0];
+            if (cc == ii) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) colb[p][r0 + 32 * e] = pv[e];
+            }
+            __syncthreads();
+            const double cv = rowb[p][ii];
+            const double d = -copysign(1.0, cv);
+            const double u = cv - d;
+            const double iu = 1.0 / u;
+            const double rk = rowb[p][cc];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = j0 + r0 + 32 * e;
+                if (r > i && r < nb16) {
+                    const double lv = colb[p][r0 + 32 * e] * iu;
+                    if (cc > ii) pv[e] = fma(-lv, rk, pv[e]);
+                    else if (cc == ii) pv[e] = lv;
+                }
+            }
+            if (r0 == ii && cc == ii) {
+                pv[0] = u;
+                dinv[i] = iu;
+                dsg[i] = d;
+                cpv[i] = cv;
+                upv[i] = u;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int r = j0 + r0 + 32 * e;
+            if (r < nb16) S[r * SLD + j0 + cc] = pv[e];
+        }
+        __syncthreads();
+        // Block row: U(i, c) = A(i, c) - sum_{j0 <= q < i} L(i, q) U(q, c), one thread per column c.
+        {
+            const int c = j0 + 16 + t;
+            if (c < nb16) {
+                double uv[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) uv[i] = S[(j0 + i) * SLD + c];
+#pragma unroll
+                for (int i = 1; i < 16; i++)
+#pragma unroll
+                    for (int q = 0; q < i; q++) uv[i] = fma(-S[(j0 + i) * SLD + j0 + q], uv[q], uv[i]);
+#pragma unroll
+                for (int i = 1; i < 16; i++) S[(j0 + i) * SLD + c] = uv[i];
+            }
+        }
+        __syncthreads();
+        // Trailing tiles (rb, cb), rb, cb > jb: S(rb, cb) -= L(rb, jb) U(jb, cb).
+        const int nt = nblk - jb - 1;
+        for (int idx = w; idx < nt * nt; idx += CQ_W) {
+            const int rb = jb + 1 + idx / nt, cb = jb + 1 + idx % nt;
+            double* ct = S + 16 * rb * SLD + 16 * cb;
+            d4 acc = tile_load(ct, SLD);
+            acc = tile_mma(acc, S + 16 * rb * SLD + j0, SLD, S + j0 * SLD + 16 * cb, SLD, -1.0);
+            tile_store(ct, SLD, acc, false);
+        }
+        __syncthreads();
+    }
+    {
+        const int c = t & 127;
+        for (int r = t >> 7; r < b; r += 4)
+            if (c < b && r > c && S[r * SLD + c] != 0.0) nz[c] = 1;
+    }
+    __syncthreads();
+    if (t < b) {
+        tau[t] = -upv[t] * dsg[t];
+        spec[t] = (nz[t] == 0 && cpv[t] >= 0.0) ? 1 : 0;
+    }
+    upper_inverse_blocked(S, dblk, dinv, nblk);
+    {
+        const int c = t & 127;
+        for (int r = t >> 7; r < b; r += 4)
+            if (c < b) ui[r * b + c] = r <= c ? S[r * SLD + c] : 0.0;
+    }
+    __syncthreads();
+    // Top block of the factored panel: D R on and above the diagonal (R column-major), Y1 below.
+    {
+        const int r = t & 127;
+        for (int c = t >> 7; c < b; c += 4)
+            if (r < b) ap[(int64_t)c * lda + r] = r > c ? S[r * SLD + c] : dsg[r] * rr[c * b + r];
+    }
+}
+
+// The reference's tau = 0 case: column i of Y exactly zero below the diagonal (Y1 part in spec, Y2
+// rows b..mp-1 checked here) with a non-negative pivot: H_i = I, tau_i = 0 and R's row i keeps the
+// pivot's sign (the row is negated back). One workgroup per panel column; the last column of a panel
+// has no Y1 part, so about half the panels scan one column.
+__global__ __launch_bounds__(1024) void cqr_special(double* __restrict__ ap, int64_t lda, int mp, int b,
+                                                    double* __restrict__ tau, const int* __restrict__ spec) {
+    const int i = blockIdx.x;
+    if (spec[i] == 0) return;
+    const double* col = ap + (int64_t)i * lda;
+    int any = 0;
+#pragma unroll 8
+    for (int r = b + (int)threadIdx.x; r < mp; r += 1024) any |= col[r] != 0.0;
+    any = __syncthreads_or(any);
+    if (any) return;
+    if (threadIdx.x == 0) tau[i] = 0.0;
+    for (int c = i + (int)threadIdx.x; c < b; c += 1024) ap[(int64_t)c * lda + i] = -ap[(int64_t)c * lda + i];
+}
+
+}  // namespace
+
+hipError_t launch_cqr_chol(const double* g, int b, int pass, const double* r1, double* r_out, double* x_out,
+                           int* flag, hipStream_t stream) {
+    if (b < 1 || b > 128 || (pass != 1 && pass != 2)) return hipErrorInvalidValue;
+    (void)hipFuncSetAttribute((const void*)cqr_chol, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    hipLaunchKernelGGL(cqr_chol, dim3(1), dim3(CQ_NT), CQ_DYN, stream, g, b, pass, r1, r_out, x_out, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_cqr_lu(const double* q, int64_t ldq, int b, const double* r, double* ap, int64_t lda, double* tau,
+                         double* ui, int* spec, hipStream_t stream) {
+    if (b < 1 || b > 128 || ldq < b) return hipErrorInvalidValue;
+    (void)hipFuncSetAttribute((const void*)cqr_lu, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    hipLaunchKernelGGL(cqr_lu, dim3(1), dim3(CQ_NT), CQ_DYN, stream, q, ldq, b, r, ap, lda, tau, ui, spec);
+    return hipGetLastError();
+}
+
+hipError_t launch_cqr_special(double* ap, int64_t lda, int mp, int b, double* tau, const int* spec,
+                              hipStream_t stream) {
+    if (b < 1 || b > 128 || mp < b) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cqr_special, dim3(b), dim3(1024), 0, stream, ap, lda, mp, b, tau, spec);
+    return hipGetLastError();
+}
+
+}  // namespace jlc

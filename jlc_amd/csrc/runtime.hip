@@ -780,6 +780,7 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"qr_wave_max_lds", nullptr, &jlc::TuneTable::qr_wave_max_lds},
         {"qr_block_threads", nullptr, &jlc::TuneTable::qr_block_threads},
         {"qr_reg_max_m", nullptr, &jlc::TuneTable::qr_reg_max_m},
+        {"qr_cqr_min_rows", nullptr, &jlc::TuneTable::qr_cqr_min_rows},
     };
     for (const F& f : fields) {
         if (std::strcmp(f.name, key) != 0) continue;
@@ -844,6 +845,7 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "qr_wave_max_lds %d\n", t.qr_wave_max_lds);
     std::fprintf(f, "qr_block_threads %d\n", t.qr_block_threads);
     std::fprintf(f, "qr_reg_max_m %d\n", t.qr_reg_max_m);
+    std::fprintf(f, "qr_cqr_min_rows %d\n", t.qr_cqr_min_rows);
     std::fclose(f);
     return JLC_OK;
 }
@@ -1566,6 +1568,57 @@ int panel_wy(const double* a, int64_t lda, int64_t m, int64_t k0, int64_t nb, co
     return JLC_OK;
 }
 
+// Workspace of a CholeskyQR2 panel: b x b matrices (row-major, b <= nb), two m x nb column-major panels
+// (Q1, Q), the breakdown flag and the tau = 0 candidates.
+struct CqrWork {
+    double *g, *r1, *x1, *r, *x2, *ui, *q1, *q;
+    int *flag, *spec;
+};
+
+// Panel [k0, k0 + b) of the column-major m-row matrix by CholeskyQR2 and Householder reconstruction
+// (qr_cqr.hip): G1 = Ap^T Ap, R1 = chol(G1), Q1 = Ap R1^-1, G2 = Q1^T Q1, R2 = chol(G2), then (when
+// neither Cholesky reported a breakdown or a loss of orthogonality, read back here before anything is
+// written to the matrix) Q = Q1 R2^-1, the modified LU of Q's top block (Y1, R_H = D R2 R1, tau, U^-1),
+// Y2 = Q_bottom U^-1 into the panel, and the tau = 0 fix-up. *done = false leaves the panel untouched
+// for the column kernels.
+int panel_cqr(double* a, int64_t lda, int64_t m, int64_t k0, int64_t b, double* tau, const CqrWork& w,
+              hipStream_t s, jlc::TileInfo* info, bool* done) {
+    *done = false;
+    const int64_t mp = m - k0;
+    double* ap = a + k0 * lda + k0;
+    const int bi = (int)b;
+    int rc;
+    if ((rc = gemm_on(JLC_ROW, JLC_COL, b, b, mp, 1.0, ap, lda, ap, lda, 0.0, w.g, b, JLC_MODE_FP64, s, info)) !=
+        JLC_OK)
+        return rc;
+    JLC_HIP_CHECK(jlc::launch_cqr_chol(w.g, bi, 1, nullptr, w.r1, w.x1, w.flag, s));
+    // Q1^T (b x mp, row-major) = X1^T Ap^T
+    if ((rc = gemm_on(JLC_COL, JLC_ROW, b, mp, b, 1.0, w.x1, b, ap, lda, 0.0, w.q1, mp, JLC_MODE_FP64, s, info)) !=
+        JLC_OK)
+        return rc;
+    if ((rc = gemm_on(JLC_ROW, JLC_COL, b, b, mp, 1.0, w.q1, mp, w.q1, mp, 0.0, w.g, b, JLC_MODE_FP64, s, info)) !=
+        JLC_OK)
+        return rc;
+    JLC_HIP_CHECK(jlc::launch_cqr_chol(w.g, bi, 2, w.r1, w.r, w.x2, w.flag, s));
+    info->launches += 2;
+    int hflag = -1;
+    JLC_HIP_CHECK(hipMemcpyAsync(&hflag, w.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    if (hflag != 0) return JLC_OK;
+    if ((rc = gemm_on(JLC_COL, JLC_ROW, b, mp, b, 1.0, w.x2, b, w.q1, mp, 0.0, w.q, mp, JLC_MODE_FP64, s, info)) !=
+        JLC_OK)
+        return rc;
+    JLC_HIP_CHECK(jlc::launch_cqr_lu(w.q, mp, bi, w.r, ap, lda, tau + k0, w.ui, w.spec, s));
+    // Y2^T (b x (mp - b), rows of ap below the top block, ld lda) = U^-T Q_bottom^T
+    if ((rc = gemm_on(JLC_COL, JLC_ROW, b, mp - b, b, 1.0, w.ui, b, w.q + b, mp, 0.0, ap + b, lda, JLC_MODE_FP64, s,
+                      info)) != JLC_OK)
+        return rc;
+    JLC_HIP_CHECK(jlc::launch_cqr_special(ap, lda, (int)mp, bi, tau + k0, w.spec, s));
+    info->launches += 2;
+    *done = true;
+    return JLC_OK;
+}
+
 }  // namespace
 
 int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, int64_t nb, int mode, void* stream,
@@ -1593,6 +1646,30 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     if (e == hipSuccess) e = t.alloc((size_t)nb * nb * 8);
     if (e == hipSuccess) e = w1.alloc((size_t)n * nb * 8);
     if (e == hipSuccess) e = w2.alloc((size_t)n * nb * 8);
+    // CholeskyQR2 panels: those with at least qr_cqr_min_rows rows and twice their width.
+    const int64_t cqr_min = jlc::tune_snapshot().qr_cqr_min_rows;
+    const auto cqr_panel = [&](int64_t k0, int64_t b) {
+        return cqr_min > 0 && m - k0 >= cqr_min && m - k0 >= 2 * b;
+    };
+    DevBuf cqp, cqs;
+    CqrWork cw{};
+    if (e == hipSuccess && cqr_panel(0, std::min(nb, kmax))) {
+        e = cqp.alloc((size_t)2 * m * nb * 8);
+        if (e == hipSuccess) e = cqs.alloc((size_t)6 * nb * nb * 8 + (size_t)(nb + 1) * sizeof(int));
+        if (e == hipSuccess) {
+            cw.q1 = cqp.d();
+            cw.q = cqp.d() + m * nb;
+            double* sm = cqs.d();
+            cw.g = sm;
+            cw.r1 = sm + nb * nb;
+            cw.x1 = sm + 2 * nb * nb;
+            cw.r = sm + 3 * nb * nb;
+            cw.x2 = sm + 4 * nb * nb;
+            cw.ui = sm + 5 * nb * nb;
+            cw.flag = reinterpret_cast<int*>(sm + 6 * nb * nb);
+            cw.spec = cw.flag + 1;
+        }
+    }
     if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "QR workspace allocation failed: %s", hipGetErrorString(e));
     jlc::TileInfo info;
     // With stats, an event pair around each panel's launches: main_ms is the panel kernels' device time.
@@ -1613,15 +1690,25 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         }
         JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
     }
+    int cqr_panels = 0, col_panels = 0;
     for (int64_t k0 = 0; k0 < kmax; k0 += nb) {
         const int64_t k1 = k0 + nb < kmax ? k0 + nb : kmax;
         const int64_t nbk = k1 - k0, mk = m - k0, nt = n - k1;
         const size_t pi = (size_t)(2 * (k0 / nb));
         if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi], s));
-        JLC_HIP_CHECK(jlc::launch_qr_panel_fused(a, lda, (int)m, (int)k0, (int)k1, tau, part.d(), dots.d(), vtmp.d(),
-                                                 s));
+        bool panel_done = false;
+        if (cw.q && cqr_panel(k0, nbk) &&
+            (rc = panel_cqr(a, lda, m, k0, nbk, tau, cw, s, &info, &panel_done)) != JLC_OK)
+            return rc;
+        if (!panel_done) {
+            JLC_HIP_CHECK(jlc::launch_qr_panel_fused(a, lda, (int)m, (int)k0, (int)k1, tau, part.d(), dots.d(),
+                                                     vtmp.d(), s));
+            info.launches += (int)nbk + 1;
+            col_panels++;
+        } else {
+            cqr_panels++;
+        }
         if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi + 1], s));
-        info.launches += (int)nbk + 1;
         if (nt <= 0) continue;
         if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
         // Transposed compact-WY update of C = A[k0:m, k1:n] held as C^T (nt x mk, row-major, ld = lda):
@@ -1653,6 +1740,8 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         JLC_HIP_CHECK(hipStreamSynchronize(s));  // workspace is freed on return
     }
     info.tile_m = (int)nb;
+    info.tile_n = cqr_panels;
+    info.tile_k = col_panels;
     fill_stats(stats, info, JLC_KERNEL_QR, kms, now_ms() - t0);
     return JLC_OK;
 }

@@ -100,6 +100,87 @@ def test_wide_and_ragged_panels_against_oracle():
         assert np.linalg.norm(Q.T @ Q - np.eye(kk)) < 1e-12, (m, n, nb)
 
 
+def _oracle_factors(A, nb):
+    m, n = A.shape
+    ref = np.asfortranarray(A).reshape(-1, order="F").copy()
+    tau_ref = orc.qr_factor(ref, m, n, nb=nb)
+    return ref.reshape(n, m).T, tau_ref
+
+
+def _cqr_panels(m, n, nb):
+    """Panels the table's CholeskyQR2 bound sends down that path (at least qr_cqr_min_rows rows and twice
+    the panel's width)."""
+    lo = native.tune_get("qr_cqr_min_rows")
+    kk = min(m, n)
+    widths = [(k0, min(nb, kk - k0)) for k0 in range(0, kk, nb)]
+    return sum(1 for k0, b in widths if lo > 0 and m - k0 >= lo and m - k0 >= 2 * b), len(widths)
+
+
+def test_cholesky_qr2_panels_against_oracle():
+    """Tall panels by CholeskyQR2 + Householder reconstruction: tau, R and the Householder vectors as the
+    oracle restatement (computeHouseholder's conventions, HouseholderQR.java:219-291), A = QR and
+    Q^T Q = I at FP64 level; with the path switched off (qr_cqr_min_rows 0) the column kernels give the
+    same factors."""
+    rng = np.random.default_rng(101)
+    saved = native.tune_get("qr_cqr_min_rows")
+    for (m, n, nb) in [(2048, 300, 128), (1024, 256, 64), (4096, 128, 128), (700, 100, 100), (1536, 1536, 128)]:
+        A = rng.standard_normal((m, n))
+        kk = min(m, n)
+        Fr, tau_ref = _oracle_factors(A, nb)
+        want, panels = _cqr_panels(m, n, nb)
+        assert want > 0
+        outs = []
+        for lo in (saved, 0):
+            native.tune_set("qr_cqr_min_rows", lo)
+            try:
+                da, d0, dt, st = _factor(A, nb=nb)
+            finally:
+                native.tune_set("qr_cqr_min_rows", saved)
+            assert (st.tile_n, st.tile_k) == ((want, panels - want) if lo else (0, panels)), (m, n, nb, lo)
+            F, tau = da.to_numpy().reshape(n, m).T, dt.to_numpy()
+            np.testing.assert_allclose(tau, tau_ref, rtol=1e-10, atol=1e-12)
+            np.testing.assert_allclose(np.triu(F[:kk]), np.triu(Fr[:kk]), rtol=0, atol=1e-10 * np.abs(Fr).max())
+            np.testing.assert_allclose(np.tril(F[:, :kk], -1), np.tril(Fr[:, :kk], -1), rtol=0, atol=1e-10)
+            recon, orth = _errors(A, da, d0, dt, nb)
+            assert recon < 1e-14 and orth < 1e-13, (m, n, nb, lo, recon, orth)
+            outs.append(F)
+        print(f"[qr] cqr {m}x{n} nb={nb}: {want} of {panels} panels, max |CQR - column| "
+              f"{np.abs(outs[0] - outs[1]).max():.2e}", flush=True)
+
+
+def test_cholesky_qr2_special_columns_and_fallback():
+    """Upper-triangular panels (every column zero below its pivot: the reference's tau = 0 case for a
+    non-negative pivot, tau = 2 with the row negated for a negative one) come out of the reconstruction
+    exactly as the oracle's; a rank-deficient panel (a repeated column) and one of condition number 1e12
+    fail CholeskyQR2's checks and run the column kernels, to the same FP64-level errors."""
+    rng = np.random.default_rng(102)
+    m, n, nb = 1024, 256, 128
+    up = np.zeros((m, n))
+    up[:n] = np.triu(rng.uniform(0.5, 1.0, (n, n)))
+    signed = up.copy()
+    signed[:n] *= rng.choice([-1.0, 1.0], n)[:, None]
+    for name, A in (("upper", up), ("signed", signed)):
+        da, d0, dt, st = _factor(A, nb=nb)
+        assert (st.tile_n, st.tile_k) == (2, 0), name
+        F, tau = da.to_numpy().reshape(n, m).T, dt.to_numpy()
+        Fr, tau_ref = _oracle_factors(A, nb)
+        assert np.array_equal(tau, tau_ref), name  # exactly 0 or 2
+        # the reference returns R = A exactly; R2 R1 carries CholeskyQR's kappa * u rounding
+        np.testing.assert_allclose(np.triu(F[:n]), np.triu(Fr[:n]), rtol=0, atol=1e-11, err_msg=name)
+        assert not np.any(np.tril(F, -1)), name
+    dup = rng.standard_normal((m, n))
+    dup[:, 5] = dup[:, 3]
+    ill = rng.standard_normal((m, n))
+    u, _ = np.linalg.qr(rng.standard_normal((m, nb)))
+    v, _ = np.linalg.qr(rng.standard_normal((nb, nb)))
+    ill[:, :nb] = (u * np.geomspace(1.0, 1e-12, nb)) @ v.T  # condition number 1e12, not a column scaling
+    for name, A in (("repeated column", dup), ("condition 1e12", ill)):
+        da, d0, dt, st = _factor(A, nb=nb)
+        assert st.tile_k >= 1 and st.tile_n + st.tile_k == 2, (name, st.tile_n, st.tile_k)
+        recon, orth = _errors(A, da, d0, dt, nb)
+        assert recon < 1e-14 and orth < 1e-13, (name, recon, orth)
+
+
 def _batch(rng, count, lo=16, hi=192):
     ms = rng.integers(lo, hi + 1, count)
     ns = np.maximum(1, ms // 2)
