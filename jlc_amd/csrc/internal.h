@@ -173,6 +173,14 @@ hipError_t launch_qr_panel_fused(double* at, int64_t lda, int m, int k0, int k1,
 // pass 2: R2 = chol(G), r_out = R2 * r1, X2 = R2^-1, *flag |= 2 on breakdown or max|G - I| > 2^-8.
 hipError_t launch_cqr_chol(const double* g, int b, int pass, const double* r1, double* r_out, double* x_out,
                            int* flag, hipStream_t stream);
+// G = X^T X (b x b row-major, both triangles) for the column-major m x b matrix X (ld ldx), b <= 128:
+// partial 16 x 16 tiles per row range in part (gram_work_doubles(m) doubles), then a fixed-order sum.
+int gram_parts(int64_t m);
+size_t gram_work_doubles(int64_t m);
+hipError_t launch_gram(const double* x, int64_t ldx, int m, int b, double* part, double* g, hipStream_t stream);
+// T (b x b row-major, upper) of the compact-WY product of b reflectors from G = V^T V and tau, as the
+// inverse of diag(1/tau) + strictly-upper(G) (reflectors with tau = 0 take no part), b <= 128.
+hipError_t launch_wy_t(const double* g, int b, const double* tau, double* T, hipStream_t stream);
 // Modified LU of Q's top b x b block (column-major, ldq): Y1 and D R into the panel's top block ap (lda),
 // tau, ui = U^-1 (row-major b x b), spec (b ints) = candidates for the tau = 0 case.
 hipError_t launch_cqr_lu(const double* q, int64_t ldq, int b, const double* r, double* ap, int64_t lda, double* tau,

@@ -44,6 +44,21 @@ constexpr int CQ_W = CQ_NT / 64;
 constexpr int SLD = 129;             // LDS matrix: 128 x SLD doubles, row-major (odd: conflict-free columns)
 constexpr int CQ_DYN = (128 * SLD + 8 * 256) * 8;  // dynamic LDS: the matrix and 8 diagonal-block inverses
 
+// 1/d and 1/sqrt(d) from the hardware estimates refined by Newton steps (within an ulp or two; the
+// IEEE division and square root sequences would sit on every elimination step's critical path).
+__device__ __forceinline__ double rcp_nr(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
+__device__ __forceinline__ double rsq_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int i = 0; i < 2; i++) y = fma(0.5 * y, fma(-d * y, y, 1.0), y);
+    return y;
+}
+
 __device__ __forceinline__ d4 mfma(d4 acc, double a, double b) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
 }
@@ -182,7 +197,7 @@ __global__ __launch_bounds__(CQ_NT) void cqr_chol(const double* __restrict__ gm,
             __syncthreads();
             const double d = rowb[p][j];
             if (t == 0 && !(d > 0.0 && d <= 1.7976931348623157e308)) bad = 1;
-            const double rk = rowb[p][c], id = 1.0 / d;
+            const double rk = rowb[p][c], id = rcp_nr(d);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int r = j0 + h + 4 * i;
@@ -192,9 +207,9 @@ __global__ __launch_bounds__(CQ_NT) void cqr_chol(const double* __restrict__ gm,
                 }
             }
             if (own) {
-                const double sd = sqrt(d);
-                pv[jj >> 2] = c >= j ? rk / sd : 0.0;
-                if (c == j) dinv[j] = 1.0 / pv[jj >> 2];
+                const double rs = rsq_nr(d);
+                pv[jj >> 2] = c >= j ? rk * rs : 0.0;
+                if (c == j) dinv[j] = rcp_nr(pv[jj >> 2]);
             }
         }
         if (act) {
@@ -314,7 +329,7 @@ __global__ __launch_bounds__(CQ_NT) void cqr_lu(const double* __restrict__ qm, i
             const double cv = rowb[p][ii];
             const double d = -copysign(1.0, cv);
             const double u = cv - d;
-            const double iu = 1.0 / u;
+            const double iu = rcp_nr(u);
             const double rk = rowb[p][cc];
 #pragma unroll
             for (int e = 0; e < 4; e++) {
@@ -391,6 +406,156 @@ __global__ __launch_bounds__(CQ_NT) void cqr_lu(const double* __restrict__ qm, i
     }
 }
 
+// Gram matrix G = X^T X of a column-major m x b matrix X (ld ldx, b <= 128), in two launches: workgroup p
+// of gram_partial stages rows [p * per, (p + 1) * per) in 64-row chunks through LDS (column c of a chunk
+// contiguous, as in memory) and accumulates the upper 16 x 16 tiles of X_p^T X_p on FP64 MFMA, one
+// tile per wavefront at a time; gram_reduce sums the partial tiles in a fixed order and writes both
+// triangles of G (b x b, row-major).
+constexpr int GR_ROWS = 64;
+constexpr int GR_LDC = GR_ROWS + 1;
+constexpr int GR_MAXT = 5;  // tiles per wavefront: 36 upper tiles of a 128 x 128 Gram over 8 wavefronts
+
+__global__ __launch_bounds__(CQ_NT) void gram_partial(const double* __restrict__ x, int64_t ldx, int m, int b,
+                                                      int per, double* __restrict__ part) {
+    __shared__ double xs[128 * GR_LDC];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, i16 = l & 15, g = l >> 4;
+    const int nblk = (b + 15) >> 4, ntile = nblk * (nblk + 1) / 2;
+    const int r0 = blockIdx.x * per, r1 = min(m, r0 + per);
+    int ti[GR_MAXT], tj[GR_MAXT];
+#pragma unroll
+    for (int q = 0; q < GR_MAXT; q++) {
+        int idx = w + CQ_W * q, a = 0;
+        while (idx >= nblk - a && a < nblk) {
+            idx -= nblk - a;
+            a++;
+        }
+        ti[q] = a;
+        tj[q] = a + idx;
+    }
+    d4 acc[GR_MAXT];
+#pragma unroll
+    for (int q = 0; q < GR_MAXT; q++) acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+    // Staging: thread t moves row pairs (2 * (t % 32), +1) of columns t / 32 + 16 j, j < 8, so one
+    // 64-row chunk is eight 16-byte loads per thread, issued for the next chunk before this one's MFMAs.
+    const int kp = 2 * (t & 31), c0 = t >> 5;
+    double2 nxt[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = c0 + 16 * j, k = k0 + kp;
+            double2 v = make_double2(0.0, 0.0);
+            if (c < b) {
+                const double* src = x + (int64_t)c * ldx + k;
+                if (k + 1 < r1 && (((uintptr_t)src) & 15) == 0) {
+                    v = *reinterpret_cast<const double2*>(src);
+                } else {
+                    if (k < r1) v.x = src[0];
+                    if (k + 1 < r1) v.y = src[1];
+                }
+            }
+            nxt[j] = v;
+        }
+    };
+    if (r0 < r1) fetch(r0);
+    for (int k0 = r0; k0 < r1; k0 += GR_ROWS) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = c0 + 16 * j;
+            xs[c * GR_LDC + kp] = nxt[j].x;
+            xs[c * GR_LDC + kp + 1] = nxt[j].y;
+        }
+        __syncthreads();
+        if (k0 + GR_ROWS < r1) fetch(k0 + GR_ROWS);
+#pragma unroll
+        for (int q = 0; q < GR_MAXT; q++) {
+            if (w + CQ_W * q < ntile) {
+                const double* xa = xs + (16 * ti[q] + i16) * GR_LDC + g;
+                const double* xb = xs + (16 * tj[q] + i16) * GR_LDC + g;
+#pragma unroll
+                for (int st = 0; st < GR_ROWS / 4; st++) acc[q] = mfma(acc[q], xa[4 * st], xb[4 * st]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < GR_MAXT; q++) {
+        const int idx = w + CQ_W * q;
+        if (idx < ntile) {
+            double* o = part + ((int64_t)blockIdx.x * ntile + idx) * 256;
+#pragma unroll
+            for (int r = 0; r < 4; r++) o[(g + 4 * r) * 16 + i16] = acc[q][r];
+        }
+    }
+}
+
+// Workgroup (tile, 16-entry group): thread (entry e, lane group pg) sums partials pg, pg + 16, ... of
+// its entry, then the 16 sums are added in a fixed order (the result does not depend on timing).
+__global__ __launch_bounds__(256) void gram_reduce(const double* __restrict__ part, int nparts, int b,
+                                                   double* __restrict__ gm) {
+    __shared__ double red[16][17];
+    const int nblk = (b + 15) >> 4, ntile = nblk * (nblk + 1) / 2;
+    const int idx = blockIdx.x / 16, eg = blockIdx.x % 16;
+    const int e = threadIdx.x & 15, pg = threadIdx.x >> 4;
+    const int ent = eg * 16 + e;  // entry of the 16 x 16 tile: row ent / 16, column ent % 16
+    double v = 0.0;
+    for (int p = pg; p < nparts; p += 16) v += part[((int64_t)p * ntile + idx) * 256 + ent];
+    red[pg][e] = v;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) sum += red[q][threadIdx.x];
+        int a = 0, rem = idx;
+        while (rem >= nblk - a) {
+            rem -= nblk - a;
+            a++;
+        }
+        const int i = 16 * a + (ent >> 4), j = 16 * (a + rem) + (ent & 15);
+        if (i < b && j < b) {
+            gm[i * b + j] = sum;
+            gm[j * b + i] = sum;
+        }
+    }
+}
+
+// T of the compact-WY form I - V T V^T of b reflectors (the reference's buildT, HouseholderQR.java:296-320,
+// restated as T^-1 = diag(1 / tau) + strictly-upper(V^T V), the inverse of the recurrence T(1:i-1, i) =
+// -tau_i T(1:i-1, 1:i-1) V(:, 1:i-1)^T v_i): G = V^T V and tau in, T (b x b row-major, upper) out. A
+// reflector with tau_i = 0 (H_i = I) takes no part: its row and column of T^-1 are e_i and T(i, i) = 0.
+// Dynamic LDS CQ_DYN.
+__global__ __launch_bounds__(CQ_NT) void wy_t_inverse(const double* __restrict__ gm, int b,
+                                                      const double* __restrict__ tau, double* __restrict__ tout) {
+    __shared__ double dinv[128];
+    __shared__ int zt[128];
+    extern __shared__ double dyn[];
+    double* S = dyn;
+    double* dblk = dyn + 128 * SLD;
+    const int t = threadIdx.x;
+    const int nb16 = (b + 15) & ~15, nblk = nb16 >> 4;
+    if (t < 128) {
+        const bool z = t < b && tau[t] == 0.0;
+        zt[t] = z ? 1 : 0;
+        dinv[t] = t < b ? (z ? 1.0 : tau[t]) : 1.0;  // 1 / U_T(t, t)
+    }
+    __syncthreads();
+    const int c = t & 127;
+    for (int r = t >> 7; r < nb16; r += 4) {
+        if (c >= nb16) break;
+        double v;
+        if (r < b && c < b) {
+            if (r == c) v = zt[r] ? 1.0 : 1.0 / tau[r];
+            else v = (r < c && !zt[r] && !zt[c]) ? gm[r * b + c] : 0.0;
+        } else {
+            v = r == c ? 1.0 : 0.0;
+        }
+        S[r * SLD + c] = v;
+    }
+    __syncthreads();
+    upper_inverse_blocked(S, dblk, dinv, nblk);
+    for (int r = t >> 7; r < b; r += 4)
+        if (c < b) tout[r * b + c] = (r <= c && !(r == c && zt[r])) ? S[r * SLD + c] : 0.0;
+}
+
 // The reference's tau = 0 case: column i of Y exactly zero below the diagonal (Y1 part in spec, Y2
 // rows b..mp-1 checked here) with a non-negative pivot: H_i = I, tau_i = 0 and R's row i keeps the
 // pivot's sign (the row is negated back). One workgroup per panel column; the last column of a panel
@@ -424,6 +589,32 @@ hipError_t launch_cqr_lu(const double* q, int64_t ldq, int b, const double* r, d
     if (b < 1 || b > 128 || ldq < b) return hipErrorInvalidValue;
     (void)hipFuncSetAttribute((const void*)cqr_lu, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
     hipLaunchKernelGGL(cqr_lu, dim3(1), dim3(CQ_NT), CQ_DYN, stream, q, ldq, b, r, ap, lda, tau, ui, spec);
+    return hipGetLastError();
+}
+
+int gram_parts(int64_t m) {
+    const int64_t p = (m + 127) / 128;
+    return (int)(p < 1 ? 1 : (p > 256 ? 256 : p));
+}
+
+size_t gram_work_doubles(int64_t m) { return (size_t)gram_parts(m) * 36 * 256; }
+
+hipError_t launch_gram(const double* x, int64_t ldx, int m, int b, double* part, double* g, hipStream_t stream) {
+    if (b < 1 || b > 128 || m < 1 || ldx < m) return hipErrorInvalidValue;
+    // Row ranges of whole 64-row chunks (16-byte aligned loads when X is), at most gram_parts(m) of them.
+    const int P0 = gram_parts(m);
+    const int per = (int)(((m + P0 - 1) / P0 + GR_ROWS - 1) / GR_ROWS * GR_ROWS);
+    const int P = (m + per - 1) / per;
+    hipLaunchKernelGGL(gram_partial, dim3(P), dim3(CQ_NT), 0, stream, x, ldx, m, b, per, part);
+    const int nblk = (b + 15) >> 4, ntile = nblk * (nblk + 1) / 2;
+    hipLaunchKernelGGL(gram_reduce, dim3((unsigned)(16 * ntile)), dim3(256), 0, stream, part, P, b, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_wy_t(const double* g, int b, const double* tau, double* T, hipStream_t stream) {
+    if (b < 1 || b > 128) return hipErrorInvalidValue;
+    (void)hipFuncSetAttribute((const void*)wy_t_inverse, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    hipLaunchKernelGGL(wy_t_inverse, dim3(1), dim3(CQ_NT), CQ_DYN, stream, g, b, tau, T);
     return hipGetLastError();
 }
 

@@ -98,10 +98,15 @@ struct ThreadCtx {
     // on ws_stream, and a call on another stream waits for it before touching the buffers.
     hipEvent_t ws_ev = nullptr;
     hipStream_t ws_stream = nullptr;
+    // Blocked QR: pinned host copy of a CholeskyQR2 panel's flag and the event that marks its arrival.
+    int* qr_flag = nullptr;
+    hipEvent_t qr_ev = nullptr;
 
     ~ThreadCtx() {
         if (!g_alive.load() || device < 0) return;
         if (pin) (void)hipHostFree(pin);
+        if (qr_flag) (void)hipHostFree(qr_flag);
+        if (qr_ev) (void)hipEventDestroy(qr_ev);
         for (int i = 0; i < NBUF; i++)
             if (buf[i]) (void)hipFree(buf[i]);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1556,22 +1561,23 @@ int gemm_on(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
     return run_gemm_device(la, lb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode, s, info, &kernel);
 }
 
-// V (as Vc = V^T, nb x mk) and T of the panel [k0, k0+nb) of a factored matrix.
+// V (as Vc = V^T, nb x mk) and T of the panel [k0, k0+nb) of a factored matrix; gpart holds
+// gram_work_doubles(m - k0) doubles.
 int panel_wy(const double* a, int64_t lda, int64_t m, int64_t k0, int64_t nb, const double* tau, double* vc,
-             double* g, double* t, hipStream_t s, jlc::TileInfo* info) {
+             double* g, double* t, double* gpart, hipStream_t s, jlc::TileInfo* info) {
     const int64_t mk = m - k0;
     JLC_HIP_CHECK(jlc::launch_build_vc(a, lda, (int)m, (int)k0, (int)nb, vc, s));
-    // G = V^T V: op(A) = Vc (row-major), op(B) = V = Vc^T (column-major view of Vc).
-    int rc = gemm_on(JLC_ROW, JLC_COL, nb, nb, mk, 1.0, vc, mk, vc, mk, 0.0, g, nb, JLC_MODE_FP64, s, info);
-    if (rc != JLC_OK) return rc;
-    JLC_HIP_CHECK(jlc::launch_build_t(g, (int)nb, tau + k0, t, s));
+    // G = V^T V, with Vc = V^T row-major, i.e. V column-major (ld mk).
+    JLC_HIP_CHECK(jlc::launch_gram(vc, mk, (int)mk, (int)nb, gpart, g, s));
+    JLC_HIP_CHECK(jlc::launch_wy_t(g, (int)nb, tau + k0, t, s));
+    info->launches += 4;
     return JLC_OK;
 }
 
 // Workspace of a CholeskyQR2 panel: b x b matrices (row-major, b <= nb), two m x nb column-major panels
 // (Q1, Q), the breakdown flag and the tau = 0 candidates.
 struct CqrWork {
-    double *g, *r1, *x1, *r, *x2, *ui, *q1, *q;
+    double *g, *r1, *x1, *r, *x2, *ui, *q1, *q, *gpart;
     int *flag, *spec;
 };
 
@@ -1588,26 +1594,27 @@ int panel_cqr(double* a, int64_t lda, int64_t m, int64_t k0, int64_t b, double* 
     double* ap = a + k0 * lda + k0;
     const int bi = (int)b;
     int rc;
-    if ((rc = gemm_on(JLC_ROW, JLC_COL, b, b, mp, 1.0, ap, lda, ap, lda, 0.0, w.g, b, JLC_MODE_FP64, s, info)) !=
-        JLC_OK)
-        return rc;
+    JLC_HIP_CHECK(jlc::launch_gram(ap, lda, (int)mp, bi, w.gpart, w.g, s));
     JLC_HIP_CHECK(jlc::launch_cqr_chol(w.g, bi, 1, nullptr, w.r1, w.x1, w.flag, s));
     // Q1^T (b x mp, row-major) = X1^T Ap^T
     if ((rc = gemm_on(JLC_COL, JLC_ROW, b, mp, b, 1.0, w.x1, b, ap, lda, 0.0, w.q1, mp, JLC_MODE_FP64, s, info)) !=
         JLC_OK)
         return rc;
-    if ((rc = gemm_on(JLC_ROW, JLC_COL, b, b, mp, 1.0, w.q1, mp, w.q1, mp, 0.0, w.g, b, JLC_MODE_FP64, s, info)) !=
-        JLC_OK)
-        return rc;
+    JLC_HIP_CHECK(jlc::launch_gram(w.q1, mp, (int)mp, bi, w.gpart, w.g, s));
     JLC_HIP_CHECK(jlc::launch_cqr_chol(w.g, bi, 2, w.r1, w.r, w.x2, w.flag, s));
-    info->launches += 2;
-    int hflag = -1;
-    JLC_HIP_CHECK(hipMemcpyAsync(&hflag, w.flag, sizeof(int), hipMemcpyDeviceToHost, s));
-    JLC_HIP_CHECK(hipStreamSynchronize(s));
-    if (hflag != 0) return JLC_OK;
+    info->launches += 6;
+    // The flag is read back while Q = Q1 R2^-1 (workspace only, harmless if the panel falls back) runs.
+    ThreadCtx& ctx = t_ctx;
+    if (!ctx.qr_flag) JLC_HIP_CHECK(hipHostMalloc((void**)&ctx.qr_flag, sizeof(int), hipHostMallocDefault));
+    if (!ctx.qr_ev) JLC_HIP_CHECK(hipEventCreateWithFlags(&ctx.qr_ev, hipEventDisableTiming));
+    *ctx.qr_flag = -1;
+    JLC_HIP_CHECK(hipMemcpyAsync(ctx.qr_flag, w.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipEventRecord(ctx.qr_ev, s));
     if ((rc = gemm_on(JLC_COL, JLC_ROW, b, mp, b, 1.0, w.x2, b, w.q1, mp, 0.0, w.q, mp, JLC_MODE_FP64, s, info)) !=
         JLC_OK)
         return rc;
+    JLC_HIP_CHECK(hipEventSynchronize(ctx.qr_ev));
+    if (*ctx.qr_flag != 0) return JLC_OK;
     JLC_HIP_CHECK(jlc::launch_cqr_lu(w.q, mp, bi, w.r, ap, lda, tau + k0, w.ui, w.spec, s));
     // Y2^T (b x (mp - b), rows of ap below the top block, ld lda) = U^-T Q_bottom^T
     if ((rc = gemm_on(JLC_COL, JLC_ROW, b, mp - b, b, 1.0, w.ui, b, w.q + b, mp, 0.0, ap + b, lda, JLC_MODE_FP64, s,
@@ -1651,8 +1658,9 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     const auto cqr_panel = [&](int64_t k0, int64_t b) {
         return cqr_min > 0 && m - k0 >= cqr_min && m - k0 >= 2 * b;
     };
-    DevBuf cqp, cqs;
+    DevBuf cqp, cqs, gpart;
     CqrWork cw{};
+    if (e == hipSuccess) e = gpart.alloc(jlc::gram_work_doubles(m) * 8);
     if (e == hipSuccess && cqr_panel(0, std::min(nb, kmax))) {
         e = cqp.alloc((size_t)2 * m * nb * 8);
         if (e == hipSuccess) e = cqs.alloc((size_t)6 * nb * nb * 8 + (size_t)(nb + 1) * sizeof(int));
@@ -1668,6 +1676,7 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
             cw.ui = sm + 5 * nb * nb;
             cw.flag = reinterpret_cast<int*>(sm + 6 * nb * nb);
             cw.spec = cw.flag + 1;
+            cw.gpart = gpart.d();
         }
     }
     if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "QR workspace allocation failed: %s", hipGetErrorString(e));
@@ -1710,7 +1719,7 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         }
         if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi + 1], s));
         if (nt <= 0) continue;
-        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
+        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), gpart.d(), s, &info)) != JLC_OK) return rc;
         // Transposed compact-WY update of C = A[k0:m, k1:n] held as C^T (nt x mk, row-major, ld = lda):
         // W^T = C^T V; W^T <- W^T T; C^T -= W^T V^T   (C <- (I - V T^T V^T) C).
         double* ct = a + k1 * lda + k0;
@@ -1755,8 +1764,9 @@ namespace {
 int apply_q_panels(int64_t m, int64_t kk, const double* a, int64_t lda, const double* tau, int64_t nb, double* x,
                    int64_t ldx, int64_t nx, bool identity_start, hipStream_t s) {
     if (kk <= 0 || nx <= 0) return JLC_OK;
-    DevBuf vc, g, t, y1, y2;
+    DevBuf vc, g, t, y1, y2, gpart;
     hipError_t e = vc.alloc((size_t)nb * m * 8);
+    if (e == hipSuccess) e = gpart.alloc(jlc::gram_work_doubles(m) * 8);
     if (e == hipSuccess) e = g.alloc((size_t)nb * nb * 8);
     if (e == hipSuccess) e = t.alloc((size_t)nb * nb * 8);
     if (e == hipSuccess) e = y1.alloc((size_t)nx * nb * 8);
@@ -1770,7 +1780,7 @@ int apply_q_panels(int64_t m, int64_t kk, const double* a, int64_t lda, const do
         const int64_t nbk = k1 - k0, mk = m - k0;
         const int64_t c0 = identity_start ? k0 : 0, nq = nx - c0;
         if (nq <= 0) continue;
-        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), s, &info)) != JLC_OK) return rc;
+        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), gpart.d(), s, &info)) != JLC_OK) return rc;
         // X <- (I - V T V^T) X on rows k0..m-1:  X^T -= ((X^T V) T^T) V^T  (X^T row-major, ld = ldx).
         double* xt = x + c0 * ldx + k0;
         if ((rc = gemm_on(JLC_ROW, JLC_COL, nq, nbk, mk, 1.0, xt, ldx, vc.d(), mk, 0.0, y1.d(), nbk, JLC_MODE_FP64, s,
