@@ -43,6 +43,7 @@ constexpr int CQ_NT = 512;           // 8 wavefronts
 constexpr int CQ_W = CQ_NT / 64;
 constexpr int SLD = 129;             // LDS matrix: 128 x SLD doubles, row-major (odd: conflict-free columns)
 constexpr int CQ_DYN = (128 * SLD + 8 * 256) * 8;  // dynamic LDS: the matrix and 8 diagonal-block inverses
+constexpr int GR_MAXT = 5;   // upper 16 x 16 tiles per wavefront: 36 tiles of a 128 x 128 matrix over 8
 
 // 1/d and 1/sqrt(d) from the hardware estimates refined by Newton steps (within an ulp or two; the
 // IEEE division and square root sequences would sit on every elimination step's critical path).
@@ -132,9 +133,15 @@ __device__ void upper_inverse_blocked(double* S, double* dblk, const double* din
         const int cb = ib + 1 + w;
         d4 res = {0.0, 0.0, 0.0, 0.0};
         if (cb < nblk) {
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-            for (int lb = ib + 1; lb <= cb; lb++)  // X(cb, cb)'s strictly lower part may hold other data
-                acc = tile_mma(acc, S + 16 * ib * SLD + 16 * lb, SLD, S + 16 * lb * SLD + 16 * cb, SLD, 1.0, lb == cb);
+            // two independent accumulator chains (alternate lb) so consecutive MFMAs do not wait on each other
+            d4 a0 = {0.0, 0.0, 0.0, 0.0}, a1 = {0.0, 0.0, 0.0, 0.0};
+            for (int lb = ib + 1; lb <= cb; lb += 2) {  // X(cb, cb)'s strictly lower part may hold other data
+                a0 = tile_mma(a0, S + 16 * ib * SLD + 16 * lb, SLD, S + 16 * lb * SLD + 16 * cb, SLD, 1.0, lb == cb);
+                if (lb + 1 <= cb)
+                    a1 = tile_mma(a1, S + 16 * ib * SLD + 16 * (lb + 1), SLD, S + 16 * (lb + 1) * SLD + 16 * cb, SLD,
+                                  1.0, lb + 1 == cb);
+            }
+            const d4 acc = a0 + a1;
             // res = -Dinv(ib) * acc: acc's register s is the B fragment of product step s.
             const int i16 = l & 15, g = l >> 4;
             const double* di = dblk + ib * 256;
@@ -148,84 +155,227 @@ __device__ void upper_inverse_blocked(double* S, double* dblk, const double* din
     }
 }
 
+// acc = sum_{lb = ib..cb} A(ib, lb) * R1(lb, cb) with A from S and R1 row-major b x b in global memory
+// (zero past b): the R1 fragments of the whole tile row are loaded before the MFMA chain.
+__device__ __forceinline__ d4 tile_times_r1(const double* S, int ib, int cb, const double* __restrict__ r1, int b,
+                                            d4* r1_tile = nullptr) {
+    const int l = threadIdx.x & 63, i16 = l & 15, g = l >> 4;
+    double bv[8][4];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int lb = ib + q;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int kr = 16 * lb + g + 4 * s, col = 16 * cb + i16;
+            bv[q][s] = (lb <= cb && kr < b && col < b) ? r1[kr * b + col] : 0.0;
+        }
+    }
+    // R1(ib, cb) itself: fragment step s of lb = ib holds row 16 ib + g + 4s, the accumulator layout
+    if (r1_tile) *r1_tile = d4{bv[0][0], bv[0][1], bv[0][2], bv[0][3]};
+    d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int lb = ib + q;
+        if (lb <= cb) {
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                acc[q & 1] = mfma(acc[q & 1], S[(16 * ib + i16) * SLD + 16 * lb + g + 4 * s], bv[q][s]);
+        }
+    }
+    return acc[0] + acc[1];
+}
+
+// Upper tile (ib, cb), ib <= cb, of the idx-th entry of the row-by-row upper tile list of nblk blocks.
+__device__ __forceinline__ void upper_tile(int idx, int nblk, int& ib, int& cb) {
+    int a = 0;
+    while (idx >= nblk - a) {
+        idx -= nblk - a;
+        a++;
+    }
+    ib = a;
+    cb = a + idx;
+}
+
 // pass 1: R1 = chol(G) (upper, R1^T R1 = G), X1 = R1^-1, both b x b row-major; flag = 1 on a pivot that
 // is not positive and finite (a plain store: pass 1 starts the panel's flag).
-// pass 2: R2 = chol(G) as above plus max|G - I| > 2^-8 (flag |= 2); r_out = R2 * R1 (R1 row-major from
-// r1) stored column-major, X2 = R2^-1 row-major.
+// pass 2: flag |= 2 when max|G - I| > 2^-8; R2 = chol(G), r_out = R2 * R1 (R1 row-major from r1) stored
+// column-major, X2 = R2^-1 row-major. When G = I + E is close to the identity (max|E| <= 2^-26, the
+// usual case: E is the first pass's loss of orthogonality, about kappa^2 u), R2 = I + P is formed
+// without elimination: P = U(E - P^T P) (U keeps the strict upper triangle and half the diagonal) from
+// P = U(E) (one step, dropped when max|E| <= 2^-40, where P^T P is below 2^-73), and
+// X2 = I - P + P^2 (P^2 dropped likewise); the truncation is below 2^-60 in both tiers.
 // G is b x b row-major (symmetric; only its upper triangle is read); padded with the identity to a
-// multiple of 16. Right-looking blocked Cholesky: for each 16-row block, 16 steps (one barrier each)
-// factor the block row across all columns, then the trailing upper tiles take the block's rank-16
-// update on FP64 MFMA.
+// multiple of 16. Elimination path: right-looking blocked Cholesky; per 16-row block, 16 steps (one
+// barrier each, the block row's updates on the first four wavefronts) factor the block row, then the
+// trailing upper tiles take the block's rank-16 update on FP64 MFMA; X by upper_inverse_blocked.
 __global__ __launch_bounds__(CQ_NT) void cqr_chol(const double* __restrict__ gm, int b, int pass,
                                                   const double* __restrict__ r1, double* __restrict__ r_out,
                                                   double* __restrict__ x_out, int* __restrict__ flag) {
     __shared__ double rowb[2][128];
     __shared__ double dinv[128];
+    __shared__ double wmax[CQ_W];
     __shared__ int bad;
     extern __shared__ double dyn[];
     double* S = dyn;
     double* dblk = dyn + 128 * SLD;
-    const int t = threadIdx.x, w = t >> 6;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, i16 = l & 15, g = l >> 4;
     const int nb16 = (b + 15) & ~15, nblk = nb16 >> 4;
-    const int c = t & 127, h = t >> 7;  // panel role: column c, rows h + 4i of the block
+    const int c = t & 127;
     if (t == 0) bad = 0;
     double dev = 0.0;
-    for (int r = h; r < nb16; r += 4) {
-        if (c >= nb16) break;
-        double v;
-        if (r < b && c < b) {
-            v = r <= c ? gm[r * b + c] : 0.0;
-            if (r <= c) dev = fmax(dev, fabs(v - (r == c ? 1.0 : 0.0)));
-        } else {
-            v = r == c ? 1.0 : 0.0;
+    {
+        // all 32 loads of a thread in flight at once, then the LDS stores
+        double v[32];
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int r = (t >> 7) + 4 * q;
+            v[q] = (r < b && c < b && r <= c) ? gm[r * b + c] : (r == c ? 1.0 : 0.0);
         }
-        S[r * SLD + c] = v;
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int r = (t >> 7) + 4 * q;
+            if (r < b && c < b && r <= c) dev = fmax(dev, fabs(v[q] - (r == c ? 1.0 : 0.0)));
+            if (r < nb16 && c < nb16) S[r * SLD + c] = v[q];
+        }
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dev = fmax(dev, __shfl_xor(dev, off, 64));  // NaN-free: fmax
+    if (l == 0) wmax[w] = dev;
     __syncthreads();
-    if (pass == 2 && !(dev <= 0x1p-8)) bad = 1;
+    double devmax = 0.0;
+#pragma unroll
+    for (int q = 0; q < CQ_W; q++) devmax = fmax(devmax, wmax[q]);
+    // a NaN entry leaves dev unchanged through fmax: count it as a failure
+    bool nan_in = false;
+    for (int r = t >> 7; r < b; r += CQ_NT / 128)
+        if (c < b && r <= c && !(S[r * SLD + c] == S[r * SLD + c])) nan_in = true;
+    if (pass == 2 && (!(devmax <= 0x1p-8) || nan_in)) bad = 1;
+    if (pass == 2 && devmax <= 0x1p-26) {
+        const bool second = devmax > 0x1p-40;
+        // S <- P = U(E) on the upper triangle (the strictly lower part is zero).
+        for (int r = t >> 7; r < nb16; r += CQ_NT / 128)
+            if (c < nb16 && r <= c) S[r * SLD + c] = r == c ? 0.5 * (S[r * SLD + c] - 1.0) : S[r * SLD + c];
+        __syncthreads();
+        const int ntile = nblk * (nblk + 1) / 2;
+        if (second) {
+            // P <- U(E - P^T P) with E = P + P^T: off the diagonal P - (P^T P), on it P - (P^T P) / 2.
+            d4 acc[GR_MAXT];
+#pragma unroll
+            for (int q = 0; q < GR_MAXT; q++) {
+                acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+                const int idx = w + CQ_W * q;
+                if (idx < ntile) {
+                    int ib, cb;
+                    upper_tile(idx, nblk, ib, cb);
+                    for (int lb = 0; lb <= ib; lb++)
+                        acc[q] = tile_mma_tn(acc[q], S + 16 * lb * SLD + 16 * ib, SLD, S + 16 * lb * SLD + 16 * cb,
+                                             SLD, 1.0);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < GR_MAXT; q++) {
+                const int idx = w + CQ_W * q;
+                if (idx < ntile) {
+                    int ib, cb;
+                    upper_tile(idx, nblk, ib, cb);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = 16 * ib + g + 4 * r, col = 16 * cb + i16;
+                        if (row < col) S[row * SLD + col] -= acc[q][r];
+                        else if (row == col) S[row * SLD + col] -= 0.5 * acc[q][r];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // X2 = I - P (+ P^2), R = R1 + P R1, upper tiles; lower parts zero.
+        for (int idx = w; idx < ntile; idx += CQ_W) {
+            int ib, cb;
+            upper_tile(idx, nblk, ib, cb);
+            d4 pp = {0.0, 0.0, 0.0, 0.0}, pq = {0.0, 0.0, 0.0, 0.0};
+            if (second) {
+                for (int lb = ib; lb <= cb; lb += 2) {
+                    pp = tile_mma(pp, S + 16 * ib * SLD + 16 * lb, SLD, S + 16 * lb * SLD + 16 * cb, SLD, 1.0);
+                    if (lb + 1 <= cb)
+                        pq = tile_mma(pq, S + 16 * ib * SLD + 16 * (lb + 1), SLD, S + 16 * (lb + 1) * SLD + 16 * cb,
+                                      SLD, 1.0);
+                }
+                pp += pq;
+            }
+            d4 r1t;
+            const d4 pr = tile_times_r1(S, ib, cb, r1, b, &r1t);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = 16 * ib + g + 4 * r, col = 16 * cb + i16;
+                if (row < b && col < b) {
+                    const double pv = row <= col ? S[row * SLD + col] : 0.0;
+                    x_out[row * b + col] = (row == col ? 1.0 : 0.0) - pv + pp[r];
+                    r_out[col * b + row] = r1t[r] + pr[r];
+                }
+            }
+        }
+        for (int r = t >> 7; r < b; r += CQ_NT / 128)
+            if (c < r) {
+                r_out[c * b + r] = 0.0;
+                x_out[r * b + c] = 0.0;
+            }
+        __syncthreads();
+        if (t == 0 && bad) *flag |= 2;
+        return;
+    }
+    // Elimination path. Panel role (first four wavefronts): column c, rows j0 + h + 2i (i < 8) of the block.
+    const int h = (t >> 7) & 1;
+    const bool panel_wave = t < 256;
     for (int jb = 0; jb < nblk; jb++) {
         const int j0 = 16 * jb;
-        const bool act = c < nb16;
-        double pv[4];
+        // a wavefront whose columns all lie left of the block has nothing to update (those entries are 0)
+        const bool work = panel_wave && 64 * (w & 1) + 63 >= j0 && 64 * (w & 1) < nb16;
+        double pv[8], cm[8];
 #pragma unroll
-        for (int i = 0; i < 4; i++) pv[i] = act ? S[(j0 + h + 4 * i) * SLD + c] : 0.0;
+        for (int i = 0; i < 8; i++) {
+            const int r = j0 + h + 2 * i;
+            pv[i] = (work && c < nb16) ? S[r * SLD + c] : 0.0;
+            cm[i] = c >= r ? 1.0 : 0.0;
+        }
 #pragma unroll
         for (int jj = 0; jj < 16; jj++) {
             const int j = j0 + jj, p = jj & 1;
-            const bool own = h == (jj & 3);
-            if (own) rowb[p][c] = pv[jj >> 2];
+            const bool own = work && h == (jj & 1);
+            if (own) rowb[p][c] = pv[jj >> 1];
             __syncthreads();
-            const double d = rowb[p][j];
-            if (t == 0 && !(d > 0.0 && d <= 1.7976931348623157e308)) bad = 1;
-            const double rk = rowb[p][c], id = rcp_nr(d);
+            if (work) {
+                // every LDS read of the step issued before the first use; updates are plain FMAs
+                const double d = rowb[p][j];
+                const double rk = rowb[p][c];
+                double rr[8];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int r = j0 + h + 4 * i;
-                if (4 * i + h > jj) {  // uniform in a wavefront
-                    const double rr = rowb[p][r];
-                    if (c >= r) pv[i] = fma(-(rr * id), rk, pv[i]);
+                for (int i = 0; i < 8; i++) rr[i] = rowb[p][j0 + h + 2 * i];
+                if (t == 0 && !(d > 0.0 && d <= 1.7976931348623157e308)) bad = 1;
+                const double f = rk * rcp_nr(d);
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (2 * i + h > jj) pv[i] = fma(-rr[i], f * cm[i], pv[i]);  // condition uniform per wavefront
+                if (own) {
+                    const double rs = rsq_nr(d);
+                    const double rv = c >= j ? rk * rs : 0.0;
+                    pv[jj >> 1] = rv;
+                    if (c == j) dinv[j] = rcp_nr(rv);
                 }
             }
-            if (own) {
-                const double rs = rsq_nr(d);
-                pv[jj >> 2] = c >= j ? rk * rs : 0.0;
-                if (c == j) dinv[j] = rcp_nr(pv[jj >> 2]);
-            }
         }
-        if (act) {
+        if (work && c < nb16) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) S[(j0 + h + 4 * i) * SLD + c] = pv[i];
+            for (int i = 0; i < 8; i++) S[(j0 + h + 2 * i) * SLD + c] = pv[i];
         }
         __syncthreads();
         // Trailing upper tiles (kb, lb), jb < kb <= lb: S(kb, lb) -= R(jb, kb)^T R(jb, lb).
         const int nt = nblk - jb - 1;
         for (int idx = w; idx < nt * (nt + 1) / 2; idx += CQ_W) {
-            int kb = jb + 1, rem = idx;
-            while (rem >= nblk - kb) {
-                rem -= nblk - kb;
-                kb++;
-            }
-            const int lb = kb + rem;
+            int kb, lb;
+            upper_tile(idx, nt, kb, lb);
+            kb += jb + 1;
+            lb += jb + 1;
             double* ct = S + 16 * kb * SLD + 16 * lb;
             d4 acc = tile_load(ct, SLD);
             acc = tile_mma_tn(acc, S + j0 * SLD + 16 * kb, SLD, S + j0 * SLD + 16 * lb, SLD, -1.0);
@@ -234,40 +384,26 @@ __global__ __launch_bounds__(CQ_NT) void cqr_chol(const double* __restrict__ gm,
         __syncthreads();
     }
     if (pass == 1) {
-        for (int r = h; r < b; r += 4)
+        for (int r = t >> 7; r < b; r += CQ_NT / 128)
             if (c < b) r_out[r * b + c] = S[r * SLD + c];
     } else {
-        // R = R2 R1 tile by tile (upper tiles; R1 from global, zero past b), stored column-major.
-        const int l = t & 63, i16 = l & 15, g = l >> 4;
+        // R = R2 R1 tile by tile (upper tiles), stored column-major; zeros below the diagonal.
         for (int idx = w; idx < nblk * (nblk + 1) / 2; idx += CQ_W) {
-            int ib = 0, rem = idx;
-            while (rem >= nblk - ib) {
-                rem -= nblk - ib;
-                ib++;
-            }
-            const int cb = ib + rem;
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-            for (int lb = ib; lb <= cb; lb++) {
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    const int kr = 16 * lb + g + 4 * s, col = 16 * cb + i16;
-                    const double bv = (kr < b && col < b) ? r1[kr * b + col] : 0.0;
-                    acc = mfma(acc, S[(16 * ib + i16) * SLD + 16 * lb + g + 4 * s], bv);
-                }
-            }
+            int ib, cb;
+            upper_tile(idx, nblk, ib, cb);
+            const d4 acc = tile_times_r1(S, ib, cb, r1, b);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = 16 * ib + g + 4 * r, col = 16 * cb + i16;
                 if (row < b && col < b) r_out[col * b + row] = acc[r];
             }
         }
-        // Zeros below the diagonal of the column-major R.
-        for (int r = h; r < b; r += 4)
+        for (int r = t >> 7; r < b; r += CQ_NT / 128)
             if (c < r) r_out[c * b + r] = 0.0;
     }
     __syncthreads();
     upper_inverse_blocked(S, dblk, dinv, nblk);
-    for (int r = h; r < b; r += 4)
+    for (int r = t >> 7; r < b; r += CQ_NT / 128)
         if (c < b) x_out[r * b + c] = r <= c ? S[r * SLD + c] : 0.0;
     if (t == 0) {
         if (pass == 1) *flag = bad ? 1 : 0;
@@ -303,55 +439,70 @@ __global__ __launch_bounds__(CQ_NT) void cqr_lu(const double* __restrict__ qm, i
     // Coalesced load of Q's top block (column contiguous in memory), identity past b.
     {
         const int r = t & 127;
-        for (int c = t >> 7; c < nb16; c += 4)
-            if (r < nb16) S[r * SLD + c] = (r < b && c < b) ? qm[(int64_t)c * ldq + r] : (r == c ? 1.0 : 0.0);
+        double v[32];
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int c = (t >> 7) + 4 * q;
+            v[q] = (r < b && c < b) ? qm[(int64_t)c * ldq + r] : (r == c ? 1.0 : 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int c = (t >> 7) + 4 * q;
+            if (r < nb16 && c < nb16) S[r * SLD + c] = v[q];
+        }
     }
     if (t < 128) nz[t] = 0;
     __syncthreads();
-    const int cc = t & 15, r0 = t >> 4;  // block-column role: column j0 + cc, rows j0 + r0 + 32e
+    // Block-column role (first four wavefronts): column j0 + cc, rows j0 + r0 + 16e, e < 8.
+    const int cc = t & 15, r0 = (t >> 4) & 15;
+    const bool panel_wave = t < 256;
     for (int jb = 0; jb < nblk; jb++) {
-        const int j0 = 16 * jb;
-        double pv[4];
+        const int j0 = 16 * jb, ne = nblk - jb;  // row groups e < ne lie inside the matrix
+        double pv[8];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int r = j0 + r0 + 32 * e;
-            pv[e] = r < nb16 ? S[r * SLD + j0 + cc] : 0.0;
-        }
+        for (int e = 0; e < 8; e++) pv[e] = (panel_wave && e < ne) ? S[(j0 + r0 + 16 * e) * SLD + j0 + cc] : 0.0;
 #pragma unroll
         for (int ii = 0; ii < 16; ii++) {
             const int i = j0 + ii, p = ii & 1;
-            if (r0 == ii) rowb[p][cc] = pv[0];
-            if (cc == ii) {
+            if (panel_wave) {
+                if (r0 == ii) rowb[p][cc] = pv[0];
+                if (cc == ii) {
 #pragma unroll
-                for (int e = 0; e < 4; e++) colb[p][r0 + 32 * e] = pv[e];
-            }
-            __syncthreads();
-            const double cv = rowb[p][ii];
-            const double d = -copysign(1.0, cv);
-            const double u = cv - d;
-            const double iu = rcp_nr(u);
-            const double rk = rowb[p][cc];
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int r = j0 + r0 + 32 * e;
-                if (r > i && r < nb16) {
-                    const double lv = colb[p][r0 + 32 * e] * iu;
-                    if (cc > ii) pv[e] = fma(-lv, rk, pv[e]);
-                    else if (cc == ii) pv[e] = lv;
+                    for (int e = 0; e < 8; e++) colb[p][r0 + 16 * e] = pv[e];
                 }
             }
-            if (r0 == ii && cc == ii) {
-                pv[0] = u;
-                dinv[i] = iu;
-                dsg[i] = d;
-                cpv[i] = cv;
-                upv[i] = u;
+            __syncthreads();
+            if (panel_wave) {
+                const double cv = rowb[p][ii];
+                const double rk = rowb[p][cc];
+                double col[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) col[e] = colb[p][r0 + 16 * e];
+                const double d = -copysign(1.0, cv);
+                const double u = cv - d;
+                const double iu = rcp_nr(u);
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    if (e < ne) {  // uniform
+                        const double lv = col[e] * iu;
+                        const double upd = fma(-lv, rk, pv[e]);
+                        const bool below = e > 0 || r0 > ii;
+                        pv[e] = below ? (cc > ii ? upd : (cc == ii ? lv : pv[e])) : pv[e];
+                    }
+                }
+                if (r0 == ii && cc == ii) {
+                    pv[0] = u;
+                    dinv[i] = iu;
+                    dsg[i] = d;
+                    cpv[i] = cv;
+                    upv[i] = u;
+                }
             }
         }
+        if (panel_wave) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int r = j0 + r0 + 32 * e;
-            if (r < nb16) S[r * SLD + j0 + cc] = pv[e];
+            for (int e = 0; e < 8; e++)
+                if (e < ne) S[(j0 + r0 + 16 * e) * SLD + j0 + cc] = pv[e];
         }
         __syncthreads();
         // Block row: U(i, c) = A(i, c) - sum_{j0 <= q < i} L(i, q) U(q, c), one thread per column c.
@@ -401,8 +552,17 @@ __global__ __launch_bounds__(CQ_NT) void cqr_lu(const double* __restrict__ qm, i
     // Top block of the factored panel: D R on and above the diagonal (R column-major), Y1 below.
     {
         const int r = t & 127;
-        for (int c = t >> 7; c < b; c += 4)
-            if (r < b) ap[(int64_t)c * lda + r] = r > c ? S[r * SLD + c] : dsg[r] * rr[c * b + r];
+        double v[32];
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int c = (t >> 7) + 4 * q;
+            v[q] = (r < b && c < b && r <= c) ? rr[c * b + r] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int c = (t >> 7) + 4 * q;
+            if (r < b && c < b) ap[(int64_t)c * lda + r] = r > c ? S[r * SLD + c] : dsg[r] * v[q];
+        }
     }
 }
 
@@ -413,7 +573,6 @@ __global__ __launch_bounds__(CQ_NT) void cqr_lu(const double* __restrict__ qm, i
 // triangles of G (b x b, row-major).
 constexpr int GR_ROWS = 64;
 constexpr int GR_LDC = GR_ROWS + 1;
-constexpr int GR_MAXT = 5;  // tiles per wavefront: 36 upper tiles of a 128 x 128 Gram over 8 wavefronts
 
 __global__ __launch_bounds__(CQ_NT) void gram_partial(const double* __restrict__ x, int64_t ldx, int m, int b,
                                                       int per, double* __restrict__ part) {
@@ -539,16 +698,26 @@ __global__ __launch_bounds__(CQ_NT) void wy_t_inverse(const double* __restrict__
     }
     __syncthreads();
     const int c = t & 127;
-    for (int r = t >> 7; r < nb16; r += 4) {
-        if (c >= nb16) break;
-        double v;
-        if (r < b && c < b) {
-            if (r == c) v = zt[r] ? 1.0 : 1.0 / tau[r];
-            else v = (r < c && !zt[r] && !zt[c]) ? gm[r * b + c] : 0.0;
-        } else {
-            v = r == c ? 1.0 : 0.0;
+    {
+        double v[32];
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int r = (t >> 7) + 4 * q;
+            v[q] = (r < c && c < b) ? gm[r * b + c] : 0.0;
         }
-        S[r * SLD + c] = v;
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int r = (t >> 7) + 4 * q;
+            if (r >= nb16 || c >= nb16) continue;
+            double x;
+            if (r < b && c < b) {
+                if (r == c) x = zt[r] ? 1.0 : 1.0 / tau[r];
+                else x = (r < c && !zt[r] && !zt[c]) ? v[q] : 0.0;
+            } else {
+                x = r == c ? 1.0 : 0.0;
+            }
+            S[r * SLD + c] = x;
+        }
     }
     __syncthreads();
     upper_inverse_blocked(S, dblk, dinv, nblk);
@@ -579,7 +748,9 @@ __global__ __launch_bounds__(1024) void cqr_special(double* __restrict__ ap, int
 hipError_t launch_cqr_chol(const double* g, int b, int pass, const double* r1, double* r_out, double* x_out,
                            int* flag, hipStream_t stream) {
     if (b < 1 || b > 128 || (pass != 1 && pass != 2)) return hipErrorInvalidValue;
-    (void)hipFuncSetAttribute((const void*)cqr_chol, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)cqr_chol, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    if (attr != hipSuccess) return attr;
     hipLaunchKernelGGL(cqr_chol, dim3(1), dim3(CQ_NT), CQ_DYN, stream, g, b, pass, r1, r_out, x_out, flag);
     return hipGetLastError();
 }
@@ -587,7 +758,9 @@ hipError_t launch_cqr_chol(const double* g, int b, int pass, const double* r1, d
 hipError_t launch_cqr_lu(const double* q, int64_t ldq, int b, const double* r, double* ap, int64_t lda, double* tau,
                          double* ui, int* spec, hipStream_t stream) {
     if (b < 1 || b > 128 || ldq < b) return hipErrorInvalidValue;
-    (void)hipFuncSetAttribute((const void*)cqr_lu, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)cqr_lu, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    if (attr != hipSuccess) return attr;
     hipLaunchKernelGGL(cqr_lu, dim3(1), dim3(CQ_NT), CQ_DYN, stream, q, ldq, b, r, ap, lda, tau, ui, spec);
     return hipGetLastError();
 }
@@ -613,7 +786,9 @@ hipError_t launch_gram(const double* x, int64_t ldx, int m, int b, double* part,
 
 hipError_t launch_wy_t(const double* g, int b, const double* tau, double* T, hipStream_t stream) {
     if (b < 1 || b > 128) return hipErrorInvalidValue;
-    (void)hipFuncSetAttribute((const void*)wy_t_inverse, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)wy_t_inverse, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
+    if (attr != hipSuccess) return attr;
     hipLaunchKernelGGL(wy_t_inverse, dim3(1), dim3(CQ_NT), CQ_DYN, stream, g, b, tau, T);
     return hipGetLastError();
 }
