@@ -94,7 +94,8 @@ typedef struct jlc_stats {
     int32_t bits_a;     /* CRT form: norm bits of op(A) rows, ||a'|| <= 2^bits_a + sqrt(k)/2 (0 otherwise) */
     int32_t bits_b;     /* CRT form: norm bits of op(B) columns */
     double main_ms;     /* device time of the dominant kernel alone (the GEMM / GEMV launch; for
-                           jlc_dgeqrf_dev the panel kernels, summed over the panels) */
+                           jlc_dgeqrf_dev the trailing-update products C -= W2^T V^T, summed over
+                           the panels) */
 } jlc_stats;
 
 /* ---- device detection and lifetime (analogue of CudaSupport / CudaContext,

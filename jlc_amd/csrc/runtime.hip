@@ -1681,7 +1681,8 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     }
     if (e != hipSuccess) return fail(JLC_ERR_ALLOC, "QR workspace allocation failed: %s", hipGetErrorString(e));
     jlc::TileInfo info;
-    // With stats, an event pair around each panel's launches: main_ms is the panel kernels' device time.
+    // With stats, an event pair around each panel's last trailing-update product (C^T -= W2^T V^T, the
+    // dominant kernel): main_ms is its device time summed over the panels.
     struct Events {
         std::vector<hipEvent_t> ev;
         ~Events() {
@@ -1700,11 +1701,11 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
     }
     int cqr_panels = 0, col_panels = 0;
+    std::vector<size_t> timed;  // event pairs recorded (panels with a trailing update)
     for (int64_t k0 = 0; k0 < kmax; k0 += nb) {
         const int64_t k1 = k0 + nb < kmax ? k0 + nb : kmax;
         const int64_t nbk = k1 - k0, mk = m - k0, nt = n - k1;
         const size_t pi = (size_t)(2 * (k0 / nb));
-        if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi], s));
         bool panel_done = false;
         if (cw.q && cqr_panel(k0, nbk) &&
             (rc = panel_cqr(a, lda, m, k0, nbk, tau, cw, s, &info, &panel_done)) != JLC_OK)
@@ -1717,7 +1718,6 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         } else {
             cqr_panels++;
         }
-        if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi + 1], s));
         if (nt <= 0) continue;
         if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), gpart.d(), s, &info)) != JLC_OK) return rc;
         // Transposed compact-WY update of C = A[k0:m, k1:n] held as C^T (nt x mk, row-major, ld = lda):
@@ -1729,9 +1729,14 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         if ((rc = gemm_on(JLC_ROW, JLC_ROW, nt, nbk, nbk, 1.0, w1.d(), nbk, t.d(), nbk, 0.0, w2.d(), nbk,
                           JLC_MODE_FP64, s, &info)) != JLC_OK)
             return rc;
+        if (stats) {
+            JLC_HIP_CHECK(hipEventRecord(pev.ev[pi], s));
+            timed.push_back(pi);
+        }
         if ((rc = gemm_on(JLC_ROW, JLC_ROW, nt, mk, nbk, -1.0, w2.d(), nbk, vc.d(), mk, 1.0, ct, lda, mode, s,
                           &info)) != JLC_OK)
             return rc;
+        if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi + 1], s));
     }
     double kms = 0.0;
     if (stats) {
@@ -1740,7 +1745,7 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         float f = 0.0f;
         JLC_HIP_CHECK(hipEventElapsedTime(&f, t_ctx.ev0, t_ctx.ev1));
         kms = f;
-        for (size_t q = 0; q + 1 < pev.ev.size(); q += 2) {
+        for (size_t q : timed) {
             float pf = 0.0f;
             JLC_HIP_CHECK(hipEventElapsedTime(&pf, pev.ev[q], pev.ev[q + 1]));
             info.main_ms += pf;

@@ -20,7 +20,7 @@ def test_traffic_summaries_parse():
     # kernel tag -> (lower, upper) plausible HBM bytes per launch
     cases = {"crt_gemm_i8_16384": (1e10, 1e12), "dgemm_f64_mfma_16384": (1e10, 1e12),
              "dgemv_32768": (8e9, 9e9), "grouped_f64_100k": (7.9e9, 2e10),
-             "panel_col_qr_32768x4096": (3.1e7, 1e8), "qr_batched_20000": (2.1e9, 3e9),
+             "dgemm_f64_mfma_qr_update_32768x4096": (1.0e9, 2e9), "qr_batched_20000": (2.1e9, 3e9),
              "dgemm_f64_mfma_8192": (1.6e9, 1e11)}
     for tag, (lo, hi) in cases.items():
         v = bench.read_traffic(tag)
