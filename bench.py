@@ -472,12 +472,13 @@ def bench_qr(args) -> int:
            "recon_err": float(np.linalg.norm(As - Q @ R) / np.linalg.norm(As)),
            "orth_err": float(np.linalg.norm(Q.T @ Q - np.eye(cn)))}
     best = min(forms.values(), key=lambda f: f["ms"])
-    # Roofline of the factorization's dominant kernel, the trailing-update product C^T -= W2^T V^T (one
-    # FP64 MFMA GEMM launch per panel, nt x mk x nb with nt = n - k1 columns left and mk = m - k0 rows):
-    # algorithmic flops 2 nt mk nb per launch, timed by HIP events around each launch.
+    # Roofline of the factorization's dominant kernel, the trailing-update product C^T -= W2^T V^T: one FP64
+    # MFMA GEMM launch per pair of panels [k0, k0 + 2nb) (both panels' reflectors at once), nt x mk x kc with
+    # nt = n - k1 columns left, mk = m - k0 rows and kc = k1 - k0 <= 2nb: algorithmic flops 2 nt mk kc per
+    # launch, timed by HIP events around each launch.
     uflops, launches = 0.0, 0
-    for k0 in range(0, n, nb):
-        k1 = min(k0 + nb, n)
+    for k0 in range(0, n, 2 * nb):
+        k1 = min(k0 + 2 * nb, n)
         if n - k1 <= 0:
             continue
         uflops += 2.0 * (n - k1) * (m - k0) * (k1 - k0)
@@ -489,7 +490,7 @@ def bench_qr(args) -> int:
                 "traffic": read_traffic("dgemm_f64_mfma_qr_update_32768x4096"),
                 "kernel": "dgemm_f64_mfma (C^T -= W2^T V^T)",
                 "kernel_ms": round(ums / launches, 5), "launches": launches,
-                "note": f"{uflops / launches:.4g} algorithmic flops per launch (K = nb = 128, beta = 1: C read "
+                "note": f"{uflops / launches:.4g} algorithmic flops per launch (K = 2nb = 256, beta = 1: C read "
                         f"and written); the products take {ums:.1f} of the {forms['fp64']['ms']:.1f} ms (FP64 "
                         f"form); panels by CholeskyQR2 + Householder reconstruction: "
                         f"{forms['fp64']['panels']['cholesky_qr2']} of {-(-n // nb)}"}

@@ -391,16 +391,34 @@ hipError_t launch_scale(int64_t m, int64_t n, double beta, double* c, int64_t ld
     return hipGetLastError();
 }
 
-int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus, int tiles_per_cu, int min_kslice) {
-    // Few output tiles and a long K: give every CU work by cutting K.
+int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus, int tiles_per_cu, int min_kslice,
+                        bool quant) {
+    // Few output tiles and a long K: give every CU work by cutting K, up to tiles_per_cu workgroups per CU
+    // (and slices of at least min_kslice). Within that bound the count is the one with the least modelled
+    // time, so the last round of resident workgroups (two per CU) is not left mostly empty: in units of one
+    // tile's 16-deep step, ceil(tiles * s / slots) rounds of ceil(k / (16 s)) + 2 steps (the +2: a
+    // workgroup's prologue and epilogue), plus (s + 2) passes over the m x n partials at 5 TB/s (4e-7 steps
+    // per element and pass). E.g. 60 tiles, k = 32768 on 256 CUs: 18 slices (1080 workgroups, three rounds,
+    // the third 11% full) -> 17 (1020, two rounds).
     if (tiles_per_cu <= 0) return 1;
     const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
     const int64_t want = ((int64_t)tiles_per_cu * device_cus + tiles - 1) / tiles;
-    int64_t s = want;
+    int64_t smax = want;
     if (min_kslice < BK) min_kslice = BK;
-    if (s > k / min_kslice) s = k / min_kslice;
-    if (s > 64) s = 64;
-    return s < 2 ? 1 : (int)s;
+    if (smax > k / min_kslice) smax = k / min_kslice;
+    if (smax > 64) smax = 64;
+    if (smax < 2) return 1;
+    if (!quant) return (int)smax;  // the tuning table's splitk_quant 0: the full count, no round model
+    const int64_t slots = 2 * (int64_t)(device_cus > 0 ? device_cus : 1);
+    auto cost = [&](int64_t s) {
+        const double rounds = (double)((tiles * s + slots - 1) / slots);
+        const double steps = (double)((k + 16 * s - 1) / (16 * s)) + 2.0;
+        return rounds * steps + (s > 1 ? (double)(s + 2) * (double)m * (double)n * 4e-7 : 0.0);
+    };
+    int64_t best = smax;
+    for (int64_t s = smax - 1; s >= 1; s--)
+        if (cost(s) < cost(best)) best = s;
+    return best < 2 ? 1 : (int)best;
 }
 
 int dgemm_quant_slices(int64_t m, int64_t n, int64_t k, int device_cus, int64_t max_partial_bytes) {

@@ -39,7 +39,8 @@ struct TuneTable {
     int crt_waves = 8;                // CRT product kernel: 4 (128 x 128 per wavefront) or 8 (128 x 64)
     int crt_group = 16;               // CRT product kernel: tile rows per column strip of the tile order
     int crt_xcd = 0;                  // CRT product kernel: 1 = keep neighbouring tiles on one XCD
-    int splitk_quant = 1;             // split K against wave quantization of many-tile grids (0 = off)
+    int splitk_quant = 1;             // split-K counts chosen against wave quantization: the count of a
+                                      // few-tile split and a 2-4 split of many-tile grids (0 = off)
     int grouped_small_tile = 16;      // grouped FP64: 16-row / 16-column tiles for m / n up to this (0: off)
     int grouped_chunk_tiles = 64;     // grouped: tiles of one problem dealt to one XCD together (whole tile rows)
     int qr_wave_max_lds = 2560;       // batched QR: size classes up to this many LDS doubles run one
@@ -59,7 +60,9 @@ hipError_t launch_dgemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, in
 
 // Split-K variant: K cut into `slices` pieces whose partial products go to
 // work (slices * m * n doubles) and are summed in order into C.
-int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus, int tiles_per_cu, int min_kslice);
+// quant: pick, within that bound, the count whose resident rounds are fullest (see dgemm_f64.hip).
+int dgemm_splitk_slices(int64_t m, int64_t n, int64_t k, int device_cus, int tiles_per_cu, int min_kslice,
+                        bool quant);
 // Split factor (1..4) against wave quantization for grids of more tiles than are resident at once, with
 // the partial products bounded by max_partial_bytes; 1 when the split does not pay.
 int dgemm_quant_slices(int64_t m, int64_t n, int64_t k, int device_cus, int64_t max_partial_bytes);
@@ -178,9 +181,15 @@ hipError_t launch_cqr_chol(const double* g, int b, int pass, const double* r1, d
 int gram_parts(int64_t m);
 size_t gram_work_doubles(int64_t m);
 hipError_t launch_gram(const double* x, int64_t ldx, int m, int b, double* part, double* g, hipStream_t stream);
-// T (b x b row-major, upper) of the compact-WY product of b reflectors from G = V^T V and tau, as the
-// inverse of diag(1/tau) + strictly-upper(G) (reflectors with tau = 0 take no part), b <= 128.
-hipError_t launch_wy_t(const double* g, int b, const double* tau, double* T, hipStream_t stream);
+// T (b x b row-major with row stride ldt, upper) of the compact-WY product of b reflectors from G = V^T V
+// and tau, as the inverse of diag(1/tau) + strictly-upper(G) (reflectors with tau = 0 take no part),
+// b <= 128.
+hipError_t launch_wy_t(const double* g, int b, const double* tau, double* T, int ldt, hipStream_t stream);
+// G = X^T Y (bx x by row-major, row stride ldg) for column-major m x bx and m x by matrices X, Y (ld ldx,
+// ldy), bx, by <= 128; part holds gram_cross_work_doubles(m) doubles.
+size_t gram_cross_work_doubles(int64_t m);
+hipError_t launch_gram_cross(const double* x, int64_t ldx, int bx, const double* y, int64_t ldy, int by, int m,
+                             double* part, double* g, int ldg, hipStream_t stream);
 // Modified LU of Q's top b x b block (column-major, ldq): Y1 and D R into the panel's top block ap (lda),
 // tau, ui = U^-1 (row-major b x b), spec (b ints) = candidates for the tau = 0 case.
 hipError_t launch_cqr_lu(const double* q, int64_t ldq, int b, const double* r, double* ap, int64_t lda, double* tau,
@@ -189,6 +198,10 @@ hipError_t launch_cqr_lu(const double* q, int64_t ldq, int b, const double* r, d
 hipError_t launch_cqr_special(double* ap, int64_t lda, int mp, int b, double* tau, const int* spec,
                               hipStream_t stream);
 hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb, double* vc, hipStream_t stream);
+// The same into rows of stride ldv, each starting with pad zeros (V of a later panel inside a combined
+// V of several panels).
+hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb, double* vc, int64_t ldv, int pad,
+                           hipStream_t stream);
 hipError_t launch_build_t(const double* g, int nb, const double* tau, double* T, hipStream_t stream);
 hipError_t launch_extract_r(const double* at, int64_t lda, int n, double* r, hipStream_t stream);
 hipError_t launch_fill_eye(double* x, int64_t ld, int rows, int cols, double diag, hipStream_t stream);

@@ -233,17 +233,19 @@ __global__ __launch_bounds__(PTHREADS) void panel_col(double* __restrict__ at, i
 }
 
 // Vc (nb x mk, row-major): Vc[j][i] = v_j at panel row i (unit diagonal, zeros above).
+// Row j < nb of vc (row stride ldv): pad zeros, then column k0 + j of V over rows k0..m-1 (zeros above
+// the diagonal, the implicit 1 on it, the stored vector below).
 __global__ __launch_bounds__(256) void build_vc(const double* __restrict__ at, int64_t lda, int m, int k0, int nb,
-                                                double* __restrict__ vc) {
-    const int mk = m - k0;
+                                                double* __restrict__ vc, int64_t ldv, int pad) {
+    const int mk = m - k0, w = pad + mk;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)nb * mk) return;
-    const int j = (int)(idx / mk), i = (int)(idx % mk);
+    if (idx >= (int64_t)nb * w) return;
+    const int j = (int)(idx / w), p = (int)(idx % w), i = p - pad;
     double v;
     if (i < j) v = 0.0;
     else if (i == j) v = 1.0;
     else v = at[(int64_t)(k0 + j) * lda + k0 + i];
-    vc[idx] = v;
+    vc[(int64_t)j * ldv + p] = v;
 }
 
 // T (nb x nb row-major, upper triangular) from G = V^T V and tau, with T(i,i) = tau_i and
@@ -848,8 +850,15 @@ hipError_t launch_qr_panel_fused(double* at, int64_t lda, int m, int k0, int k1,
 }
 
 hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb, double* vc, hipStream_t stream) {
-    const int64_t total = (int64_t)nb * (m - k0);
-    hipLaunchKernelGGL(build_vc, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, at, lda, m, k0, nb, vc);
+    return launch_build_vc(at, lda, m, k0, nb, vc, m - k0, 0, stream);
+}
+
+hipError_t launch_build_vc(const double* at, int64_t lda, int m, int k0, int nb, double* vc, int64_t ldv, int pad,
+                           hipStream_t stream) {
+    const int64_t total = (int64_t)nb * (pad + m - k0);
+    if (ldv < pad + m - k0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(build_vc, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, at, lda, m, k0, nb, vc,
+                       ldv, pad);
     return hipGetLastError();
 }
 

@@ -647,12 +647,87 @@ __global__ __launch_bounds__(CQ_NT) void gram_partial(const double* __restrict__
     }
 }
 
+// X^T Y for two column-major m x b matrices: as gram_partial, with both staged side by side and all
+// nblk x nblk tiles (tile idx = ti * nblk + tj, up to eight per wavefront).
+constexpr int GRC_MAXT = 8;
+__global__ __launch_bounds__(CQ_NT) void gram_cross_partial(const double* __restrict__ x, int64_t ldx, int bx,
+                                                            const double* __restrict__ y, int64_t ldy, int by, int m,
+                                                            int b, int per, double* __restrict__ part) {
+    extern __shared__ double xys[];  // two 128 x GR_LDC chunks: X, then Y
+    double* xs = xys;
+    double* ys = xys + 128 * GR_LDC;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, i16 = l & 15, g = l >> 4;
+    const int nblk = (b + 15) >> 4, ntile = nblk * nblk;
+    const int r0 = blockIdx.x * per, r1 = min(m, r0 + per);
+    d4 acc[GRC_MAXT];
+#pragma unroll
+    for (int q = 0; q < GRC_MAXT; q++) acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+    const int kp = 2 * (t & 31), c0 = t >> 5;
+    double2 nx[8], ny[8];
+    auto fetch1 = [&](const double* src0, int64_t ld, int nc, int k0, double2 (&o)[8]) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = c0 + 16 * j, k = k0 + kp;
+            double2 v = make_double2(0.0, 0.0);
+            if (c < nc) {
+                const double* src = src0 + (int64_t)c * ld + k;
+                if (k + 1 < r1 && (((uintptr_t)src) & 15) == 0) {
+                    v = *reinterpret_cast<const double2*>(src);
+                } else {
+                    if (k < r1) v.x = src[0];
+                    if (k + 1 < r1) v.y = src[1];
+                }
+            }
+            o[j] = v;
+        }
+    };
+    if (r0 < r1) {
+        fetch1(x, ldx, bx, r0, nx);
+        fetch1(y, ldy, by, r0, ny);
+    }
+    for (int k0 = r0; k0 < r1; k0 += GR_ROWS) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = c0 + 16 * j;
+            xs[c * GR_LDC + kp] = nx[j].x;
+            xs[c * GR_LDC + kp + 1] = nx[j].y;
+            ys[c * GR_LDC + kp] = ny[j].x;
+            ys[c * GR_LDC + kp + 1] = ny[j].y;
+        }
+        __syncthreads();
+        if (k0 + GR_ROWS < r1) {
+            fetch1(x, ldx, bx, k0 + GR_ROWS, nx);
+            fetch1(y, ldy, by, k0 + GR_ROWS, ny);
+        }
+#pragma unroll
+        for (int q = 0; q < GRC_MAXT; q++) {
+            const int idx = w + CQ_W * q;
+            if (idx < ntile) {
+                const double* xa = xs + (16 * (idx / nblk) + i16) * GR_LDC + g;
+                const double* yb = ys + (16 * (idx % nblk) + i16) * GR_LDC + g;
+#pragma unroll
+                for (int st = 0; st < GR_ROWS / 4; st++) acc[q] = mfma(acc[q], xa[4 * st], yb[4 * st]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < GRC_MAXT; q++) {
+        const int idx = w + CQ_W * q;
+        if (idx < ntile) {
+            double* o = part + ((int64_t)blockIdx.x * ntile + idx) * 256;
+#pragma unroll
+            for (int r = 0; r < 4; r++) o[(g + 4 * r) * 16 + i16] = acc[q][r];
+        }
+    }
+}
+
 // Workgroup (tile, 16-entry group): thread (entry e, lane group pg) sums partials pg, pg + 16, ... of
 // its entry, then the 16 sums are added in a fixed order (the result does not depend on timing).
-__global__ __launch_bounds__(256) void gram_reduce(const double* __restrict__ part, int nparts, int b,
-                                                   double* __restrict__ gm) {
+__global__ __launch_bounds__(256) void gram_reduce(const double* __restrict__ part, int nparts, int b, int cross,
+                                                   double* __restrict__ gm, int ldg, int bx, int by) {
     __shared__ double red[16][17];
-    const int nblk = (b + 15) >> 4, ntile = nblk * (nblk + 1) / 2;
+    const int nblk = (b + 15) >> 4, ntile = cross ? nblk * nblk : nblk * (nblk + 1) / 2;
     const int idx = blockIdx.x / 16, eg = blockIdx.x % 16;
     const int e = threadIdx.x & 15, pg = threadIdx.x >> 4;
     const int ent = eg * 16 + e;  // entry of the 16 x 16 tile: row ent / 16, column ent % 16
@@ -664,15 +739,23 @@ __global__ __launch_bounds__(256) void gram_reduce(const double* __restrict__ pa
         double sum = 0.0;
 #pragma unroll
         for (int q = 0; q < 16; q++) sum += red[q][threadIdx.x];
-        int a = 0, rem = idx;
-        while (rem >= nblk - a) {
-            rem -= nblk - a;
-            a++;
+        int a, cb;
+        if (cross) {
+            a = idx / nblk;
+            cb = idx % nblk;
+        } else {
+            int rem = idx;
+            a = 0;
+            while (rem >= nblk - a) {
+                rem -= nblk - a;
+                a++;
+            }
+            cb = a + rem;
         }
-        const int i = 16 * a + (ent >> 4), j = 16 * (a + rem) + (ent & 15);
-        if (i < b && j < b) {
-            gm[i * b + j] = sum;
-            gm[j * b + i] = sum;
+        const int i = 16 * a + (ent >> 4), j = 16 * cb + (ent & 15);
+        if (i < bx && j < by) {
+            gm[i * ldg + j] = sum;
+            if (!cross) gm[j * ldg + i] = sum;
         }
     }
 }
@@ -683,7 +766,8 @@ __global__ __launch_bounds__(256) void gram_reduce(const double* __restrict__ pa
 // reflector with tau_i = 0 (H_i = I) takes no part: its row and column of T^-1 are e_i and T(i, i) = 0.
 // Dynamic LDS CQ_DYN.
 __global__ __launch_bounds__(CQ_NT) void wy_t_inverse(const double* __restrict__ gm, int b,
-                                                      const double* __restrict__ tau, double* __restrict__ tout) {
+                                                      const double* __restrict__ tau, double* __restrict__ tout,
+                                                      int ldt) {
     __shared__ double dinv[128];
     __shared__ int zt[128];
     extern __shared__ double dyn[];
@@ -722,7 +806,7 @@ __global__ __launch_bounds__(CQ_NT) void wy_t_inverse(const double* __restrict__
     __syncthreads();
     upper_inverse_blocked(S, dblk, dinv, nblk);
     for (int r = t >> 7; r < b; r += 4)
-        if (c < b) tout[r * b + c] = (r <= c && !(r == c && zt[r])) ? S[r * SLD + c] : 0.0;
+        if (c < b) tout[r * ldt + c] = (r <= c && !(r == c && zt[r])) ? S[r * SLD + c] : 0.0;
 }
 
 // The reference's tau = 0 case: column i of Y exactly zero below the diagonal (Y1 part in spec, Y2
@@ -780,16 +864,38 @@ hipError_t launch_gram(const double* x, int64_t ldx, int m, int b, double* part,
     const int P = (m + per - 1) / per;
     hipLaunchKernelGGL(gram_partial, dim3(P), dim3(CQ_NT), 0, stream, x, ldx, m, b, per, part);
     const int nblk = (b + 15) >> 4, ntile = nblk * (nblk + 1) / 2;
-    hipLaunchKernelGGL(gram_reduce, dim3((unsigned)(16 * ntile)), dim3(256), 0, stream, part, P, b, g);
+    hipLaunchKernelGGL(gram_reduce, dim3((unsigned)(16 * ntile)), dim3(256), 0, stream, part, P, b, 0, g, b, b, b);
     return hipGetLastError();
 }
 
-hipError_t launch_wy_t(const double* g, int b, const double* tau, double* T, hipStream_t stream) {
-    if (b < 1 || b > 128) return hipErrorInvalidValue;
+size_t gram_cross_work_doubles(int64_t m) { return (size_t)gram_parts(m) * 64 * 256; }
+
+hipError_t launch_gram_cross(const double* x, int64_t ldx, int bx, const double* y, int64_t ldy, int by, int m,
+                             double* part, double* g, int ldg, hipStream_t stream) {
+    if (bx < 1 || bx > 128 || by < 1 || by > 128 || m < 1 || ldx < m || ldy < m || ldg < by)
+        return hipErrorInvalidValue;
+    const int b = bx > by ? bx : by;
+    const int P0 = gram_parts(m);
+    const int per = (int)(((m + P0 - 1) / P0 + GR_ROWS - 1) / GR_ROWS * GR_ROWS);
+    const int P = (m + per - 1) / per;
+    constexpr int lds = 2 * 128 * GR_LDC * 8;
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)gram_cross_partial, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (attr != hipSuccess) return attr;
+    hipLaunchKernelGGL(gram_cross_partial, dim3(P), dim3(CQ_NT), lds, stream, x, ldx, bx, y, ldy, by, m, b, per,
+                       part);
+    const int nblk = (b + 15) >> 4;
+    hipLaunchKernelGGL(gram_reduce, dim3((unsigned)(16 * nblk * nblk)), dim3(256), 0, stream, part, P, b, 1, g, ldg,
+                       bx, by);
+    return hipGetLastError();
+}
+
+hipError_t launch_wy_t(const double* g, int b, const double* tau, double* T, int ldt, hipStream_t stream) {
+    if (b < 1 || b > 128 || ldt < b) return hipErrorInvalidValue;
     static const hipError_t attr =
         hipFuncSetAttribute((const void*)wy_t_inverse, hipFuncAttributeMaxDynamicSharedMemorySize, CQ_DYN);
     if (attr != hipSuccess) return attr;
-    hipLaunchKernelGGL(wy_t_inverse, dim3(1), dim3(CQ_NT), CQ_DYN, stream, g, b, tau, T);
+    hipLaunchKernelGGL(wy_t_inverse, dim3(1), dim3(CQ_NT), CQ_DYN, stream, g, b, tau, T, ldt);
     return hipGetLastError();
 }
 

@@ -569,7 +569,8 @@ int run_fp64(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, cons
     }
     *kernel = JLC_KERNEL_MFMA_F64;
     const jlc::TuneTable tt = jlc::tune_snapshot();
-    int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load(), tt.splitk_tiles_per_cu, tt.splitk_min_kslice);
+    int slices = jlc::dgemm_splitk_slices(m, n, k, g_cus.load(), tt.splitk_tiles_per_cu, tt.splitk_min_kslice,
+                                          tt.splitk_quant != 0);
     if (slices == 1 && tt.splitk_quant)
         slices = jlc::dgemm_quant_slices(m, n, k, g_cus.load(),
                                          fixed ? (int64_t)fixed_bytes : (int64_t)QUANT_SPLIT_MAX_BYTES);
@@ -1569,7 +1570,7 @@ int panel_wy(const double* a, int64_t lda, int64_t m, int64_t k0, int64_t nb, co
     JLC_HIP_CHECK(jlc::launch_build_vc(a, lda, (int)m, (int)k0, (int)nb, vc, s));
     // G = V^T V, with Vc = V^T row-major, i.e. V column-major (ld mk).
     JLC_HIP_CHECK(jlc::launch_gram(vc, mk, (int)mk, (int)nb, gpart, g, s));
-    JLC_HIP_CHECK(jlc::launch_wy_t(g, (int)nb, tau + k0, t, s));
+    JLC_HIP_CHECK(jlc::launch_wy_t(g, (int)nb, tau + k0, t, (int)nb, s));
     info->launches += 4;
     return JLC_OK;
 }
@@ -1648,11 +1649,12 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     hipError_t e = part.alloc((size_t)2 * (2 * G + 1 + 128) * 8);
     if (e == hipSuccess) e = vtmp.alloc((size_t)2 * m * 8);
     if (e == hipSuccess) e = dots.alloc((size_t)2 * G * 128 * 8);
-    if (e == hipSuccess) e = vc.alloc((size_t)nb * m * 8);
-    if (e == hipSuccess) e = g.alloc((size_t)nb * nb * 8);
-    if (e == hipSuccess) e = t.alloc((size_t)nb * nb * 8);
-    if (e == hipSuccess) e = w1.alloc((size_t)n * nb * 8);
-    if (e == hipSuccess) e = w2.alloc((size_t)n * nb * 8);
+    // Panels go in pairs: V and T of a pair (2nb reflectors) update the rest of the matrix in one pass.
+    if (e == hipSuccess) e = vc.alloc((size_t)2 * nb * m * 8);
+    if (e == hipSuccess) e = g.alloc((size_t)3 * nb * nb * 8);     // G of a panel, V_A^T V_B, G_AB T_B
+    if (e == hipSuccess) e = t.alloc((size_t)4 * nb * nb * 8);     // T of the pair (2nb x 2nb)
+    if (e == hipSuccess) e = w1.alloc((size_t)n * 2 * nb * 8);
+    if (e == hipSuccess) e = w2.alloc((size_t)n * 2 * nb * 8);
     // CholeskyQR2 panels: those with at least qr_cqr_min_rows rows and twice their width.
     const int64_t cqr_min = jlc::tune_snapshot().qr_cqr_min_rows;
     const auto cqr_panel = [&](int64_t k0, int64_t b) {
@@ -1660,7 +1662,7 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     };
     DevBuf cqp, cqs, gpart;
     CqrWork cw{};
-    if (e == hipSuccess) e = gpart.alloc(jlc::gram_work_doubles(m) * 8);
+    if (e == hipSuccess) e = gpart.alloc(std::max(jlc::gram_work_doubles(m), jlc::gram_cross_work_doubles(m)) * 8);
     if (e == hipSuccess && cqr_panel(0, std::min(nb, kmax))) {
         e = cqp.alloc((size_t)2 * m * nb * 8);
         if (e == hipSuccess) e = cqs.alloc((size_t)6 * nb * nb * 8 + (size_t)(nb + 1) * sizeof(int));
@@ -1701,42 +1703,93 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
         JLC_HIP_CHECK(hipEventRecord(t_ctx.ev0, s));
     }
     int cqr_panels = 0, col_panels = 0;
-    std::vector<size_t> timed;  // event pairs recorded (panels with a trailing update)
-    for (int64_t k0 = 0; k0 < kmax; k0 += nb) {
-        const int64_t k1 = k0 + nb < kmax ? k0 + nb : kmax;
-        const int64_t nbk = k1 - k0, mk = m - k0, nt = n - k1;
-        const size_t pi = (size_t)(2 * (k0 / nb));
-        bool panel_done = false;
-        if (cw.q && cqr_panel(k0, nbk) &&
-            (rc = panel_cqr(a, lda, m, k0, nbk, tau, cw, s, &info, &panel_done)) != JLC_OK)
-            return rc;
-        if (!panel_done) {
+    std::vector<size_t> timed;  // event pairs recorded (pairs with a trailing update)
+    // Factors panel [k0, k1) in place (CholeskyQR2 where it applies, else the column kernels).
+    const auto factor = [&](int64_t k0, int64_t k1) -> int {
+        bool done = false;
+        int r = JLC_OK;
+        if (cw.q && cqr_panel(k0, k1 - k0) && (r = panel_cqr(a, lda, m, k0, k1 - k0, tau, cw, s, &info, &done)) != JLC_OK)
+            return r;
+        if (!done) {
             JLC_HIP_CHECK(jlc::launch_qr_panel_fused(a, lda, (int)m, (int)k0, (int)k1, tau, part.d(), dots.d(),
                                                      vtmp.d(), s));
-            info.launches += (int)nbk + 1;
+            info.launches += (int)(k1 - k0) + 1;
             col_panels++;
         } else {
             cqr_panels++;
         }
-        if (nt <= 0) continue;
-        if ((rc = panel_wy(a, lda, m, k0, nbk, tau, vc.d(), g.d(), t.d(), gpart.d(), s, &info)) != JLC_OK) return rc;
-        // Transposed compact-WY update of C = A[k0:m, k1:n] held as C^T (nt x mk, row-major, ld = lda):
-        // W^T = C^T V; W^T <- W^T T; C^T -= W^T V^T   (C <- (I - V T^T V^T) C).
-        double* ct = a + k1 * lda + k0;
-        if ((rc = gemm_on(JLC_ROW, JLC_COL, nt, nbk, mk, 1.0, ct, lda, vc.d(), mk, 0.0, w1.d(), nbk, mode, s,
-                          &info)) != JLC_OK)
-            return rc;
-        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nt, nbk, nbk, 1.0, w1.d(), nbk, t.d(), nbk, 0.0, w2.d(), nbk,
-                          JLC_MODE_FP64, s, &info)) != JLC_OK)
-            return rc;
-        if (stats) {
+        return JLC_OK;
+    };
+    // C <- (I - V T^T V^T) C for C = A[k0:m, c0:c0+nc] held as C^T (nc x mk, row-major, ld = lda), V as Vc
+    // (kb x mk, row stride mk) and T (kb x kb, row stride ldt): W^T = C^T V; W^T <- W^T T; C^T -= W^T V^T.
+    const auto update = [&](int64_t k0, int64_t c0, int64_t nc, int64_t kb, const double* tm, int64_t ldt,
+                            int umode, bool timed_product, size_t pi) -> int {
+        const int64_t mk = m - k0;
+        double* ct = a + c0 * lda + k0;
+        int r;
+        if (nc <= 128 && kb <= 128 && (umode & 0xff) == JLC_MODE_FP64) {
+            // a panel-wide C: W^T = C^T V by the Gram kernel (the split-K GEMM has one output tile to spread)
+            JLC_HIP_CHECK(jlc::launch_gram_cross(ct, lda, (int)nc, vc.d(), mk, (int)kb, (int)mk, gpart.d(), w1.d(),
+                                                 (int)kb, s));
+            info.launches += 2;
+        } else if ((r = gemm_on(JLC_ROW, JLC_COL, nc, kb, mk, 1.0, ct, lda, vc.d(), mk, 0.0, w1.d(), kb, umode, s,
+                                &info)) != JLC_OK) {
+            return r;
+        }
+        if ((r = gemm_on(JLC_ROW, JLC_ROW, nc, kb, kb, 1.0, w1.d(), kb, tm, ldt, 0.0, w2.d(), kb, JLC_MODE_FP64, s,
+                         &info)) != JLC_OK)
+            return r;
+        if (timed_product) {
             JLC_HIP_CHECK(hipEventRecord(pev.ev[pi], s));
             timed.push_back(pi);
         }
-        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nt, mk, nbk, -1.0, w2.d(), nbk, vc.d(), mk, 1.0, ct, lda, mode, s,
+        if ((r = gemm_on(JLC_ROW, JLC_ROW, nc, mk, kb, -1.0, w2.d(), kb, vc.d(), mk, 1.0, ct, lda, umode, s, &info)) !=
+            JLC_OK)
+            return r;
+        if (timed_product) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi + 1], s));
+        return JLC_OK;
+    };
+    double* const ga = g.d();                // G of one panel (nb x nb)
+    double* const gab = g.d() + nb * nb;     // V_A^T V_B
+    double* const xab = g.d() + 2 * nb * nb; // G_AB T_B
+    for (int64_t k0 = 0; k0 < kmax; k0 += 2 * nb) {
+        // Pair of panels A = [k0, ka), B = [ka, kb1) (B may be absent); T of the pair is 2nb x 2nb.
+        const int64_t ka = std::min(k0 + nb, kmax), kb1 = std::min(k0 + 2 * nb, kmax);
+        const int64_t nbA = ka - k0, nbB = kb1 - ka, mk = m - k0, kc = nbA + nbB;
+        const int64_t ldt = kc;
+        const size_t pi = (size_t)(2 * (k0 / nb));
+        if ((rc = factor(k0, ka)) != JLC_OK) return rc;
+        if (n - ka <= 0) continue;
+        // V_A, G_A = V_A^T V_A, T_A into the pair's T.
+        JLC_HIP_CHECK(jlc::launch_build_vc(a, lda, (int)m, (int)k0, (int)nbA, vc.d(), mk, 0, s));
+        JLC_HIP_CHECK(jlc::launch_gram(vc.d(), mk, (int)mk, (int)nbA, gpart.d(), ga, s));
+        JLC_HIP_CHECK(jlc::launch_wy_t(ga, (int)nbA, tau + k0, t.d(), (int)ldt, s));
+        info.launches += 4;
+        if (nbB == 0) {
+            if ((rc = update(k0, ka, n - ka, nbA, t.d(), ldt, mode, stats != nullptr, pi)) != JLC_OK) return rc;
+            continue;
+        }
+        // Panel B's columns take A's reflectors, then B is factored.
+        if ((rc = update(k0, ka, nbB, nbA, t.d(), ldt, mode, false, 0)) != JLC_OK) return rc;
+        if ((rc = factor(ka, kb1)) != JLC_OK) return rc;
+        if (n - kb1 <= 0) continue;
+        // V_B below V_A in Vc (its first nbA entries zero), G_B and T_B, then T_AB = -T_A (V_A^T V_B) T_B.
+        double* vcb = vc.d() + nbA * mk;
+        JLC_HIP_CHECK(jlc::launch_build_vc(a, lda, (int)m, (int)ka, (int)nbB, vcb, mk, (int)nbA, s));
+        JLC_HIP_CHECK(jlc::launch_gram(vcb + nbA, mk, (int)(mk - nbA), (int)nbB, gpart.d(), ga, s));
+        double* tb = t.d() + nbA * ldt + nbA;
+        JLC_HIP_CHECK(jlc::launch_wy_t(ga, (int)nbB, tau + ka, tb, (int)ldt, s));
+        JLC_HIP_CHECK(jlc::launch_gram_cross(vc.d() + nbA, mk, (int)nbA, vcb + nbA, mk, (int)nbB, (int)(mk - nbA),
+                                             gpart.d(), gab, (int)nbB, s));
+        JLC_HIP_CHECK(jlc::launch_fill_eye(t.d() + nbA * ldt, ldt, (int)nbB, (int)nbA, 0.0, s));
+        info.launches += 7;
+        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nbA, nbB, nbB, 1.0, gab, nbB, tb, ldt, 0.0, xab, nbB, JLC_MODE_FP64, s,
                           &info)) != JLC_OK)
             return rc;
-        if (stats) JLC_HIP_CHECK(hipEventRecord(pev.ev[pi + 1], s));
+        if ((rc = gemm_on(JLC_ROW, JLC_ROW, nbA, nbB, nbA, -1.0, t.d(), ldt, xab, nbB, 0.0, t.d() + nbA, ldt,
+                          JLC_MODE_FP64, s, &info)) != JLC_OK)
+            return rc;
+        if ((rc = update(k0, kb1, n - kb1, kc, t.d(), ldt, mode, stats != nullptr, pi)) != JLC_OK) return rc;
     }
     double kms = 0.0;
     if (stats) {

@@ -73,12 +73,15 @@ def test_noalloc_null_stream_and_degenerate_shapes():
 
 
 def test_noalloc_rejects_partials_beyond_the_scratch():
-    """With a tuning that splits K into many slices over many tiles, the partials of a 2048^2 output
-    (64 slices x 2048^2 doubles = 2 GiB) exceed the 256 MiB scratch: the call fails with a message and
-    launches nothing; jlc_dgemm_dev serves the same call from its own workspace."""
+    """With a tuning that splits K into many slices over many tiles (the round model switched off, so the
+    full count is taken), the partials of a 2048^2 output (64 slices x 2048^2 doubles = 2 GiB) exceed the
+    256 MiB scratch: the call fails with a message and launches nothing; jlc_dgemm_dev serves the same
+    call from its own workspace."""
     old = native.tune_get("splitk_tiles_per_cu")
+    old_quant = native.tune_get("splitk_quant")
     try:
         native.tune_set("splitk_tiles_per_cu", 64)
+        native.tune_set("splitk_quant", 0)
         m = n = 2048
         k = 65536
         da, db, dc = native.DeviceArray(m * k), native.DeviceArray(k * n), native.DeviceArray(m * n)
@@ -87,6 +90,7 @@ def test_noalloc_rejects_partials_beyond_the_scratch():
         assert "scratch" in native.last_error()
     finally:
         native.tune_set("splitk_tiles_per_cu", old)
+        native.tune_set("splitk_quant", old_quant)
 
 
 def test_noalloc_skips_the_quantization_split_its_scratch_cannot_hold():
