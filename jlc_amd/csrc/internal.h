@@ -48,7 +48,7 @@ struct TuneTable {
     int qr_block_threads = 256;       // batched QR: threads per matrix above qr_wave_max_lds (256/512/1024)
     int qr_reg_max_m = 192;           // batched QR: matrices with m <= this (at most 192), n <= 128 held
                                       // in registers; larger ones in LDS
-    int qr_cqr_min_rows = 512;        // blocked QR: panels of at least this many rows (and 2 * width)
+    int qr_cqr_min_rows = 256;        // blocked QR: panels of at least this many rows (and 2 * width)
                                       // by CholeskyQR2 + reconstruction (0 = column kernels only)
 };
 TuneTable tune_snapshot();

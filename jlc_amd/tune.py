@@ -14,6 +14,10 @@ Tuned entries:
   and the threads per matrix above it (256, 512 or 1024: more lanes on each column step); and
   qr_reg_max_m, the largest m (<= 192, with n <= 128) held in registers instead of LDS; timed on
   a batch with the config's shape distribution (m uniform on [16, 192], n = m / 2).
+* qr_cqr_min_rows: the panel height from which blocked QR factors a 128-wide panel by CholeskyQR2 with
+  Householder reconstruction (a fixed handful of launches) instead of one launch per column; single
+  panels of each candidate height are timed both ways and the bound is the smallest height from which
+  CholeskyQR2 stays faster.
 * hip_min_flops: the offload gate of the host-side dispatch (the HIP analogue
   of GemmDispatch.CUDA_THRESHOLD, src/main/java/net/faulj/compute/GemmDispatch.java:32).
   The end-to-end host-pointer GEMM (staging included) is timed per size and
@@ -154,6 +158,37 @@ def tune_qr_batched(quick: bool) -> dict:
             "count": count, "ms": {f"{r},{b},{t}": round(v, 3) for (r, b, t), v in res.items()}}
 
 
+def tune_qr_panel(quick: bool) -> dict:
+    heights = [256, 384, 512, 768, 1024, 2048, 4096] if not quick else [256, 512, 1024, 4096]
+    b = 128
+    rng = np.random.default_rng(9)
+    res = {}
+    saved = native.tune_get("qr_cqr_min_rows")
+    for m in heights:
+        a0 = native.DeviceArray.from_numpy(rng.standard_normal(m * b))
+        work = native.DeviceArray(m * b)
+        tau = native.DeviceArray(b)
+        times = {}
+        for name, bound in (("cholesky_qr2", 1), ("columns", 0)):
+            native.tune_set("qr_cqr_min_rows", bound)
+
+            def run():
+                native.dev_copy(work.ptr, a0.ptr, m * b * 8)
+                native.dgeqrf_dev(m, b, work.ptr, m, tau.ptr, b)
+            times[name] = _time_dev(run, 5)
+        res[m] = times
+    native.tune_set("qr_cqr_min_rows", saved)
+    chosen = 0
+    for m in sorted(heights, reverse=True):  # smallest height from which CholeskyQR2 stays faster
+        if res[m]["cholesky_qr2"] < res[m]["columns"]:
+            chosen = m
+        else:
+            break
+    native.tune_set("qr_cqr_min_rows", chosen)
+    return {"chosen": {"qr_cqr_min_rows": chosen},
+            "ms": {str(m): {k: round(v, 4) for k, v in t.items()} for m, t in res.items()}}
+
+
 def tune_offload(cpu_gflops: float, quick: bool) -> dict:
     sizes = [32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024]
     if quick:
@@ -187,10 +222,10 @@ def main(argv=None) -> int:
     ap.add_argument("--out", default=str(native.TUNE_TABLE))
     ap.add_argument("--cpu-gflops", type=float, default=100.0)
     ap.add_argument("--quick", action="store_true")
-    ap.add_argument("--only", default="", help="comma list of splitk,grouped,qr,offload (default all); the "
-                                               "other entries keep the loaded table's values")
+    ap.add_argument("--only", default="", help="comma list of splitk,grouped,qr,qrpanel,offload (default all); "
+                                               "the other entries keep the loaded table's values")
     args = ap.parse_args(argv)
-    only = set(args.only.split(",")) if args.only else {"splitk", "grouped", "qr", "offload"}
+    only = set(args.only.split(",")) if args.only else {"splitk", "grouped", "qr", "qrpanel", "offload"}
     rc, count, name = native.probe()
     if rc != native.JLC_OK:
         print(f"no usable device: {native.last_error()}", file=sys.stderr)
@@ -202,6 +237,8 @@ def main(argv=None) -> int:
         report["grouped"] = tune_grouped(args.quick)
     if "qr" in only:
         report["qr_batched"] = tune_qr_batched(args.quick)
+    if "qrpanel" in only:
+        report["qr_panel"] = tune_qr_panel(args.quick)
     if "offload" in only:
         report["offload"] = tune_offload(args.cpu_gflops, args.quick)
     out = Path(args.out)
