@@ -15,7 +15,12 @@
 //     residues a' mod m_t and b' mod m_t fit in int8. They are written in the
 //     MFMA fragment order of the slice form: one 1 KiB block per (modulus,
 //     32-row block, 32-deep k chunk), lane l holding row l%32, k in
-//     [16*(l/32), +16).
+//     [16*(l/32), +16). Within a modulus' plane the blocks are k-major
+//     (block (rb, kc) at (kc * row_blocks + rb) KiB), so the operand strips
+//     of neighbouring C tiles for one k step are contiguous: with row-block-
+//     major planes a tile's 8 row blocks sat kpad KiB apart (512 KiB at
+//     K = 16384), on the same few L2 sets, and the sharing between the tiles
+//     resident on one XCD was lost to conflict misses.
 //  3. Products. Each modulus is one INT8 GEMM with exact INT32 accumulation
 //     (|sum| <= k * 2^14, folded modulo m_t every 65536 k when k is larger);
 //     the epilogue reduces the sum modulo m_t and stores the int8 residue of
@@ -331,8 +336,10 @@ __device__ __forceinline__ v4i residues16(const int (&D0)[16], const int (&D1)[1
 }
 
 // Residue planes of op(X) (element (r, k) at X[r*sr + k*sk], R x Kd, zero
-// padded): one wavefront per 1 KiB block; lane l handles row l%32,
-// k in [16*(l/32), +16) and writes 16 bytes per modulus.
+// padded): one wavefront per 1 KiB block, taken row block by row block (the
+// reads of neighbouring wavefronts are neighbours in X); lane l handles row
+// l%32, k in [16*(l/32), +16) and writes 16 bytes per modulus to block
+// (rb, kc) at kc * row_blocks + rb of the k-major plane.
 // VEC: k-contiguous rows (sk == 1), 16-byte aligned, Kd a multiple of 16:
 // each lane reads its 16 doubles as eight 16-byte loads.
 template <bool VEC>
@@ -379,7 +386,8 @@ __global__ __launch_bounds__(256) void crt_pack(const double* __restrict__ X, in
         const double mid = trunc(rest * 0x1p-22);
         image_digits((int)h, (int)mid, (int)fma(-mid, 0x1p22, rest), D0[j], D1[j], E2[j]);
     }
-    const int64_t off = (int64_t)blk * 64 + lane;
+    const int rblocks = nblocks / kchunks;
+    const int64_t off = ((int64_t)kc * rblocks + rb) * 64 + lane;
     for (int t = 0; t < T.nmod; t++) out[t * plane_stride + off] = residues16(D0, D1, E2, T, t);
 }
 
@@ -433,6 +441,8 @@ __device__ __forceinline__ void fold_acc(v16i& acc, int m, float invf) {
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void crt_gemm_i8(
     int kchunks, const v4i* __restrict__ Ap, int64_t a_plane, const v4i* __restrict__ Bp, int64_t b_plane,
     v4i* __restrict__ Cr, int64_t c_plane, int cblocks_n, int tiles_m, int tiles_n, int group, int xcd, CrtTables T) {
+    // row blocks of the A and B planes (k-major: block (rb, kc) at kc * row_blocks + rb)
+    const int rba = tiles_m * 8, rbb = tiles_n * 8;
     extern __shared__ v4i lds[];  // [NBUF_LDS][32 blocks: A kc*8+rb (0..15), B 16 + kc*8+cb][64 lanes]
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -440,23 +450,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int t = blockIdx.y;
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn, group, xcd != 0);
-    const v4i* A = Ap + t * a_plane + (int64_t)tm * 8 * kchunks * 64;
-    const v4i* B = Bp + t * b_plane + (int64_t)tn * 8 * kchunks * 64;
-    // Loads of a stage: 32 blocks of 1 KiB; wavefront w moves blocks w, w+8, w+16, w+24.
+    const v4i* A = Ap + t * a_plane + (int64_t)tm * 8 * 64;
+    const v4i* B = Bp + t * b_plane + (int64_t)tn * 8 * 64;
+    // Loads of a stage: 32 blocks of 1 KiB; wavefront w moves blocks w, w+8 (A) and w+16, w+24 (B).
     const v4i* src[4];
     int dst[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int b = w + 8 * i;  // 0..15 A, 16..31 B; block b: row/col block (b&15)>>1, kc b&1
         const int rb = (b & 15) >> 1, kc = b & 1;
-        src[i] = (b < 16 ? A : B) + ((int64_t)rb * kchunks + kc) * 64 + lane;
+        src[i] = (i < 2 ? A : B) + ((int64_t)kc * (i < 2 ? rba : rbb) + rb) * 64 + lane;
         dst[i] = ((b & 16) + kc * 8 + rb) * 64 + lane;
     }
     const int stages = kchunks / 2;
+    const int64_t sa = (int64_t)2 * rba * 64, sb = (int64_t)2 * rbb * 64;  // one stage (two k chunks)
     v4i r[PREFETCH][4];
     auto load = [&](int s, v4i (&x)[4]) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) x[i] = src[i][(int64_t)s * 128];
+        for (int i = 0; i < 4; i++) x[i] = src[i][(int64_t)s * (i < 2 ? sa : sb)];
     };
     auto store = [&](int buf, const v4i (&x)[4]) {
         v4i* base = lds + buf * STAGE_V4;
@@ -556,19 +567,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const int t = blockIdx.y;
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    // block w + 4i of a stage: operand i >> 2, row/col block (w >> 1) + 2 (i & 3), k chunk w & 1
-    const int64_t blk_stride = (int64_t)2 * kchunks * 64;  // two row blocks apart
-    const v4i* srcA = Ap + t * a_plane + ((int64_t)tm * 8 * kchunks + (int64_t)(w >> 1) * kchunks + (w & 1)) * 64 + lane;
-    const v4i* srcB = Bp + t * b_plane + ((int64_t)tn * 8 * kchunks + (int64_t)(w >> 1) * kchunks + (w & 1)) * 64 + lane;
+    // block w + 4i of a stage: operand i >> 2, row/col block (w >> 1) + 2 (i & 3), k chunk w & 1; planes
+    // k-major (block (rb, kc) at kc * row_blocks + rb)
+    const int rba = tiles_m * 8, rbb = tiles_n * 8;
+    constexpr int64_t blk_stride = 2 * 64;  // two row blocks apart
+    const v4i* srcA = Ap + t * a_plane + ((int64_t)(w & 1) * rba + tm * 8 + (w >> 1)) * 64 + lane;
+    const v4i* srcB = Bp + t * b_plane + ((int64_t)(w & 1) * rbb + tn * 8 + (w >> 1)) * 64 + lane;
     const int dst0 = ((w & 1) * 8 + (w >> 1)) * 64 + lane;  // + 2i blocks; B at + 16 blocks
     const int stages = kchunks / 2;
+    const int64_t sa = (int64_t)2 * rba * 64, sb = (int64_t)2 * rbb * 64;  // one stage (two k chunks)
     v4i r0[8], r1[8];
     auto load = [&](int s, v4i (&r)[8]) {
-        const int64_t o = (int64_t)s * 128;
+        const int64_t oa = (int64_t)s * sa, ob = (int64_t)s * sb;
 #pragma unroll
-        for (int i = 0; i < 4; i++) r[i] = srcA[o + i * blk_stride];
+        for (int i = 0; i < 4; i++) r[i] = srcA[oa + i * blk_stride];
 #pragma unroll
-        for (int i = 0; i < 4; i++) r[4 + i] = srcB[o + i * blk_stride];
+        for (int i = 0; i < 4; i++) r[4 + i] = srcB[ob + i * blk_stride];
     };
     auto store = [&](int buf, const v4i (&r)[8]) {
         v4i* base = lds + buf * STAGE_V4 + dst0;
