@@ -622,6 +622,14 @@ __global__ __launch_bounds__(NT) void qr_batched_block(const QrBatchDesc* __rest
 // every thread reads v at its rows unconditionally, forms its column's dot product by three shuffles
 // and updates its column in registers. The owners of column k + 1 then form reflector k + 1 before the
 // next barrier: one barrier per column.
+// 1/d from the hardware estimate refined by two Newton steps (within an ulp or two of the IEEE quotient)
+// for normal d: the reflector's reciprocals sit on the per-column critical path of qr_batched_reg.
+__device__ __forceinline__ double recip_nr(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
 template <int R>
 __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __restrict__ descs) {
     __shared__ double vbuf[2][8 * R];
@@ -656,7 +664,7 @@ __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __rest
         if (k < m - 1 && M >= 2.2250738585072014e-308) {
             // x / M as x * (1 / M) while 1 / M is a normal number (M below 2^1000), else by division.
             const bool huge = M > 0x1p1000;
-            const double inv = 1.0 / M;
+            const double inv = recip_nr(M);
             double sqr[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -668,18 +676,20 @@ __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __rest
             sq += __shfl_xor(sq, 1, 64);
             sq += __shfl_xor(sq, 2, 64);
             sq += __shfl_xor(sq, 4, 64);
-            const double xnorm = sqrt(sq);
             const double x0n = huge ? x0 / M : x0 * inv;
-            if (xnorm == 0.0) {
+            if (sq == 0.0) {
                 if (x0n < 0.0) {  // H = I - 2 e_k e_k^T negates the column's pivot
                     t = 2.0;
                     beta = -x0;
                 }
             } else {
-                const double nbeta = -copysign(hypot(x0n, xnorm), x0n);
+                // |x0n| <= 1 and sq <= m after the scaling by M, so sqrt(x0n^2 + sq) cannot overflow or
+                // underflow (it is at least 1: M is |x0| or the largest |x_i| below the pivot); |nbeta| and
+                // |x0n - nbeta| >= 1 keep both reciprocals' arguments normal.
+                const double nbeta = -copysign(sqrt(fma(x0n, x0n, sq)), x0n);
                 beta = nbeta * M;
-                t = (nbeta - x0n) / nbeta;
-                v0inv = 1.0 / ((x0n - nbeta) * M);
+                t = (nbeta - x0n) * recip_nr(nbeta);
+                v0inv = recip_nr((x0n - nbeta) * M);
             }
         }
 #pragma unroll
