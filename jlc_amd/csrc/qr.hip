@@ -632,7 +632,10 @@ __device__ __forceinline__ double recip_nr(double d) {
 
 template <int R>
 __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __restrict__ descs) {
-    __shared__ double vbuf[2][8 * R];
+    // v of the current reflector, thread-major: row sub + 8r at [sub * (R + 2) + r], so each thread's R rows
+    // are contiguous (16-byte LDS reads) and the eight threads of a column start on different banks
+    constexpr int VS = R + 2;
+    __shared__ __attribute__((aligned(16))) double vbuf[2][8 * VS];
     __shared__ double tbuf[2];
     const QrBatchDesc d = descs[blockIdx.x];
     const int m = d.m, n = d.n, kmax = m < n ? m : n;
@@ -700,7 +703,7 @@ __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __rest
             } else if (i > k && v0inv != 0.0) {
                 a[r] *= v0inv;  // rows >= m stay zero
             }
-            vbuf[buf][i] = i == k ? 1.0 : (i > k && t != 0.0 ? a[r] : 0.0);
+            vbuf[buf][sub * VS + r] = i == k ? 1.0 : (i > k && t != 0.0 ? a[r] : 0.0);
         }
         if (sub == 0) {
             tbuf[buf] = t;
@@ -715,14 +718,14 @@ __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __rest
         if (t != 0.0 && own && c > k) {
             double dr[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int r = 0; r < R; r++) dr[r & 3] += vbuf[buf][sub + 8 * r] * a[r];
+            for (int r = 0; r < R; r++) dr[r & 3] += vbuf[buf][sub * VS + r] * a[r];
             double dot = (dr[0] + dr[1]) + (dr[2] + dr[3]);
             dot += __shfl_xor(dot, 1, 64);
             dot += __shfl_xor(dot, 2, 64);
             dot += __shfl_xor(dot, 4, 64);
             const double tw = t * dot;
 #pragma unroll
-            for (int r = 0; r < R; r++) a[r] -= tw * vbuf[buf][sub + 8 * r];
+            for (int r = 0; r < R; r++) a[r] -= tw * vbuf[buf][sub * VS + r];
         }
         if (c == k + 1 && k + 1 < kmax) reflect(k + 1, buf ^ 1);
         __syncthreads();
