@@ -665,21 +665,23 @@ __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __rest
         const double M = fmax(mx, fabs(x0));
         double t = 0.0, beta = x0, v0inv = 0.0;
         if (k < m - 1 && M >= 2.2250738585072014e-308) {
-            // x / M as x * (1 / M) while 1 / M is a normal number (M below 2^1000), else by division.
-            const bool huge = M > 0x1p1000;
-            const double inv = recip_nr(M);
+            // x / M as (x s) * (1 / (M s)) with s = 1, or s = 2^-100 when M > 2^1000 so that 1 / (M s) stays a
+            // normal number (the power-of-two scaling is exact; an x s that underflows has x / M below
+            // 2^-1900). No division: these products sit on the per-column critical path.
+            const double sc = M > 0x1p1000 ? 0x1p-100 : 1.0;
+            const double inv = recip_nr(M * sc);
             double sqr[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int i = sub + 8 * r;
-                const double y = i > k ? (huge ? a[r] / M : a[r] * inv) : 0.0;
+                const double y = i > k ? (a[r] * sc) * inv : 0.0;
                 sqr[r & 3] += y * y;
             }
             double sq = (sqr[0] + sqr[1]) + (sqr[2] + sqr[3]);
             sq += __shfl_xor(sq, 1, 64);
             sq += __shfl_xor(sq, 2, 64);
             sq += __shfl_xor(sq, 4, 64);
-            const double x0n = huge ? x0 / M : x0 * inv;
+            const double x0n = (x0 * sc) * inv;
             if (sq == 0.0) {
                 if (x0n < 0.0) {  // H = I - 2 e_k e_k^T negates the column's pivot
                     t = 2.0;

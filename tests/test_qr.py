@@ -232,6 +232,42 @@ def test_batched_rejects_oversized_matrix():
         assert ei.value.status == native.JLC_ERR_INVALID_ARG
 
 
+def test_batched_register_kernel_extreme_scales_against_oracle():
+    # Matrices of the register kernel's sizes scaled near the top of the FP64 range (column maxima above
+    # 2^1000, where the reflector scales by 2^-100 before taking 1 / max) and near the bottom (1 / max still
+    # normal); the oracle forms x / max by division.
+    rng = np.random.default_rng(79)
+    mats = []
+    for (m, n), scale in (((40, 20), 1e305), ((150, 75), 3e305), ((192, 96), 1e305), ((64, 32), 1e-300),
+                          ((100, 50), 2e-290), ((33, 16), 1.0)):
+        mats.append(rng.uniform(-1.0, 1.0, (m, n)) * scale)
+    mats[5][:, 3] *= 1e303  # one column far above the others
+    flat = np.concatenate([np.asfortranarray(M).reshape(-1, order="F") for M in mats])
+    offs = np.concatenate([[0], np.cumsum([M.size for M in mats])[:-1]])
+    toffs = np.concatenate([[0], np.cumsum([min(M.shape) for M in mats])[:-1]])
+    da = native.DeviceArray.from_numpy(flat)
+    dt = native.DeviceArray(int(sum(min(M.shape) for M in mats)))
+    descs = [native.QrDesc(M.shape[0], M.shape[1], M.shape[0], da.at(int(o)), dt.at(int(t)))
+             for M, o, t in zip(mats, offs, toffs)]
+    native.dgeqrf_batched_dev(descs)
+    got, taus = da.to_numpy(), dt.to_numpy()
+    for i, M in enumerate(mats):
+        m, n = M.shape
+        f = got[offs[i]:offs[i] + m * n].copy()
+        tau = taus[toffs[i]:toffs[i] + n]
+        assert np.all(np.isfinite(f)) and np.all(np.isfinite(tau))
+        ref = np.asfortranarray(M).reshape(-1, order="F").copy()
+        tau_ref = orc.qr_factor(ref, m, n)
+        np.testing.assert_allclose(tau, tau_ref, rtol=1e-10, atol=1e-12)
+        Rg, Rr = np.triu(f.reshape(n, m).T[:n]), np.triu(ref.reshape(n, m).T[:n])
+        np.testing.assert_allclose(Rg, Rr, rtol=1e-9, atol=1e-12 * np.abs(Rr).max())
+        Q = orc.qr_form_q(f, tau, m, n).reshape(n, m).T
+        assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13
+        # A = QR compared after scaling both by the matrix's norm (products of 1e305 entries would overflow)
+        s = 1.0 / np.abs(M).max()
+        assert np.linalg.norm(M * s - Q @ (Rg * s)) / np.linalg.norm(M * s) < 1e-13
+
+
 def test_batched_largest_tiles_and_special_columns():
     # The largest accepted tiles (m*n = 20220 and n = 256), beside matrices whose first column is zero,
     # zero below a negative pivot, or below the FP64 minimum.
