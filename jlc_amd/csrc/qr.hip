@@ -33,6 +33,14 @@ constexpr int PCHUNK = 512;   // panel rows per workgroup
 constexpr int PTHREADS = 512;
 constexpr int PWAVES = PTHREADS / 64;
 
+// 1/d from the hardware estimate refined by two Newton steps (within an ulp or two of the IEEE quotient)
+// for normal d: the batched kernels' reflector reciprocals sit on each column's critical path.
+__device__ __forceinline__ double recip_nr(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -399,23 +407,27 @@ __global__ __launch_bounds__(64) void qr_batched_wave(const QrBatchDesc* __restr
         const double M = fmax(wave_max(mx), fabs(x0));
         double t = 0.0, beta = x0, v0inv = 0.0;
         if (k < m - 1 && M >= 2.2250738585072014e-308) {
+            // x / M as (x s) * (1 / (M s)), s = 2^-100 when M > 2^1000 (else 1), and sqrt(x0n^2 + sum) for
+            // the hypotenuse of the scaled values (at least 1): no division on the column's critical path
+            const double sc = M > 0x1p1000 ? 0x1p-100 : 1.0;
+            const double inv = recip_nr(M * sc);
             double sq = 0.0;
             for (int i = k + 1 + lane; i < m; i += 64) {
-                const double y = col[i] / M;
+                const double y = (col[i] * sc) * inv;
                 sq += y * y;
             }
-            const double xnorm = sqrt(wave_sum(sq));
-            const double x0n = x0 / M;
-            if (xnorm == 0.0) {
+            sq = wave_sum(sq);
+            const double x0n = (x0 * sc) * inv;
+            if (sq == 0.0) {
                 if (x0n < 0.0) {  // H = I - 2 e_k e_k^T negates the column's pivot
                     t = 2.0;
                     beta = -x0;
                 }
             } else {
-                const double nbeta = -copysign(hypot(x0n, xnorm), x0n);
+                const double nbeta = -copysign(sqrt(fma(x0n, x0n, sq)), x0n);
                 beta = nbeta * M;
-                t = (nbeta - x0n) / nbeta;
-                v0inv = 1.0 / ((x0n - nbeta) * M);
+                t = (nbeta - x0n) * recip_nr(nbeta);
+                v0inv = recip_nr((x0n - nbeta) * M);
             }
         }
         if (v0inv != 0.0)
@@ -464,14 +476,22 @@ __global__ __launch_bounds__(64) void qr_batched_wave(const QrBatchDesc* __restr
     }
 }
 
+// lo / hi for 0 <= lo <= hi, hi > 0, as (lo s) * (1 / (hi s)): both scaled by a power of two that keeps
+// hi s inside [2^-900, 2^900], so its reciprocal is a normal number (no division: these ratios sit on
+// the next pivot's critical path).
+__device__ __forceinline__ double ratio_le1(double lo, double hi) {
+    const double sc = hi < 0x1p-900 ? 0x1p100 : (hi > 0x1p900 ? 0x1p-100 : 1.0);
+    return (lo * sc) * recip_nr(hi * sc);
+}
+
 // (max |x|, sum (x / max)^2) of two parts of a column, merged (a max of 0 is an empty part).
 __device__ __forceinline__ void merge_norm(double& m1, double& s1, double m2, double s2) {
     if (m2 > m1) {
-        const double r = m1 / m2;
+        const double r = ratio_le1(m1, m2);
         s1 = s2 + s1 * r * r;
         m1 = m2;
     } else if (m2 > 0.0) {
-        const double r = m2 / m1;
+        const double r = ratio_le1(m2, m1);
         s1 += s2 * r * r;
     }
 }
@@ -530,19 +550,22 @@ __global__ __launch_bounds__(NT) void qr_batched_block(const QrBatchDesc* __rest
         const double M = fmax(mx, fabs(x0));
         double t = 0.0, beta = x0, v0inv = 0.0;
         if (k < m - 1 && M >= 2.2250738585072014e-308) {
-            const double r = mx / M;
+            // scaled as in qr_batched_wave: no division on the column's critical path
+            const double sc = M > 0x1p1000 ? 0x1p-100 : 1.0;
+            const double inv = recip_nr(M * sc);
+            const double r = (mx * sc) * inv;
             const double xnorm = sqrt(sqm * r * r);
-            const double x0n = x0 / M;
+            const double x0n = (x0 * sc) * inv;
             if (xnorm == 0.0) {
                 if (x0n < 0.0) {  // H = I - 2 e_k e_k^T negates the column's pivot
                     t = 2.0;
                     beta = -x0;
                 }
             } else {
-                const double nbeta = -copysign(hypot(x0n, xnorm), x0n);
+                const double nbeta = -copysign(sqrt(fma(x0n, x0n, xnorm * xnorm)), x0n);
                 beta = nbeta * M;
-                t = (nbeta - x0n) / nbeta;
-                v0inv = 1.0 / ((x0n - nbeta) * M);
+                t = (nbeta - x0n) * recip_nr(nbeta);
+                v0inv = recip_nr((x0n - nbeta) * M);
             }
         }
         if (k > 0 && prev_scale != 0.0) {  // column k - 1 becomes v (no one reads it in this step)
@@ -622,14 +645,6 @@ __global__ __launch_bounds__(NT) void qr_batched_block(const QrBatchDesc* __rest
 // every thread reads v at its rows unconditionally, forms its column's dot product by three shuffles
 // and updates its column in registers. The owners of column k + 1 then form reflector k + 1 before the
 // next barrier: one barrier per column.
-// 1/d from the hardware estimate refined by two Newton steps (within an ulp or two of the IEEE quotient)
-// for normal d: the reflector's reciprocals sit on the per-column critical path of qr_batched_reg.
-__device__ __forceinline__ double recip_nr(double d) {
-    double r = __builtin_amdgcn_rcp(d);
-    r = fma(r, fma(-d, r, 1.0), r);
-    return fma(r, fma(-d, r, 1.0), r);
-}
-
 template <int R>
 __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __restrict__ descs) {
     // v of the current reflector, thread-major: row sub + 8r at [sub * (R + 2) + r], so each thread's R rows
