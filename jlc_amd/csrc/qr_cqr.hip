@@ -782,6 +782,8 @@ __global__ __launch_bounds__(CQ_NT) void wy_t_inverse(const double* __restrict__
     }
     __syncthreads();
     const int c = t & 127;
+    // the diagonal entry of column c, 1 / tau_c (the one division of this thread)
+    const double diag_c = (c < b && !zt[c]) ? 1.0 / tau[c] : 1.0;
     {
         double v[32];
 #pragma unroll
@@ -795,7 +797,7 @@ __global__ __launch_bounds__(CQ_NT) void wy_t_inverse(const double* __restrict__
             if (r >= nb16 || c >= nb16) continue;
             double x;
             if (r < b && c < b) {
-                if (r == c) x = zt[r] ? 1.0 : 1.0 / tau[r];
+                if (r == c) x = diag_c;
                 else x = (r < c && !zt[r] && !zt[c]) ? v[q] : 0.0;
             } else {
                 x = r == c ? 1.0 : 0.0;
