@@ -228,8 +228,10 @@ int jlc_dgemm_grouped_dev(int64_t count, const jlc_gemm_desc* descs, int mode, v
  * table's qr_cqr_min_rows rows and twice its width is factored by CholeskyQR2
  * with Householder reconstruction (the same reflectors, tau and R in exact
  * arithmetic; a panel whose Cholesky breaks down or loses orthogonality runs
- * the column kernels instead); stats->tile_n counts the panels that took that
- * path, stats->tile_k the panels factored column by column. */
+ * the column kernels instead); a panel of at most qr_reg_max_m rows and 128
+ * columns that does not take it is factored column by column on one workgroup,
+ * others by the column kernels; stats->tile_n counts the CholeskyQR2 panels,
+ * stats->tile_k the panels factored column by column. */
 int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, int64_t nb, int mode, void* stream,
                    jlc_stats* stats);
 /* Thin Q (column-major m x n, ldq) from the factors (buildQ, HouseholderQR.java:390-444), n <= m. */

@@ -646,13 +646,12 @@ __global__ __launch_bounds__(NT) void qr_batched_block(const QrBatchDesc* __rest
 // and updates its column in registers. The owners of column k + 1 then form reflector k + 1 before the
 // next barrier: one barrier per column.
 template <int R>
-__global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __restrict__ descs) {
+__device__ __forceinline__ void qr_reg_factor(const QrBatchDesc d) {
     // v of the current reflector, thread-major: row sub + 8r at [sub * (R + 2) + r], so each thread's R rows
     // are contiguous (16-byte LDS reads) and the eight threads of a column start on different banks
     constexpr int VS = R + 2;
     __shared__ __attribute__((aligned(16))) double vbuf[2][8 * VS];
     __shared__ double tbuf[2];
-    const QrBatchDesc d = descs[blockIdx.x];
     const int m = d.m, n = d.n, kmax = m < n ? m : n;
     const int c = threadIdx.x >> 3, sub = threadIdx.x & 7;
     const bool own = c < n;
@@ -754,6 +753,19 @@ __global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __rest
             if (i < m) d.a[(int64_t)c * d.lda + i] = a[r];
         }
     }
+}
+
+// A batch: one workgroup (8 threads per column) per matrix.
+template <int R>
+__global__ __launch_bounds__(1024) void qr_batched_reg(const QrBatchDesc* __restrict__ descs) {
+    qr_reg_factor<R>(descs[blockIdx.x]);
+}
+
+// One panel of the blocked QR (m <= 8R rows, n <= 128 columns) on one workgroup: one launch instead of the
+// column kernels' n + 1.
+template <int R>
+__global__ __launch_bounds__(1024) void qr_panel_reg(QrBatchDesc d) {
+    qr_reg_factor<R>(d);
 }
 
 // Upper trapezoids of stacked row blocks: dst(s, j), s < rows_total, column-major (ldd), is row
@@ -944,6 +956,23 @@ hipError_t launch_qr_batched_reg(const QrBatchDesc* descs, int count, int rows_p
         case 16: hipLaunchKernelGGL(qr_batched_reg<16>, dim3(count), dim3(threads), 0, stream, descs); break;
         case 20: hipLaunchKernelGGL(qr_batched_reg<20>, dim3(count), dim3(threads), 0, stream, descs); break;
         case 24: hipLaunchKernelGGL(qr_batched_reg<24>, dim3(count), dim3(threads), 0, stream, descs); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_qr_panel_reg(double* a, int64_t lda, int m, int n, double* tau, hipStream_t stream) {
+    if (m < 1 || n < 1 || n > 128 || m > 192) return hipErrorInvalidValue;
+    const int rows = ((m + 31) / 32) * 4;  // rows per thread: m <= 8 * rows
+    const int threads = ((n * 8 + 63) / 64) * 64;
+    const QrBatchDesc d{m, n, lda, a, tau};
+    switch (rows) {
+        case 4: hipLaunchKernelGGL(qr_panel_reg<4>, dim3(1), dim3(threads), 0, stream, d); break;
+        case 8: hipLaunchKernelGGL(qr_panel_reg<8>, dim3(1), dim3(threads), 0, stream, d); break;
+        case 12: hipLaunchKernelGGL(qr_panel_reg<12>, dim3(1), dim3(threads), 0, stream, d); break;
+        case 16: hipLaunchKernelGGL(qr_panel_reg<16>, dim3(1), dim3(threads), 0, stream, d); break;
+        case 20: hipLaunchKernelGGL(qr_panel_reg<20>, dim3(1), dim3(threads), 0, stream, d); break;
+        case 24: hipLaunchKernelGGL(qr_panel_reg<24>, dim3(1), dim3(threads), 0, stream, d); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

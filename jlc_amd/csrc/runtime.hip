@@ -1704,19 +1704,26 @@ int jlc_dgeqrf_dev(int64_t m, int64_t n, double* a, int64_t lda, double* tau, in
     }
     int cqr_panels = 0, col_panels = 0;
     std::vector<size_t> timed;  // event pairs recorded (pairs with a trailing update)
-    // Factors panel [k0, k1) in place (CholeskyQR2 where it applies, else the column kernels).
+    const int64_t reg_rows = std::min<int64_t>(jlc::tune_snapshot().qr_reg_max_m, 192);
+    // Factors panel [k0, k1) in place: CholeskyQR2 where it applies, else column by column (on one
+    // workgroup, the batched QR's register kernel, when the panel has at most qr_reg_max_m rows; else the
+    // column kernels).
     const auto factor = [&](int64_t k0, int64_t k1) -> int {
         bool done = false;
         int r = JLC_OK;
         if (cw.q && cqr_panel(k0, k1 - k0) && (r = panel_cqr(a, lda, m, k0, k1 - k0, tau, cw, s, &info, &done)) != JLC_OK)
             return r;
-        if (!done) {
+        if (done) {
+            cqr_panels++;
+        } else if (m - k0 <= reg_rows && k1 - k0 <= 128) {
+            JLC_HIP_CHECK(jlc::launch_qr_panel_reg(a + k0 * lda + k0, lda, (int)(m - k0), (int)(k1 - k0), tau + k0, s));
+            info.launches += 1;
+            col_panels++;
+        } else {
             JLC_HIP_CHECK(jlc::launch_qr_panel_fused(a, lda, (int)m, (int)k0, (int)k1, tau, part.d(), dots.d(),
                                                      vtmp.d(), s));
             info.launches += (int)(k1 - k0) + 1;
             col_panels++;
-        } else {
-            cqr_panels++;
         }
         return JLC_OK;
     };

@@ -148,6 +148,31 @@ def test_cholesky_qr2_panels_against_oracle():
               f"{np.abs(outs[0] - outs[1]).max():.2e}", flush=True)
 
 
+def test_register_panels_against_oracle_and_column_kernels():
+    """Panels of at most qr_reg_max_m rows (and 128 columns) that CholeskyQR2 does not take are factored on
+    one workgroup by the batched QR's register kernel: tau, R and the Householder vectors as the oracle's
+    (and as the column kernels', qr_reg_max_m 0), A = QR and Q^T Q = I at FP64 level."""
+    rng = np.random.default_rng(103)
+    saved = native.tune_get("qr_reg_max_m")
+    for (m, n, nb) in [(300, 300, 128), (192, 150, 64), (250, 200, 128), (160, 96, 96), (520, 300, 128)]:
+        A = rng.standard_normal((m, n))
+        kk = min(m, n)
+        Fr, tau_ref = _oracle_factors(A, nb)
+        for reg in (saved, 0):
+            native.tune_set("qr_reg_max_m", reg)
+            try:
+                da, d0, dt, st = _factor(A, nb=nb)
+            finally:
+                native.tune_set("qr_reg_max_m", saved)
+            F, tau = da.to_numpy().reshape(n, m).T, dt.to_numpy()
+            np.testing.assert_allclose(tau, tau_ref, rtol=1e-10, atol=1e-12)
+            np.testing.assert_allclose(np.triu(F[:kk]), np.triu(Fr[:kk]), rtol=0, atol=1e-10 * np.abs(Fr).max())
+            np.testing.assert_allclose(np.tril(F[:, :kk], -1), np.tril(Fr[:, :kk], -1), rtol=0, atol=1e-10)
+            recon, orth = _errors(A, da, d0, dt, nb)
+            assert recon < 1e-14 and orth < 1e-13, (m, n, nb, reg, recon, orth)
+            assert st.tile_n + st.tile_k == (kk + nb - 1) // nb, (m, n, nb, reg)
+
+
 def test_cholesky_qr2_special_columns_and_fallback():
     """Upper-triangular panels (every column zero below its pivot: the reference's tau = 0 case for a
     non-negative pivot, tau = 2 with the row negated for a negative one) come out of the reconstruction
