@@ -153,6 +153,9 @@ __device__ void upper_inverse_blocked(double* S, double* dblk, const double* din
         if (w == CQ_W - 1) tile_store(S + 16 * ib * (SLD + 1), SLD, tile_load(dblk + ib * 256, 16), true);
         __syncthreads();
     }
+    // With one block (b <= 16) the loop above does not run: the last diagonal inverse, stored by wavefront
+    // 0 alone, must be complete before any wavefront reads S.
+    if (nblk == 1) __syncthreads();
 }
 
 // acc = sum_{lb = ib..cb} A(ib, lb) * R1(lb, cb) with A from S and R1 row-major b x b in global memory

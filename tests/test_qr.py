@@ -148,6 +148,28 @@ def test_cholesky_qr2_panels_against_oracle():
               f"{np.abs(outs[0] - outs[1]).max():.2e}", flush=True)
 
 
+def test_narrow_cholesky_qr2_panels_against_oracle():
+    """CholeskyQR2 panels at most 16 wide (every panel at nb = 16 or 8, and ragged 8- and 12-column last
+    panels): their triangular inverses (X = R^-1, U^-1, T) are single 16 x 16 blocks, whose result one
+    wavefront stores before the others read it. tau, R and the Householder vectors as the oracle's."""
+    rng = np.random.default_rng(104)
+    for (m, n, nb) in [(600, 160, 16), (512, 40, 8), (1000, 264, 128), (777, 520, 32), (900, 300, 96)]:
+        A = rng.standard_normal((m, n))
+        kk = min(m, n)
+        Fr, tau_ref = _oracle_factors(A, nb)
+        want, panels = _cqr_panels(m, n, nb)
+        for rep in range(3):  # the same factors on every run
+            da, d0, dt, st = _factor(A, nb=nb)
+            assert st.tile_n == want and want > 0, (m, n, nb, st.tile_n, want)
+            F, tau = da.to_numpy().reshape(n, m).T, dt.to_numpy()
+            np.testing.assert_allclose(tau, tau_ref, rtol=1e-10, atol=1e-12)
+            np.testing.assert_allclose(np.triu(F[:kk]), np.triu(Fr[:kk]), rtol=0, atol=1e-10 * np.abs(Fr).max())
+            np.testing.assert_allclose(np.tril(F[:, :kk], -1), np.tril(Fr[:, :kk], -1), rtol=0, atol=1e-10,
+                                       err_msg=f"{(m, n, nb)} run {rep}")
+            recon, orth = _errors(A, da, d0, dt, nb)
+            assert recon < 1e-14 and orth < 1e-13, (m, n, nb, recon, orth)
+
+
 def test_register_panels_against_oracle_and_column_kernels():
     """Panels of at most qr_reg_max_m rows (and 128 columns) that CholeskyQR2 does not take are factored on
     one workgroup by the batched QR's register kernel: tau, R and the Householder vectors as the oracle's
