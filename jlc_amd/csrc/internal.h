@@ -168,6 +168,8 @@ struct QrBatchDesc {
 // One blocked-QR panel (column-major m x n at a, lda; m <= 192, n <= 128) factored in place on one
 // workgroup by the register kernel of the batched QR (same reflectors and tau as the column kernels).
 hipError_t launch_qr_panel_reg(double* a, int64_t lda, int m, int n, double* tau, hipStream_t stream);
+// out[0..1] (device memory) = max(0, a[0..m)), max(0, b[0..n)): one workgroup (the count choosers' inputs).
+hipError_t launch_max2_nonneg(const float* a, int64_t m, const float* b, int64_t n, float* out, hipStream_t stream);
 int qr_panel_chunk();
 // Factors panel columns [k0, k1) of the column-major m-row matrix at (in place, tau[k0..k1)), one
 // launch per column; part holds 2 * (2G + 1 + 128) doubles, dots 2 * G * 128, xbuf 2 * m

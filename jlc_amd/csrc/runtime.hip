@@ -101,11 +101,16 @@ struct ThreadCtx {
     // Blocked QR: pinned host copy of a CholeskyQR2 panel's flag and the event that marks its arrival.
     int* qr_flag = nullptr;
     hipEvent_t qr_ev = nullptr;
+    // Split forms' count choosers: the two statistic maxima, on the device and their pinned host copy.
+    float* choose_dev = nullptr;
+    float* choose_pin = nullptr;
 
     ~ThreadCtx() {
         if (!g_alive.load() || device < 0) return;
         if (pin) (void)hipHostFree(pin);
         if (qr_flag) (void)hipHostFree(qr_flag);
+        if (choose_pin) (void)hipHostFree(choose_pin);
+        if (choose_dev) (void)hipFree(choose_dev);
         if (qr_ev) (void)hipEventDestroy(qr_ev);
         for (int i = 0; i < NBUF; i++)
             if (buf[i]) (void)hipFree(buf[i]);
@@ -357,11 +362,8 @@ int validate_gemm(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha,
 // spread_b are the largest row / column gaps (in bits) between the scale 2^e
 // and the typical |x| (geometric mean), so matching FP64 needs
 // 7S >= 53 + spread_a + spread_b - 4.
-int choose_slices(const float* spread_a, int64_t m, const float* spread_b, int64_t n, int digit_bits,
-                  int max_slices) {
-    float sa = 0.0f, sb = 0.0f;
-    for (int64_t i = 0; i < m; i++) sa = spread_a[i] > sa ? spread_a[i] : sa;
-    for (int64_t j = 0; j < n; j++) sb = spread_b[j] > sb ? spread_b[j] : sb;
+// Slice count from the largest row spread of op(A) (sa) and column spread of op(B) (sb), both >= 0.
+int choose_slices(float sa, float sb, int digit_bits, int max_slices) {
     double bits = 53.0 + sa + sb - 4.0;
     int s = (int)std::ceil(bits / digit_bits);
     if (s < jlc::SPLIT_MIN_SLICES) s = jlc::SPLIT_MIN_SLICES;
@@ -369,6 +371,21 @@ int choose_slices(const float* spread_a, int64_t m, const float* spread_b, int64
 }
 
 int ensure_buf(int slot, size_t bytes);
+
+// max(0, a[0..m)) and max(0, b[0..n)) of device statistics (v > cur, so NaN is never taken) for the count
+// choosers: reduced on the device and read back as 8 bytes of pinned memory, so the host waits for one
+// small copy rather than copying and scanning one statistic per row and column.
+int device_max2(const float* a, int64_t m, const float* b, int64_t n, hipStream_t s, float* ma, float* mb) {
+    ThreadCtx& ctx = t_ctx;
+    if (!ctx.choose_dev) JLC_HIP_CHECK(hipMalloc((void**)&ctx.choose_dev, 256));
+    if (!ctx.choose_pin) JLC_HIP_CHECK(hipHostMalloc((void**)&ctx.choose_pin, 2 * sizeof(float), hipHostMallocDefault));
+    JLC_HIP_CHECK(jlc::launch_max2_nonneg(a, m, b, n, ctx.choose_dev, s));
+    JLC_HIP_CHECK(hipMemcpyAsync(ctx.choose_pin, ctx.choose_dev, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+    JLC_HIP_CHECK(hipStreamSynchronize(s));
+    *ma = ctx.choose_pin[0];
+    *mb = ctx.choose_pin[1];
+    return JLC_OK;
+}
 
 // Runs a slice form; in automatic mode *fallback is set (nothing launched
 // beyond the statistics) when the spread needs more slices than the form has.
@@ -396,11 +413,9 @@ int run_split(bool bf16, int la, int lb, int64_t m, int64_t n, int64_t k, double
     p.stats_part = reinterpret_cast<double*>(sb + stat_bytes);
     JLC_HIP_CHECK(jlc::launch_split_stats(la, lb, a, lda, b, ldb, &p, stream));
     if (slices == 0) {
-        std::vector<float> host((size_t)(p.mpad + p.npad));
-        JLC_HIP_CHECK(hipMemcpyAsync(host.data(), p.spread_a, host.size() * sizeof(float), hipMemcpyDeviceToHost,
-                                     stream));
-        JLC_HIP_CHECK(hipStreamSynchronize(stream));
-        slices = choose_slices(host.data(), m, host.data() + p.mpad, n, bf16 ? 8 : 7, max_slices);
+        float sa = 0.0f, sbm = 0.0f;
+        if ((rc = device_max2(p.spread_a, m, p.spread_b, n, stream, &sa, &sbm)) != JLC_OK) return rc;
+        slices = choose_slices(sa, sbm, bf16 ? 8 : 7, max_slices);
         if (slices == 0) {
             *fallback = true;
             return JLC_OK;
@@ -512,13 +527,8 @@ int run_crt(int la, int lb, int64_t m, int64_t n, int64_t k, double alpha, const
     JLC_HIP_CHECK(jlc::launch_crt_stats_norms(la, lb, a, lda, b, ldb, sp, ta, tb, sig_a, sig_b, stream));
     int pa, pb;
     if (nmod == 0) {
-        std::vector<float> host((size_t)(p.mpad + p.npad));
-        JLC_HIP_CHECK(hipMemcpyAsync(host.data(), sig_a, host.size() * sizeof(float), hipMemcpyDeviceToHost,
-                                     stream));
-        JLC_HIP_CHECK(hipStreamSynchronize(stream));
         float sa = 0.0f, sbm = 0.0f;
-        for (int64_t i = 0; i < m; i++) sa = std::max(sa, host[i]);
-        for (int64_t j = 0; j < n; j++) sbm = std::max(sbm, host[p.mpad + j]);
+        if ((rc = device_max2(sig_a, m, sig_b, n, stream, &sa, &sbm)) != JLC_OK) return rc;
         nmod = crt_choose(sa, sbm, k, &pa, &pb);
         if (nmod == 0) {
             *fallback = true;
