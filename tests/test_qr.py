@@ -170,6 +170,30 @@ def test_narrow_cholesky_qr2_panels_against_oracle():
             assert recon < 1e-14 and orth < 1e-13, (m, n, nb, recon, orth)
 
 
+def test_random_shapes_and_panel_widths_against_oracle():
+    """Seeded shapes across every panel path (CholeskyQR2, one workgroup, column kernels), panel pairs and
+    ragged last panels: m, n in [1, 1100] (tall, square and wide), nb in {8, ..., 128}; tau, R and the
+    Householder vectors as the oracle's."""
+    rng = np.random.default_rng(105)
+    for _ in range(24):
+        m = int(rng.integers(1, 1101))
+        n = int(rng.integers(1, 601)) if rng.random() < 0.7 else int(rng.integers(1, 1101))
+        nb = int(rng.choice([8, 16, 24, 32, 48, 64, 96, 128]))
+        A = rng.standard_normal((m, n))
+        kk = min(m, n)
+        Fr, tau_ref = _oracle_factors(A, nb)
+        da, d0, dt, st = _factor(A, nb=nb)
+        F, tau = da.to_numpy().reshape(n, m).T, dt.to_numpy()
+        tag = f"{(m, n, nb)} panels {st.tile_n}/{st.tile_k}"
+        np.testing.assert_allclose(tau, tau_ref, rtol=1e-10, atol=1e-12, err_msg=tag)
+        np.testing.assert_allclose(np.triu(F[:kk]), np.triu(Fr[:kk]), rtol=0, atol=1e-10 * np.abs(Fr).max(),
+                                   err_msg=tag)
+        np.testing.assert_allclose(np.tril(F[:, :kk], -1), np.tril(Fr[:, :kk], -1), rtol=0, atol=1e-10, err_msg=tag)
+        if m >= n:
+            recon, orth = _errors(A, da, d0, dt, nb)
+            assert recon < 1e-14 and orth < 1e-13, (tag, recon, orth)
+
+
 def test_register_panels_against_oracle_and_column_kernels():
     """Panels of at most qr_reg_max_m rows (and 128 columns) that CholeskyQR2 does not take are factored on
     one workgroup by the batched QR's register kernel: tau, R and the Householder vectors as the oracle's
