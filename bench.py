@@ -577,6 +577,134 @@ def bench_qr_batched(args) -> int:
     return 0
 
 
+def _wy_panel(rng: np.random.Generator, rows: int, kb: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit lower-trapezoidal V (rows x kb, row-major) with tau_j = 2 / v_j^T v_j and T from buildT's
+    recurrence T[:i, i] = T[:i, :i] (-tau_i V[:, :i]^T v_i) (BlockedHessenbergQR.java:239-274)."""
+    V = np.tril(0.3 * rng.standard_normal((rows, kb)), -1)
+    V[np.arange(kb), np.arange(kb)] = 1.0
+    G = V.T @ V
+    tau = 2.0 / np.diag(G)
+    T = np.zeros((kb, kb))
+    for i in range(kb):
+        T[i, i] = tau[i]
+        if i:
+            T[:i, i] = T[:i, :i] @ (-tau[i] * G[:i, i])
+    return np.ascontiguousarray(V).reshape(-1), T.reshape(-1).copy()
+
+
+def bench_wy(args) -> int:
+    """The compact-WY trailing updates of two n = 4096 reductions, every panel in order, on device pointers:
+    BlockedHessenbergQR (panel 32: H from the left, H and Q from the right, BlockedHessenbergQR.java:120-137)
+    and BlockedBidiagonalization (panel 64: B[k:, k+pw:] from the left and the right,
+    BlockedBidiagonalization.java:107-122, 296-316, 498-510). The panels' V and T are generated and
+    resident before timing; a step is one whole sweep of updates (the panel factorizations are not timed)."""
+    from oracle import oracle as orc
+    n = 4096
+    rng = np.random.default_rng(args.seed)
+    A = rng.uniform(-1.0, 1.0, n * n)
+    Q0 = np.eye(n).reshape(-1)
+    sweeps = {}
+    # Hessenberg: panel k, m = n - k - 1 reflector rows; H[k+1:, k:] <- Q^T H, H[:, k+1:] <- H Q, Q[:, k+1:] <- Q Q_k
+    ups, vts = [], []
+    for k in range(0, n - 2, 32):
+        pw, m = min(32, n - 2 - k), n - k - 1
+        vts.append(_wy_panel(rng, m, pw))
+        p = len(vts) - 1
+        ups += [("h", native.LEFT, 1, k + 1, k, m, n - k, pw, p), ("h", native.RIGHT, 0, 0, k + 1, n, m, pw, p),
+                ("q", native.RIGHT, 0, 0, k + 1, n, m, pw, p)]
+    sweeps["hessenberg"] = (32, ups, vts)
+    # bidiagonalization: panel k, B[k:, k+pw:] from the left (V: n-k rows) and the right (V: n-k-pw rows)
+    ups, vts = [], []
+    for k in range(0, n, 64):
+        pw = min(64, n - k)
+        if k + pw >= n:
+            break
+        vts.append(_wy_panel(rng, n - k, pw))
+        ups.append(("h", native.LEFT, 0, k, k + pw, n - k, n - k - pw, pw, len(vts) - 1))
+        vts.append(_wy_panel(rng, n - k - pw, pw))
+        ups.append(("h", native.RIGHT, 0, k, k + pw, n - k, n - k - pw, pw, len(vts) - 1))
+    sweeps["bidiagonal"] = (64, ups, vts)
+
+    d_a0, d_q0 = native.DeviceArray.from_numpy(A), native.DeviceArray.from_numpy(Q0)
+    d_h, d_q = native.DeviceArray(n * n), native.DeviceArray(n * n)
+    stream = native.stream()
+    out = {}
+    for name, (kb, ups, vts) in sweeps.items():
+        dv = [(native.DeviceArray.from_numpy(v), native.DeviceArray.from_numpy(t)) for v, t in vts]
+        flops = sum(4.0 * r * c * w + 2.0 * w * w * (c if side == native.LEFT else r)
+                    for _, side, _, _, _, r, c, w, _ in ups)
+        nbytes = sum(16.0 * r * c for _, _, _, _, _, r, c, _, _ in ups)
+
+        def run(count=None):
+            for tgt, side, tt, r0, c0, r, c, w, p in ups[:count]:
+                base = d_h if tgt == "h" else d_q
+                native.wy_apply_dev(side, tt, r, c, w, dv[p][0].ptr, w, dv[p][1].ptr, w, base.at(r0 * n + c0), n,
+                                    native.MODE_FP64, stream, stats=False)
+
+        def reset():
+            native.dev_copy(d_h.ptr, d_a0.ptr, n * n * 8)
+            native.dev_copy(d_q.ptr, d_q0.ptr, n * n * 8)
+            native.synchronize()
+
+        times = []
+        for it in range(args.warmup + max(1, args.steps)):
+            reset()
+            t1 = time.perf_counter()
+            run()
+            native.synchronize()
+            if it >= args.warmup:
+                times.append((time.perf_counter() - t1) * 1e3)
+        ms = float(np.median(times))
+        # parity and the CPU path on the first updates of the sweep (two panels)
+        sample = 6 if name == "hessenberg" else 4
+        reset()
+        run(sample)
+        native.synchronize()
+        got_h, got_q = d_h.to_numpy(), d_q.to_numpy()
+        exp_h, exp_q = A.copy(), Q0.copy()
+        t1 = time.perf_counter()
+        for tgt, side, tt, r0, c0, r, c, w, p in ups[:sample]:
+            v, t = vts[p]
+            orc.wy_apply(side, tt, r, c, w, v, w, t, w, exp_h if tgt == "h" else exp_q, n, c_off=r0 * n + c0)
+        cpu_ms = (time.perf_counter() - t1) * 1e3
+        cpu_flops = sum(4.0 * r * c * w + 2.0 * w * w * (c if side == native.LEFT else r)
+                        for _, side, _, _, _, r, c, w, _ in ups[:sample])
+        err = max(float(np.max(np.abs(g - e)) / max(1e-300, float(np.max(np.abs(e)))))
+                  for g, e in ((got_h, exp_h), (got_q, exp_q)))
+        out[name] = {"panel": kb, "updates": len(ups), "launches": 3 * len(ups), "ms": round(ms, 3),
+                     "tflops": round(flops / (ms * 1e-3) / 1e12, 3), "flops": flops,
+                     "algorithmic_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "sample_updates": sample, "sample_max_rel_err": err,
+                     "cpu_ms": round(cpu_ms, 1), "cpu_gflops": round(cpu_flops / (cpu_ms * 1e-3) / 1e9, 2)}
+        log(f"{name}: {out[name]}")
+        del dv
+    hs, bd = out["hessenberg"], out["bidiagonal"]
+    total_ms = hs["ms"] + bd["ms"]
+    total_flops = hs["flops"] + bd["flops"]
+    cpu_gflops = (hs["cpu_gflops"] + bd["cpu_gflops"]) / 2
+    print(json.dumps({
+        "metric": "compact-WY trailing updates of n=4096 Hessenberg (panel 32) and bidiagonal (panel 64) sweeps",
+        "value": round(total_flops / (total_ms * 1e-3) / 1e12, 3), "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total_ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "wy sweeps n=4096: Hessenberg panel 32 (3 updates per panel), bidiagonal panel 64 "
+                               "(2 updates per panel)", "flops": total_flops},
+        "sweeps": out,
+        # Hessenberg updates (panel 32) carry kb/4 = 8 flop per algorithmic byte, below FP64's ridge of
+        # 78.6 / 8 = 9.8: that sweep is priced against HBM with C read and written once per update
+        "roofline": {"bound": "hbm", "achieved": hs["algorithmic_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(hs["algorithmic_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "wy_apply_dev Hessenberg sweep (3 dgemm_f64 launches per update)",
+                     "kernel_ms": hs["ms"],
+                     "note": "algorithmic bytes 16 * rows * cols per update (C read and written once); the three-"
+                             "product form reads C twice, so a sweep at this figure's peak is out of reach"},
+        "cpu_baseline": {"value": round(cpu_gflops / 1e3, 4), "unit": "TFLOP/s", "cores": orc.threads(),
+                         "kind": "port", "sample": "first 2 panels of each sweep (10 updates), oracle "
+                                                   f"applyBlockLeft/Right restatement, {orc.threads()} threads"}}),
+        flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -595,7 +723,7 @@ def main() -> int:
     ap.add_argument("--forms", default="", help="split forms to time, comma list of crt,int8,bf16 (default all)")
     ap.add_argument("--host-path", action="store_true",
                     help="also time the host-pointer entry point (PCIe staging included) on the value form")
-    ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched"], default="dgemm",
+    ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched", "wy"], default="dgemm",
                     help="dgemm: the 16384^3 metric workload; batched: the grouped-GEMM configuration")
     ap.add_argument("--count", type=int, default=100000, help="problems in the batched workload")
     ap.add_argument("--no-per-call", action="store_true",
@@ -630,6 +758,8 @@ def main() -> int:
         return bench_qr(args)
     if args.workload == "qr_batched":
         return bench_qr_batched(args)
+    if args.workload == "wy":
+        return bench_wy(args)
 
     m = n = k = args.size
     da, db, dc = native.DeviceArray(m * k), native.DeviceArray(k * n), native.DeviceArray(m * n)

@@ -81,6 +81,7 @@ extern "C" {
 #define JLC_KERNEL_GROUPED 6    /* persistent grouped kernel */
 #define JLC_KERNEL_QR 7         /* Householder QR (panel kernels + compact-WY GEMMs) */
 #define JLC_KERNEL_SPLIT_CRT 8  /* modular INT8 products, CRT reconstruction in FP64 */
+#define JLC_KERNEL_WY 9         /* compact-WY kernels (partial W, W2, update), kb <= 64 */
 
 typedef struct jlc_stats {
     double kernel_ms;   /* device time of the call's kernels (HIP events) */

@@ -50,6 +50,8 @@ struct TuneTable {
                                       // in registers; larger ones in LDS
     int qr_cqr_min_rows = 256;        // blocked QR: panels of at least this many rows (and 2 * width)
                                       // by CholeskyQR2 + reconstruction (0 = column kernels only)
+    int wy_left_wgs_per_cu = 1;       // compact-WY updates (kb <= 64, FP64): partial-W workgroups per CU,
+    int wy_right_wgs_per_cu = 2;      // left and right side (0 = the three-GEMM sequence for that side)
 };
 TuneTable tune_snapshot();
 
@@ -170,6 +172,23 @@ struct QrBatchDesc {
 hipError_t launch_qr_panel_reg(double* a, int64_t lda, int m, int n, double* tau, hipStream_t stream);
 // out[0..1] (device memory) = max(0, a[0..m)), max(0, b[0..n)): one workgroup (the count choosers' inputs).
 hipError_t launch_max2_nonneg(const float* a, int64_t m, const float* b, int64_t n, float* out, hipStream_t stream);
+// Compact-WY update kernels for 1 <= kb <= 64 (wy.hip): side 0 (left) C <- C - V op(T) V^T C,
+// side 1 (right) C <- C - C V op(T) V^T. wy_plan sizes the partial-W slices for a grid of about
+// device_cus * wgs_per_cu workgroups (c, ldc: the block, for its line alignment); part holds part_elems
+// doubles, w2 w2_elems.
+struct WyPlan {
+    int kb_pad = 0;          // 16 * ceil(kb / 16)
+    int slices = 0;          // partial-W slots
+    int span = 0;            // rows (left) or columns (right) per slot
+    int cshift = 0;          // C's column tiles start this many columns before the block (line alignment)
+    int64_t part_elems = 0;
+    int64_t w2_elems = 0;
+};
+WyPlan wy_plan(int side, int64_t m, int64_t n, int64_t kb, const double* c, int64_t ldc, int device_cus,
+               int wgs_per_cu);
+hipError_t launch_wy_update(int side, int trans_t, int m, int n, int kb, const double* v, int64_t ldv,
+                            const double* t, int64_t ldt, double* c, int64_t ldc, const WyPlan& plan, double* part,
+                            double* w2, hipStream_t stream);
 int qr_panel_chunk();
 // Factors panel columns [k0, k1) of the column-major m-row matrix at (in place, tau[k0..k1)), one
 // launch per column; part holds 2 * (2G + 1 + 128) doubles, dots 2 * G * 128, xbuf 2 * m

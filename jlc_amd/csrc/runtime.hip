@@ -94,10 +94,14 @@ struct ThreadCtx {
     void* pin = nullptr;
     size_t pin_cap = 0;
     // The split-form, split-K and WY workspace (buf[3..9]) is shared by every call of this thread,
-    // whatever stream the call runs on: ws_ev is recorded after the last launch that used it,
-    // on ws_stream, and a call on another stream waits for it before touching the buffers.
+    // whatever stream the call runs on: ws_ev marks the end of the last use, on ws_stream, and a call
+    // on another stream waits for it before touching the buffers. A use on the thread's own stream is
+    // marked lazily (ws_pending), by the first later call on another stream: each event record costs a
+    // gap of several microseconds between the kernels around it, which a stream of calls on one stream
+    // never needs.
     hipEvent_t ws_ev = nullptr;
     hipStream_t ws_stream = nullptr;
+    bool ws_pending = false;
     // Blocked QR: pinned host copy of a CholeskyQR2 panel's flag and the event that marks its arrival.
     int* qr_flag = nullptr;
     hipEvent_t qr_ev = nullptr;
@@ -166,14 +170,27 @@ int ensure_init() {
 // workspace on another stream (a no-op when the last user ran on s: a stream is ordered).
 int ws_acquire(hipStream_t s) {
     ThreadCtx& c = t_ctx;
-    if (c.ws_stream && c.ws_stream != s) JLC_HIP_CHECK(hipStreamWaitEvent(s, c.ws_ev, 0));
+    if (c.ws_stream && c.ws_stream != s) {
+        if (c.ws_pending) {
+            // the last use ran on the thread's own stream (which lives as long as the thread): mark it now
+            JLC_HIP_CHECK(hipEventRecord(c.ws_ev, c.ws_stream));
+            c.ws_pending = false;
+        }
+        JLC_HIP_CHECK(hipStreamWaitEvent(s, c.ws_ev, 0));
+    }
     return JLC_OK;
 }
 
-// Marks the end of this call's workspace use on stream s.
+// Marks the end of this call's workspace use on stream s: at once on a caller's stream (it may be
+// destroyed before the next call), lazily on the thread's own stream.
 int ws_release(hipStream_t s) {
     ThreadCtx& c = t_ctx;
-    JLC_HIP_CHECK(hipEventRecord(c.ws_ev, s));
+    if (s == c.stream) {
+        c.ws_pending = true;
+    } else {
+        JLC_HIP_CHECK(hipEventRecord(c.ws_ev, s));
+        c.ws_pending = false;
+    }
     c.ws_stream = s;
     return JLC_OK;
 }
@@ -649,6 +666,24 @@ int run_wy_apply(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const 
     WorkspaceUse ws(s);
     int rc = ws.acquire();
     if (rc != JLC_OK) return rc;
+    const jlc::TuneTable tt = jlc::tune_snapshot();
+    const int wgs = side == JLC_LEFT ? tt.wy_left_wgs_per_cu : tt.wy_right_wgs_per_cu;
+    if ((mode & 0xff) == JLC_MODE_FP64 && kb <= 64 && wgs > 0 && ((m + 63) / 64) * ((n + 63) / 64) < INT32_MAX) {
+        // panels of at most 64 reflectors: the WY kernels (partial W, W2, update), C streamed twice
+        const jlc::WyPlan plan = jlc::wy_plan(side, m, n, kb, c, ldc, g_cus.load(), wgs);
+        if ((rc = ensure_buf(8, (size_t)plan.part_elems * sizeof(double))) != JLC_OK) return rc;
+        if ((rc = ensure_buf(9, (size_t)plan.w2_elems * sizeof(double))) != JLC_OK) return rc;
+        JLC_HIP_CHECK(jlc::launch_wy_update(side, trans_t, (int)m, (int)n, (int)kb, v, ldv, t, ldt, c, ldc, plan,
+                                            static_cast<double*>(t_ctx.buf[8]), static_cast<double*>(t_ctx.buf[9]),
+                                            s));
+        *kernel = JLC_KERNEL_WY;
+        info->tile_m = 64;
+        info->tile_n = 64;
+        info->tile_k = plan.kb_pad;
+        info->slices = plan.slices;
+        info->launches += 3;
+        return JLC_OK;
+    }
     const size_t wbytes = (size_t)(side == JLC_LEFT ? kb * n : m * kb) * sizeof(double);
     if ((rc = ensure_buf(8, wbytes)) != JLC_OK) return rc;
     if ((rc = ensure_buf(9, wbytes)) != JLC_OK) return rc;
@@ -797,6 +832,8 @@ bool tune_field(jlc::TuneTable& t, const char* key, double* get, const double* s
         {"qr_block_threads", nullptr, &jlc::TuneTable::qr_block_threads},
         {"qr_reg_max_m", nullptr, &jlc::TuneTable::qr_reg_max_m},
         {"qr_cqr_min_rows", nullptr, &jlc::TuneTable::qr_cqr_min_rows},
+        {"wy_left_wgs_per_cu", nullptr, &jlc::TuneTable::wy_left_wgs_per_cu},
+        {"wy_right_wgs_per_cu", nullptr, &jlc::TuneTable::wy_right_wgs_per_cu},
     };
     for (const F& f : fields) {
         if (std::strcmp(f.name, key) != 0) continue;
@@ -862,6 +899,8 @@ int jlc_tune_write(const char* path) {
     std::fprintf(f, "qr_block_threads %d\n", t.qr_block_threads);
     std::fprintf(f, "qr_reg_max_m %d\n", t.qr_reg_max_m);
     std::fprintf(f, "qr_cqr_min_rows %d\n", t.qr_cqr_min_rows);
+    std::fprintf(f, "wy_left_wgs_per_cu %d\n", t.wy_left_wgs_per_cu);
+    std::fprintf(f, "wy_right_wgs_per_cu %d\n", t.wy_right_wgs_per_cu);
     std::fclose(f);
     return JLC_OK;
 }

@@ -37,7 +37,7 @@ MODE_SPLIT_BF16 = 2
 MODE_SPLIT_CRT = 3
 
 KERNEL_NAMES = {0: "none", 1: "scale", 2: "mfma_f64", 3: "gemv", 4: "split_int8", 5: "split_bf16", 6: "grouped",
-                7: "qr", 8: "split_crt"}
+                7: "qr", 8: "split_crt", 9: "wy"}
 
 
 def mode_slices(mode: int, slices: int) -> int:

@@ -99,8 +99,101 @@ def _device_case(side, trans_t, n_full, r0, c0, kb, seed, mode=native.MODE_FP64)
 ])
 def test_device_wy_apply_n4096_against_restatement(side, trans_t, r0, c0, kb, what):
     expected, got, k, st = _device_case(side, trans_t, 4096, r0, c0, kb, seed=zlib.crc32(what.encode()))
-    assert st.kernel_name == "mfma_f64"
+    assert st.kernel_name == "wy"
     _parity(what, expected, got, k)
+
+
+def _poisoned_case(side, trans_t, m, n, kb, seed, mode=native.MODE_FP64, col_off=0, ldc=None):
+    """An m x n block at column col_off of a larger row-major array (ldc, default n + 3; one extra row)
+    whose surroundings, V's columns past kb (ldv = kb + 2) and T's past kb (ldt = kb + 1) hold Inf: none
+    of it is part of the operation, so the update must stay finite, leave the surroundings alone and
+    match the restatement."""
+    rng = np.random.default_rng(seed)
+    ldc = ldc or n + col_off + 3
+    ldv, ldt = kb + 2, kb + 1
+    full = np.full((m + 1, ldc), np.inf)
+    full[:m, col_off:col_off + n] = rng.uniform(-1.0, 1.0, (m, n))
+    rows = m if side == native.LEFT else n
+    V, T, _ = reflectors(rng, rows, kb) if rows >= kb else (0.3 * rng.standard_normal((rows, kb)),
+                                                          0.1 * np.triu(rng.standard_normal((kb, kb))), None)
+    Vp = np.full((rows, ldv), np.inf)
+    Vp[:, :kb] = V
+    Tp = np.full((kb, ldt), np.inf)
+    Tp[:, :kb] = T
+    Vf, Tf, Cf = Vp.reshape(-1).copy(), Tp.reshape(-1).copy(), full.reshape(-1).copy()
+    dV, dT, dC = (native.DeviceArray.from_numpy(x) for x in (Vf, Tf, Cf))
+    st = native.wy_apply_dev(side, trans_t, m, n, kb, dV.ptr, ldv, dT.ptr, ldt, dC.at(col_off), ldc, mode)
+    expected = Cf.copy()
+    orc.wy_apply(side, trans_t, m, n, kb, Vf, ldv, Tf, ldt, expected, ldc, c_off=col_off)
+    got = dC.to_numpy()
+    inside = np.zeros((m + 1, ldc), bool)
+    inside[:m, col_off:col_off + n] = True
+    inside = inside.reshape(-1)
+    assert np.array_equal(got[~inside], expected[~inside]), "entries outside the block changed"
+    assert np.all(np.isfinite(got[inside])), "a value outside the operation reached the block"
+    return expected[inside], got[inside], max(m, n, kb), st
+
+
+# Ragged widths (kb not a multiple of 16, one reflector, the 64 limit), blocks smaller than one tile,
+# one row or column, heights that leave a partial slice or a partial unrolled row group.
+WY_SHAPES = [(1, 1, 1), (5, 3, 2), (63, 65, 7), (65, 63, 16), (129, 300, 30), (300, 129, 33), (1000, 17, 48),
+             (17, 1000, 64), (777, 555, 64), (2049, 1031, 32), (1, 700, 5), (700, 1, 5)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("side", [native.LEFT, native.RIGHT])
+@pytest.mark.parametrize("trans_t", [0, 1])
+def test_wy_kernels_ragged_shapes_against_restatement(side, trans_t):
+    for m, n, kb in WY_SHAPES:
+        what = f"side {side} trans {trans_t} m {m} n {n} kb {kb}"
+        print(what, flush=True)
+        expected, got, k, st = _poisoned_case(side, trans_t, m, n, kb, seed=zlib.crc32(what.encode()))
+        assert st.kernel_name == "wy", what
+        _parity(what, expected, got, k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("side", [native.LEFT, native.RIGHT])
+def test_wy_kernels_line_shifted_tiles_against_restatement(side):
+    """Blocks whose first column is not on a 128-byte line of a row stride that is a multiple of 16
+    doubles (H[:, k+1:] of the Hessenberg driver): C's tiles then start before the block, and the columns
+    before it must neither be read into a sum nor written."""
+    for col_off in (1, 7, 15, 16, 33):
+        for m, n, kb in ((300, 200, 32), (129, 1000, 64), (1000, 70, 30), (64, 64, 16)):
+            ldc = ((n + col_off + 5 + 15) // 16) * 16
+            what = f"side {side} offset {col_off} m {m} n {n} kb {kb} ldc {ldc}"
+            print(what, flush=True)
+            expected, got, k, st = _poisoned_case(side, side, m, n, kb, seed=zlib.crc32(what.encode()),
+                                                  col_off=col_off, ldc=ldc)
+            assert st.kernel_name == "wy", what
+            _parity(what, expected, got, k)
+
+
+@pytest.mark.gpu
+def test_wy_kernels_slice_counts_and_gemm_sequence_agree():
+    """The partial-W slice count (wy_left/right_wgs_per_cu 1, 2, 8: one to many slots, a last slot shorter
+    than the rest) and the three-GEMM sequence (0) all match the restatement; kb = 65 takes the GEMM
+    sequence by itself."""
+    keys = ("wy_left_wgs_per_cu", "wy_right_wgs_per_cu")
+    saved = [native.tune_get(k) for k in keys]
+    try:
+        for wgs in (1, 2, 8, 0):
+            for k in keys:
+                native.tune_set(k, wgs)
+            for side, trans_t, m, n, kb in ((native.LEFT, 1, 3001, 2500, 32), (native.RIGHT, 0, 2500, 3001, 64)):
+                what = f"wgs {wgs} side {side} m {m} n {n} kb {kb}"
+                print(what, flush=True)
+                expected, got, k, st = _poisoned_case(side, trans_t, m, n, kb, seed=zlib.crc32(what.encode()))
+                assert st.kernel_name == ("wy" if wgs else "mfma_f64"), what
+                _parity(what, expected, got, k)
+        for k, v in zip(keys, saved):
+            native.tune_set(k, v)
+        expected, got, k, st = _poisoned_case(native.LEFT, 0, 300, 200, 65, seed=65)
+        assert st.kernel_name == "mfma_f64"
+        _parity("kb 65", expected, got, k)
+    finally:
+        for k, v in zip(keys, saved):
+            native.tune_set(k, v)
 
 
 @pytest.mark.gpu
