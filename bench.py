@@ -694,10 +694,11 @@ def bench_wy(args) -> int:
         # 78.6 / 8 = 9.8: that sweep is priced against HBM with C read and written once per update
         "roofline": {"bound": "hbm", "achieved": hs["algorithmic_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(hs["algorithmic_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "wy_apply_dev Hessenberg sweep (3 dgemm_f64 launches per update)",
+                     "kernel": "wy_apply_dev Hessenberg sweep (wy_partial, wy_w2, wy_update per update)",
                      "kernel_ms": hs["ms"],
-                     "note": "algorithmic bytes 16 * rows * cols per update (C read and written once); the three-"
-                             "product form reads C twice, so a sweep at this figure's peak is out of reach"},
+                     "note": "algorithmic bytes 16 * rows * cols per update (C read and written once); the "
+                             "partial-W kernel reads C once more, so 24 B per element reach HBM and 2/3 of the "
+                             "peak is this form's ceiling"},
         "cpu_baseline": {"value": round(cpu_gflops / 1e3, 4), "unit": "TFLOP/s", "cores": orc.threads(),
                          "kind": "port", "sample": "first 2 panels of each sweep (10 updates), oracle "
                                                    f"applyBlockLeft/Right restatement, {orc.threads()} threads"}}),

@@ -357,33 +357,47 @@ __global__ __launch_bounds__(256) void wy_w2_right(int m, int kb, int slices, co
 }
 
 // ---- 3: C[r][c] -= sum_{k < kb} A[r][k] B[c][k] (A m x kb, B n x kb, both k-contiguous).
-// Workgroup: one 64 x 64 tile, four wavefronts in 2 x 2, each 32 x 32 (2 x 2 fragments). The operand
-// rows (-A and B, 64 x KB each, k-contiguous, L2 hits) are loaded first and the tile's C entries into the
-// accumulators after them, so the LDS writes of the operands (stride KB + 4) wait for the operands alone
-// while C's HBM reads are in flight; KB / 4 MFMA steps; stores.
+// Workgroup: one 64 x 64 tile, four wavefronts in 2 x 2, each 32 x 32 (2 x 2 fragments). K runs in chunks
+// of KCH <= 32 (LDS 2 x 64 x (KCH + 4) doubles, four workgroups per CU even at KB = 64). The first chunk's
+// operand rows (-A and B, k-contiguous, L2 hits) are loaded before the tile's C entries go into the
+// accumulators, so their LDS writes wait for the operands alone while C's HBM reads are in flight; each
+// later chunk's operands are loaded during the previous chunk's MFMAs. Stores last.
+template <int KF>
+constexpr int wy_kch() { return KF <= 2 ? 16 * KF : (KF == 4 ? 32 : 16); }
+
 template <int KF>
 __global__ __launch_bounds__(256) void wy_update(int m, int n, int kb, const double* __restrict__ A, int64_t lda,
                                                  const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
                                                  int64_t ldc, int tiles_n, int cshift) {
-    constexpr int KB = 16 * KF, SK = KB + 4;
+    constexpr int KB = 16 * KF, KCH = wy_kch<KF>(), NCH = KB / KCH, SK = KCH + 4, PER = 64 * KCH / 256;
     __shared__ double as[64 * SK];
     __shared__ double bs[64 * SK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1, g = lane >> 4, c16 = lane & 15;
     // tile columns start cshift before column 0, so that tile rows begin on 128-byte lines
     const int r0 = (blockIdx.x / tiles_n) * 64, c0 = (blockIdx.x % tiles_n) * 64 - cshift;
     const bool interior = r0 + 64 <= m && c0 >= 0 && c0 + 64 <= n;
-    // operand loads first (L2 hits), then the C tile (HBM): the LDS writes then wait for the operands alone
-    constexpr int PER = 64 * KB / 256;
     double xa[PER], xb[PER];
+    auto load_ops = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < PER; u++) {
-        const int e = t + 256 * u;
-        const int rr = e / KB, k = e % KB;
-        const int64_t kc = min(k, kb - 1);
-        const int64_t ra = min(r0 + rr, m - 1), rb = max(0, min(c0 + rr, n - 1));
-        xa[u] = A[ra * lda + kc];
-        xb[u] = B[rb * ldb + kc];
-    }
+        for (int u = 0; u < PER; u++) {
+            const int e = t + 256 * u;
+            const int rr = e / KCH;
+            const int64_t kc = min(k0 + e % KCH, kb - 1);
+            const int64_t ra = min(r0 + rr, m - 1), rb = max(0, min(c0 + rr, n - 1));
+            xa[u] = A[ra * lda + kc];
+            xb[u] = B[rb * ldb + kc];
+        }
+    };
+    auto store_ops = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = t + 256 * u;
+            const int rr = e / KCH, k = e % KCH;
+            as[rr * SK + k] = k0 + k < kb ? -xa[u] : 0.0;
+            bs[rr * SK + k] = k0 + k < kb ? xb[u] : 0.0;
+        }
+    };
+    load_ops(0);
     d4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -397,25 +411,24 @@ __global__ __launch_bounds__(256) void wy_update(int m, int n, int kb, const dou
             }
         }
 #pragma unroll
-    for (int u = 0; u < PER; u++) {
-        const int e = t + 256 * u;
-        const int rr = e / KB, k = e % KB;
-        as[rr * SK + k] = k < kb ? -xa[u] : 0.0;
-        bs[rr * SK + k] = k < kb ? xb[u] : 0.0;
-    }
-    __syncthreads();
+    for (int ch = 0; ch < NCH; ch++) {
+        if (ch) __syncthreads();  // the previous chunk's fragment reads are done
+        store_ops(ch * KCH);
+        __syncthreads();
+        if (ch + 1 < NCH) load_ops((ch + 1) * KCH);
 #pragma unroll
-    for (int kk = 0; kk < KB; kk += 4) {
-        double a[2], b[2];
+        for (int kk = 0; kk < KCH; kk += 4) {
+            double a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
-            a[i] = as[(wm * 32 + 16 * i + c16) * SK + kk + g];
-            b[i] = bs[(wn * 32 + 16 * i + c16) * SK + kk + g];
+            for (int i = 0; i < 2; i++) {
+                a[i] = as[(wm * 32 + 16 * i + c16) * SK + kk + g];
+                b[i] = bs[(wn * 32 + 16 * i + c16) * SK + kk + g];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
         }
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
     }
 #pragma unroll
     for (int i = 0; i < 2; i++)
