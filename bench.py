@@ -706,6 +706,26 @@ def bench_wy(args) -> int:
     return 0
 
 
+def bench_wy_update(args) -> int:
+    """Single compact-WY updates of one 4096 x 4096 device block (row stride 4096): kb = 32 and 64, left then
+    right, three calls each (the counter runs of profiles/wy_update_4096_traffic.txt use this leg)."""
+    n = 4096
+    rng = np.random.default_rng(args.seed)
+    a = native.DeviceArray.from_numpy(rng.uniform(-1.0, 1.0, n * n))
+    out = {}
+    for kb in (32, 64):
+        v, t = _wy_panel(rng, n, kb)
+        dv, dt = native.DeviceArray.from_numpy(v), native.DeviceArray.from_numpy(t)
+        for side, name in ((native.LEFT, "left"), (native.RIGHT, "right")):
+            ms = [native.wy_apply_dev(side, 0, n, n, kb, dv.ptr, kb, dt.ptr, kb, a.ptr, n).kernel_ms
+                  for _ in range(3)]
+            out[f"{name}_kb{kb}"] = {"ms": [round(x, 4) for x in ms], "gbs_algorithmic": round(16.0 * n * n / (min(ms) * 1e-3) / 1e9, 1)}
+            log(f"wy update {name} kb {kb}: {out[f'{name}_kb{kb}']}")
+    print(json.dumps({"metric": "compact-WY update of a 4096 x 4096 block, ms per call (three calls per case)",
+                      "unit": "ms", "dtype": "f64", "data": "synthetic", "cases": out}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -724,7 +744,7 @@ def main() -> int:
     ap.add_argument("--forms", default="", help="split forms to time, comma list of crt,int8,bf16 (default all)")
     ap.add_argument("--host-path", action="store_true",
                     help="also time the host-pointer entry point (PCIe staging included) on the value form")
-    ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched", "wy"], default="dgemm",
+    ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched", "wy", "wy_update"], default="dgemm",
                     help="dgemm: the 16384^3 metric workload; batched: the grouped-GEMM configuration")
     ap.add_argument("--count", type=int, default=100000, help="problems in the batched workload")
     ap.add_argument("--no-per-call", action="store_true",
@@ -761,6 +781,8 @@ def main() -> int:
         return bench_qr_batched(args)
     if args.workload == "wy":
         return bench_wy(args)
+    if args.workload == "wy_update":
+        return bench_wy_update(args)
 
     m = n = k = args.size
     da, db, dc = native.DeviceArray(m * k), native.DeviceArray(k * n), native.DeviceArray(m * n)

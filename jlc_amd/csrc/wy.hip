@@ -269,6 +269,20 @@ __global__ __launch_bounds__(256) void wy_partial_right(int m, int n, int kb, co
 // op(T) staged in LDS (row stride KB + 1: threads of one wavefront read distinct banks).
 constexpr int WY_W2 = 16;
 
+// p[0] + p[stride] + ... over `slices` slots, added in slot order; the loads go out eight at a time (slots
+// past the last read the last one and add zero), so the sum waits for one load latency per eight slots.
+__device__ __forceinline__ double wy_slot_sum(const double* __restrict__ p, int64_t stride, int slices) {
+    double w = 0.0;
+    for (int s0 = 0; s0 < slices; s0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = p[(int64_t)min(s0 + j, slices - 1) * stride];
+#pragma unroll
+        for (int j = 0; j < 8; j++) w += s0 + j < slices ? x[j] : 0.0;
+    }
+    return w;
+}
+
 // 2L: W2T[col][r] = sum_i opT(r, i) W[i][col], W[i][col] = sum_s P[s][i][col];
 // opT(r, i) = T[r][i] (trans 0) or T[i][r] (trans 1), zero past kb.
 template <int KF>
@@ -288,20 +302,22 @@ __global__ __launch_bounds__(256) void wy_w2_left(int n, int kb, int slices, con
         const int e = min(t + 256 * u, KB * KB - 1), r = min(e / KB, kb - 1), i = min(e % KB, kb - 1);
         tv[u] = trans ? T[(int64_t)i * ldt + r] : T[(int64_t)r * ldt + i];
     }
+    // the slot sums next (their loads overlap op(T)'s), then both go to LDS
+    constexpr int WPER = KB * WY_W2 / 256;
+    double wsum[WPER];
+#pragma unroll
+    for (int u = 0; u < WPER; u++) {
+        const int e = t + 256 * u;
+        const int i = e / WY_W2, col = min(c0 + e % WY_W2, n - 1);
+        wsum[u] = wy_slot_sum(P + (int64_t)i * n + col, (int64_t)KB * n, slices);
+    }
 #pragma unroll
     for (int u = 0; u < TPER; u++) {
         const int e = t + 256 * u, r = e / KB, i = e % KB;
         if (e < KB * KB) ts[r * ST + i] = (r < kb && i < kb) ? tv[u] : 0.0;
     }
 #pragma unroll
-    for (int e = t; e < KB * WY_W2; e += 256) {
-        const int i = e / WY_W2, col = min(c0 + e % WY_W2, n - 1);
-        const double* p = P + (int64_t)i * n + col;
-        double w = p[0];
-#pragma unroll 8
-        for (int sl = 1; sl < slices; sl++) w += p[(int64_t)sl * KB * n];
-        ws[e] = w;
-    }
+    for (int u = 0; u < WPER; u++) ws[t + 256 * u] = wsum[u];
     __syncthreads();
     for (int e = t; e < KB * WY_W2; e += 256) {
         const int cl = e / KB, r = e % KB;
@@ -331,20 +347,21 @@ __global__ __launch_bounds__(256) void wy_w2_right(int m, int kb, int slices, co
         const int e = min(t + 256 * u, KB * KB - 1), c = min(e / KB, kb - 1), i = min(e % KB, kb - 1);
         tv[u] = trans ? T[(int64_t)c * ldt + i] : T[(int64_t)i * ldt + c];
     }
+    constexpr int WPER = KB * WY_W2 / 256;
+    double wsum[WPER];
+#pragma unroll
+    for (int u = 0; u < WPER; u++) {
+        const int e = t + 256 * u;
+        const int rl = e / KB, i = e % KB;
+        wsum[u] = wy_slot_sum(P + (int64_t)min(r0 + rl, m - 1) * KB + i, (int64_t)m * KB, slices);
+    }
 #pragma unroll
     for (int u = 0; u < TPER; u++) {
         const int e = t + 256 * u, c = e / KB, i = e % KB;
         if (e < KB * KB) ts[c * ST + i] = (c < kb && i < kb) ? tv[u] : 0.0;
     }
 #pragma unroll
-    for (int e = t; e < KB * WY_W2; e += 256) {
-        const int rl = e / KB, i = e % KB;
-        const double* p = P + (int64_t)min(r0 + rl, m - 1) * KB + i;
-        double w = p[0];
-#pragma unroll 8
-        for (int sl = 1; sl < slices; sl++) w += p[(int64_t)sl * m * KB];
-        ws[e] = w;
-    }
+    for (int u = 0; u < WPER; u++) ws[t + 256 * u] = wsum[u];
     __syncthreads();
     for (int e = t; e < KB * WY_W2; e += 256) {
         const int rl = e / KB, c = e % KB;
