@@ -694,6 +694,8 @@ def bench_wy(args) -> int:
         # 78.6 / 8 = 9.8: that sweep is priced against HBM with C read and written once per update
         "roofline": {"bound": "hbm", "achieved": hs["algorithmic_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(hs["algorithmic_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     # HBM bytes of one kb = 32 update of the whole 4096 block (its three launches), counted
+                     "traffic_one_update_4096": read_traffic("wy_update_4096"),
                      "kernel": "wy_apply_dev Hessenberg sweep (wy_partial, wy_w2, wy_update per update)",
                      "kernel_ms": hs["ms"],
                      "note": "algorithmic bytes 16 * rows * cols per update (C read and written once); the "
