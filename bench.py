@@ -728,6 +728,45 @@ def bench_wy_update(args) -> int:
     return 0
 
 
+def bench_multiply(args) -> int:
+    """Matrix.multiply on heap matrices (the reference's facade, Matrix.java:1120-1122, routed by
+    BlockedMultiply.multiply to the HIP plug): a new row-major C = A B, A and B staged over PCIe and C copied
+    back inside every call. n = 4096 and 8192, U[-1, 1) entries; median wall clock of the timed calls, and
+    the device kernel time of one call through jlc_dgemm's stats beside it."""
+    from jlc_amd.compute import DispatchPolicy, Matrix
+    rng = np.random.default_rng(args.seed)
+    out = {}
+    for n in (4096, 8192):
+        a = Matrix(n, n, rng.uniform(-1.0, 1.0, n * n))
+        b = Matrix(n, n, rng.uniform(-1.0, 1.0, n * n))
+        assert DispatchPolicy.defaultPolicy().selectForMultiply(n, n, n).value == "HIP"
+        times = []
+        for it in range(args.warmup + max(1, args.steps)):
+            t1 = time.perf_counter()
+            c = a.multiply(b)
+            if it >= args.warmup:
+                times.append((time.perf_counter() - t1) * 1e3)
+        ms = float(np.median(times))
+        st = native.dgemm(native.ROW, native.ROW, n, n, n, 1.0, a.getRawData(), n, b.getRawData(), n, 0.0,
+                          np.empty(n * n), n)
+        # spot check: 64 entries of C against numpy's product of those rows and columns
+        idx = rng.integers(0, n, (64, 2))
+        ref = np.einsum("ij,ji->i", a.to_numpy()[idx[:, 0]], b.to_numpy()[:, idx[:, 1]])
+        got = c.to_numpy()[idx[:, 0], idx[:, 1]]
+        out[str(n)] = {"ms": round(ms, 3), "tflops_incl_pcie": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 3),
+                       "kernel_ms": round(st.kernel_ms, 3), "call_ms": round(st.total_ms, 3),
+                       "max_rel_err_64_entries": float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)))}
+        log(f"Matrix.multiply {n}: {out[str(n)]}")
+    big = out["8192"]
+    print(json.dumps({
+        "metric": "Matrix.multiply 8192x8192 (heap matrices, PCIe staging inside the call)",
+        "value": big["tflops_incl_pcie"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": big["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic", "config": {"workload": "Matrix.multiply n = 4096 and 8192 through BlockedMultiply"},
+        "sizes": out}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -746,7 +785,7 @@ def main() -> int:
     ap.add_argument("--forms", default="", help="split forms to time, comma list of crt,int8,bf16 (default all)")
     ap.add_argument("--host-path", action="store_true",
                     help="also time the host-pointer entry point (PCIe staging included) on the value form")
-    ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched", "wy", "wy_update"], default="dgemm",
+    ap.add_argument("--workload", choices=["dgemm", "layouts", "batched", "qr", "qr_batched", "wy", "wy_update", "multiply"], default="dgemm",
                     help="dgemm: the 16384^3 metric workload; batched: the grouped-GEMM configuration")
     ap.add_argument("--count", type=int, default=100000, help="problems in the batched workload")
     ap.add_argument("--no-per-call", action="store_true",
@@ -785,6 +824,8 @@ def main() -> int:
         return bench_wy(args)
     if args.workload == "wy_update":
         return bench_wy_update(args)
+    if args.workload == "multiply":
+        return bench_multiply(args)
 
     m = n = k = args.size
     da, db, dc = native.DeviceArray(m * k), native.DeviceArray(k * n), native.DeviceArray(m * n)
