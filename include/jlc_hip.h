@@ -161,7 +161,9 @@ int jlc_dgemm_dev_noalloc(int layout_a, int layout_b, int64_t m, int64_t n, int6
  *   side == JLC_RIGHT (V row-major n x kb, ldv):  C <- C - ((C V) op(T)) V^T
  *     replaces applyBlockRight (BlockedHessenbergQR.java:308-345, the colStart = k + 1 form its
  *     callers use, trans_t = 0) and the right block update of BlockedBidiagonalization (:581-603).
- * The two large products run in `mode` (as jlc_dgemm_dev), the kb x kb product in FP64; the W and
+ * In JLC_MODE_FP64 with kb <= 64 the update runs three kernels (partial sums of V^T C or C V over
+ * slices of C, W2 = op(T) W, then C -= A B^T tile by tile; stats->kernel = JLC_KERNEL_WY); otherwise
+ * the two large products run in `mode` (as jlc_dgemm_dev) and the kb x kb product in FP64. The W and
  * W2 intermediates live in the calling thread's workspace. kb == 0 leaves C unchanged. The device
  * form is asynchronous unless stats is non-NULL; the host form stages only V, T and C's block. */
 #define JLC_LEFT 0
