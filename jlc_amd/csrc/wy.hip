@@ -117,24 +117,29 @@ __global__ __launch_bounds__(256) void wy_partial_left(int m, int n, int kb, con
         }
         wy_left_mfma<KF, WY_U>(a0, b0, vok, acc);
     }
-    // tail: fewer than 4 * WY_U rows, those past r1 selected to zero (C's row r1 - 1 is loaded instead)
-    for (int r = r0 + groups * 4 * WY_U; r < r1; r += 4) {
-        const int row = r + g;
-        const bool ok = row < r1;
-        const int64_t rr = min(row, r1 - 1);
-        double a[KF], b[2];
+    // tail: fewer than 4 * WY_U rows as one group whose loads all go out together (row r1 - 1 is loaded
+    // for the rows past r1, which are selected to zero)
+    const int rt = r0 + groups * 4 * WY_U;
+    if (rt < r1) {
+        double a[WY_U][KF], b[WY_U][2];
 #pragma unroll
-        for (int i = 0; i < KF; i++) {
-            const double x = V[rr * ldv + vc[i]];
-            a[i] = ok && vok[i] ? x : 0.0;
+        for (int u = 0; u < WY_U; u++) {
+            const int64_t rr = min(rt + 4 * u + g, r1 - 1);
+#pragma unroll
+            for (int i = 0; i < KF; i++) a[u][i] = V[rr * ldv + vc[i]];
+            b[u][0] = C[rr * ldc + ca];
+            b[u][1] = C[rr * ldc + cb];
         }
-        const double xa = C[rr * ldc + ca], xb = C[rr * ldc + cb];
-        b[0] = ok ? xa : 0.0;
-        b[1] = ok ? xb : 0.0;
 #pragma unroll
-        for (int i = 0; i < KF; i++)
+        for (int u = 0; u < WY_U; u++) {
+            const bool ok = rt + 4 * u + g < r1;
 #pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+            for (int i = 0; i < KF; i++) {
+                const double av = ok && vok[i] ? a[u][i] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = mfma(av, ok ? b[u][j] : 0.0, acc[i][j]);
+            }
+        }
     }
     double* ps = P + (int64_t)blockIdx.y * KB * n;
 #pragma unroll
@@ -497,10 +502,11 @@ WyPlan wy_plan(int side, int64_t m, int64_t n, int64_t kb, const double* c, int6
     p.cshift = (ldc % 16 == 0 && addr % 8 == 0) ? (int)((addr / 8) % 16) : 0;
     const int64_t target = (int64_t)std::max(1, device_cus) * std::max(1, wgs_per_cu);
     if (side == JLC_LEFT) {
-        // slices of rows, a multiple of WY_ROWS each
+        // slices of rows, a multiple of WY_ROWS each and at least 4 * WY_ROWS (a short slice costs the
+        // W2 kernel one more slot to sum and hides no latency of its own)
         const int64_t blocks = (n + p.cshift + 127) / 128;
         int64_t s = std::max<int64_t>(1, (target + blocks - 1) / blocks);
-        s = std::min<int64_t>(s, (m + WY_ROWS - 1) / WY_ROWS);
+        s = std::max<int64_t>(1, std::min<int64_t>(s, m / (4 * WY_ROWS)));
         int64_t span = (m + s - 1) / s;
         span = (span + WY_ROWS - 1) / WY_ROWS * WY_ROWS;
         p.span = (int)span;
