@@ -162,7 +162,7 @@ constexpr int WY_CH = 64;          // columns per staged chunk
 constexpr int WY_SR = WY_CH + 4;   // LDS row stride (doubles): fragment reads of 16 rows hit distinct banks
 
 template <int KF>
-__global__ __launch_bounds__(256) void wy_partial_right(int m, int n, int kb, const double* __restrict__ C,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void wy_partial_right(int m, int n, int kb, const double* __restrict__ C,
                                                         int64_t ldc, const double* __restrict__ V, int64_t ldv,
                                                         double* __restrict__ P, int cols_per_slice, int cshift) {
     constexpr int KB = 16 * KF;
@@ -209,18 +209,14 @@ __global__ __launch_bounds__(256) void wy_partial_right(int m, int n, int kb, co
             for (int i = 0; i < KF; i++) b[s][i] = V[vrr * ldv + vc[i]];
         }
     };
-    // Loads complete in issue order, so each chunk's V rows are issued before its C prefetch: the MFMAs
-    // of a chunk then never wait on the next chunk's C. Two V buffers, used alternately by name.
-    auto run_chunk = [&](int kc, const double (&bc)[4][KF], double (&bn)[4][KF]) {
+    auto stage_chunk = [&](int kc) {
         __syncthreads();  // the previous chunk's fragment reads are done
         const bool cok = kc + lane >= k0 && kc + lane < k1;
 #pragma unroll
         for (int e = 0; e < 16; e++) cs[(w + 4 * e) * WY_SR + lane] = cok ? x[e] : 0.0;
         __syncthreads();
-        if (kc + WY_CH < k1) {
-            load_v(kc + WY_CH, bn);
-            load_chunk(kc + WY_CH);
-        }
+    };
+    auto mfma_chunk = [&](int kc, const double (&bc)[4][KF]) {
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             const int kk = 16 * w + 4 * s;  // this wavefront's k step within the chunk
@@ -236,14 +232,42 @@ __global__ __launch_bounds__(256) void wy_partial_right(int m, int n, int kb, co
             }
         }
     };
-    double vb0[4][KF], vb1[4][KF];
-    if (k0 < k1) {
-        load_v(kbase, vb0);
-        load_chunk(kbase);
-    }
-    for (int kc = kbase; kc < k1; kc += 2 * WY_CH) {
-        run_chunk(kc, vb0, vb1);
-        if (kc + WY_CH < k1) run_chunk(kc + WY_CH, vb1, vb0);
+    // Loads complete in issue order, so each chunk's V rows are issued before the next chunk's C
+    // prefetch: the MFMAs of a chunk then never wait on that prefetch.
+    if (KF <= 2) {
+        // V rows one chunk ahead as well, in two buffers used alternately by name
+        double vb0[4][KF], vb1[4][KF];
+        if (k0 < k1) {
+            load_v(kbase, vb0);
+            load_chunk(kbase);
+        }
+        for (int kc = kbase; kc < k1; kc += 2 * WY_CH) {
+            stage_chunk(kc);
+            if (kc + WY_CH < k1) {
+                load_v(kc + WY_CH, vb1);
+                load_chunk(kc + WY_CH);
+            }
+            mfma_chunk(kc, vb0);
+            if (kc + WY_CH < k1) {
+                stage_chunk(kc + WY_CH);
+                if (kc + 2 * WY_CH < k1) {
+                    load_v(kc + 2 * WY_CH, vb0);
+                    load_chunk(kc + 2 * WY_CH);
+                }
+                mfma_chunk(kc + WY_CH, vb1);
+            }
+        }
+    } else {
+        // wider panels: the accumulators take 4 * KF registers of every lane, so V rows are loaded for the
+        // current chunk only (after its staging, before the next chunk's C) and two workgroups fit a CU
+        if (k0 < k1) load_chunk(kbase);
+        for (int kc = kbase; kc < k1; kc += WY_CH) {
+            stage_chunk(kc);
+            double vb[4][KF];
+            load_v(kc, vb);
+            if (kc + WY_CH < k1) load_chunk(kc + WY_CH);
+            mfma_chunk(kc, vb);
+        }
     }
     // wavefront-ordered sum of the four partial 64 x KB blocks through LDS (64 * KB <= 64 * WY_SR doubles
     // for KB <= 64), then the slice's block to P
@@ -502,11 +526,11 @@ WyPlan wy_plan(int side, int64_t m, int64_t n, int64_t kb, const double* c, int6
     p.cshift = (ldc % 16 == 0 && addr % 8 == 0) ? (int)((addr / 8) % 16) : 0;
     const int64_t target = (int64_t)std::max(1, device_cus) * std::max(1, wgs_per_cu);
     if (side == JLC_LEFT) {
-        // slices of rows, a multiple of WY_ROWS each and at least 4 * WY_ROWS (a short slice costs the
+        // slices of rows, a multiple of WY_ROWS each and at least 2 * WY_ROWS (a short slice costs the
         // W2 kernel one more slot to sum and hides no latency of its own)
         const int64_t blocks = (n + p.cshift + 127) / 128;
         int64_t s = std::max<int64_t>(1, (target + blocks - 1) / blocks);
-        s = std::max<int64_t>(1, std::min<int64_t>(s, m / (4 * WY_ROWS)));
+        s = std::max<int64_t>(1, std::min<int64_t>(s, m / (2 * WY_ROWS)));
         int64_t span = (m + s - 1) / s;
         span = (span + WY_ROWS - 1) / WY_ROWS * WY_ROWS;
         p.span = (int)span;
