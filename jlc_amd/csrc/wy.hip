@@ -298,18 +298,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void w
 // op(T) staged in LDS (row stride KB + 1: threads of one wavefront read distinct banks).
 constexpr int WY_W2 = 16;
 
-// p[0] + p[stride] + ... over `slices` slots, added in slot order; the loads go out eight at a time (slots
-// past the last read the last one and add zero), so the sum waits for one load latency per eight slots.
-__device__ __forceinline__ double wy_slot_sum(const double* __restrict__ p, int64_t stride, int slices) {
-    double w = 0.0;
+// w[u] = p[u][0] + p[u][stride] + ... over `slices` slots, added in slot order, for NE entries at once:
+// the loads of all entries go out eight slots at a time (slots past the last read the last one and add
+// zero), so the sums wait for one load latency per eight slots.
+template <int NE>
+__device__ __forceinline__ void wy_slot_sums(const double* const (&p)[NE], int64_t stride, int slices,
+                                             double (&w)[NE]) {
+#pragma unroll
+    for (int u = 0; u < NE; u++) w[u] = 0.0;
     for (int s0 = 0; s0 < slices; s0 += 8) {
-        double x[8];
+        double x[NE][8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) x[j] = p[(int64_t)min(s0 + j, slices - 1) * stride];
+        for (int u = 0; u < NE; u++)
 #pragma unroll
-        for (int j = 0; j < 8; j++) w += s0 + j < slices ? x[j] : 0.0;
+            for (int j = 0; j < 8; j++) x[u][j] = p[u][(int64_t)min(s0 + j, slices - 1) * stride];
+#pragma unroll
+        for (int u = 0; u < NE; u++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) w[u] += s0 + j < slices ? x[u][j] : 0.0;
     }
-    return w;
 }
 
 // 2L: W2T[col][r] = sum_i opT(r, i) W[i][col], W[i][col] = sum_s P[s][i][col];
@@ -334,12 +341,13 @@ __global__ __launch_bounds__(256) void wy_w2_left(int n, int kb, int slices, con
     // the slot sums next (their loads overlap op(T)'s), then both go to LDS
     constexpr int WPER = KB * WY_W2 / 256;
     double wsum[WPER];
+    const double* pe[WPER];
 #pragma unroll
     for (int u = 0; u < WPER; u++) {
         const int e = t + 256 * u;
-        const int i = e / WY_W2, col = min(c0 + e % WY_W2, n - 1);
-        wsum[u] = wy_slot_sum(P + (int64_t)i * n + col, (int64_t)KB * n, slices);
+        pe[u] = P + (int64_t)(e / WY_W2) * n + min(c0 + e % WY_W2, n - 1);
     }
+    wy_slot_sums<WPER>(pe, (int64_t)KB * n, slices, wsum);
 #pragma unroll
     for (int u = 0; u < TPER; u++) {
         const int e = t + 256 * u, r = e / KB, i = e % KB;
@@ -378,12 +386,13 @@ __global__ __launch_bounds__(256) void wy_w2_right(int m, int kb, int slices, co
     }
     constexpr int WPER = KB * WY_W2 / 256;
     double wsum[WPER];
+    const double* pe[WPER];
 #pragma unroll
     for (int u = 0; u < WPER; u++) {
         const int e = t + 256 * u;
-        const int rl = e / KB, i = e % KB;
-        wsum[u] = wy_slot_sum(P + (int64_t)min(r0 + rl, m - 1) * KB + i, (int64_t)m * KB, slices);
+        pe[u] = P + (int64_t)min(r0 + e / KB, m - 1) * KB + e % KB;
     }
+    wy_slot_sums<WPER>(pe, (int64_t)m * KB, slices, wsum);
 #pragma unroll
     for (int u = 0; u < TPER; u++) {
         const int e = t + 256 * u, c = e / KB, i = e % KB;
