@@ -8,9 +8,9 @@
  * All entry points are thread-safe: each host thread gets its own HIP stream
  * and its own staging buffers, so concurrent callers (the reference calls
  * its GEMM from many ForkJoin threads) never share mutable state. One thread's
- * device-pointer calls may run on different streams: the split-form and
- * split-K workspace the thread's calls share is ordered across streams (a
- * call on another stream than the workspace's last user waits for that
+ * device-pointer calls may run on different streams: the split-form, split-K
+ * and compact-WY workspace the thread's calls share is ordered across streams
+ * (a call on another stream than the workspace's last user waits for that
  * user's launches on the device). A stream passed to the library must stay
  * alive until the work queued on it has finished.
  *
