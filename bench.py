@@ -13,6 +13,11 @@ once, and are resident when the timed region starts. A step is one full GEMM.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--phi P] [--size N]
 Prints one JSON line.
+
+Other legs (--workload): layouts (8192^3, transposed and column-major operands, matvec), batched (100,000
+uneven GEMMs in one grouped launch), qr (blocked Householder QR 32768 x 4096), qr_batched (20,000 small
+QRs), wy (the compact-WY update sweeps of n = 4096 Hessenberg and bidiagonal reductions), wy_update
+(single 4096 x 4096 WY updates, for counter runs) and multiply (Matrix.multiply on heap matrices).
 """
 from __future__ import annotations
 
