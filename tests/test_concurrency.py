@@ -94,14 +94,16 @@ def test_one_thread_two_streams_match_sequential_results():
                                         "mfma_f64"]
     s1, s2 = native.stream_create(), native.stream_create()
     try:
-        for rep in range(2):
+        # rep 0, 1: two caller streams (the workspace's end is marked on the stream at each call's end);
+        # rep 2: the thread's own stream (NULL), whose last use is marked only when the call on stream 2
+        # arrives; rep 3: stream 2 first, then the thread's own stream
+        for rep, (first, second) in enumerate(((s1, s2), (s1, s2), (None, s2), (s2, None))):
             for case in cases:  # clear C so stale results cannot pass
                 native.DeviceArray.copy_from(case[6], np.full(case[0] * case[1], np.nan))
             for i in range(0, len(cases), 2):
-                call(cases[i], s1, False)       # asynchronous on stream 1
-                call(cases[i + 1], s2, False)   # same thread, same workspace, stream 2
-            native.stream_synchronize(s1)
-            native.stream_synchronize(s2)
+                call(cases[i], first, False)        # asynchronous
+                call(cases[i + 1], second, False)   # same thread, same workspace, the other stream
+            native.synchronize()
             for i, case in enumerate(cases):
                 assert np.array_equal(case[6].to_numpy(), expected[i][0]), (rep, i, case[:4])
     finally:

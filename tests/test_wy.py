@@ -250,3 +250,45 @@ def test_wy_apply_arguments():
     c = np.arange(16.0)
     native.wy_apply(native.LEFT, 0, 4, 4, 0, None, 1, None, 1, c, 4)  # kb == 0: unchanged
     assert np.array_equal(c, np.arange(16.0))
+
+
+@pytest.mark.gpu
+def test_wy_kernels_on_two_streams_match_sequential_results():
+    """Two asynchronous WY updates from one thread share the thread's WY workspace (partial slots and W2):
+    the first on the thread's own stream, whose end is marked only when the second call arrives on a
+    caller's stream, then the reverse. Results equal the same calls made one at a time, bit for bit."""
+    rng = np.random.default_rng(4242)
+    cases = []
+    for side, m, n, kb in ((native.LEFT, 3000, 2000, 32), (native.RIGHT, 1500, 2500, 64),
+                           (native.RIGHT, 2200, 1800, 48), (native.LEFT, 900, 1300, 16)):
+        rows = m if side == native.LEFT else n
+        V, T, _ = reflectors(rng, rows, kb)
+        C0 = rng.uniform(-1.0, 1.0, m * n)
+        cases.append((side, m, n, kb, native.DeviceArray.from_numpy(np.ascontiguousarray(V).reshape(-1)),
+                      native.DeviceArray.from_numpy(T.reshape(-1).copy()), C0, native.DeviceArray(m * n)))
+
+    def call(case, stream):
+        side, m, n, kb, dv, dt, _, dc = case
+        native.wy_apply_dev(side, 0, m, n, kb, dv.ptr, kb, dt.ptr, kb, dc.ptr, n, native.MODE_FP64, stream,
+                            stats=False)
+
+    expected = []
+    for case in cases:
+        native.DeviceArray.copy_from(case[7], case[6])
+        call(case, None)
+        native.synchronize()
+        expected.append(case[7].to_numpy())
+    s2 = native.stream_create()
+    try:
+        for first, second in ((None, s2), (s2, None)):
+            for case in cases:
+                native.DeviceArray.copy_from(case[7], case[6])
+            native.synchronize()
+            for i in range(0, len(cases), 2):
+                call(cases[i], first)
+                call(cases[i + 1], second)
+            native.synchronize()
+            for i, case in enumerate(cases):
+                assert np.array_equal(case[7].to_numpy(), expected[i]), (first, i, case[:4])
+    finally:
+        native.stream_destroy(s2)
