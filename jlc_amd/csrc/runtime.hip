@@ -668,7 +668,7 @@ int run_wy_apply(int side, int trans_t, int64_t m, int64_t n, int64_t kb, const 
     if (rc != JLC_OK) return rc;
     const jlc::TuneTable tt = jlc::tune_snapshot();
     const int wgs = side == JLC_LEFT ? tt.wy_left_wgs_per_cu : tt.wy_right_wgs_per_cu;
-    if ((mode & 0xff) == JLC_MODE_FP64 && kb <= 64 && wgs > 0 && ((m + 63) / 64) * ((n + 63) / 64) < INT32_MAX) {
+    if ((mode & 0xff) == JLC_MODE_FP64 && kb <= 64 && wgs > 0 && ((m + 63) / 64) * ((n + 47) / 32) < INT32_MAX) {
         // panels of at most 64 reflectors: the WY kernels (partial W, W2, update), C streamed twice
         const jlc::WyPlan plan = jlc::wy_plan(side, m, n, kb, c, ldc, g_cus.load(), wgs);
         if ((rc = ensure_buf(8, (size_t)plan.part_elems * sizeof(double))) != JLC_OK) return rc;

@@ -12,7 +12,7 @@
 //      in slot order and applies op(T) from LDS, writing W2 with k contiguous (left: W2^T, n x KB;
 //      right: W2, m x KB);
 //   3. update: C -= A B^T with A and B k-contiguous (left: A = V, B = W2^T; right: A = W2, B = V).
-//      Each 64 x 64 tile of C is loaded into the MFMA accumulators, updated over K = KB and stored:
+//      Each 64 x 32 tile of C is loaded into the MFMA accumulators, updated over K = KB and stored:
 //      C is read and written once.
 // KB = 16 * ceil(kb / 16). Every product runs on v_mfma_f64_16x16x4. Reflector columns past kb and
 // rows / columns past the block are selected to zero after a clamped (in-range) load, so any m, n >= 1
@@ -412,87 +412,99 @@ __global__ __launch_bounds__(256) void wy_w2_right(int m, int kb, int slices, co
 }
 
 // ---- 3: C[r][c] -= sum_{k < kb} A[r][k] B[c][k] (A m x kb, B n x kb, both k-contiguous).
-// Workgroup: one 64 x 64 tile, four wavefronts in 2 x 2, each 32 x 32 (2 x 2 fragments). K runs in chunks
-// of KCH <= 32 (LDS 2 x 64 x (KCH + 4) doubles, four workgroups per CU even at KB = 64). The first chunk's
+// Workgroup: one WY_TM x WY_TN tile, four wavefronts in 2 x 2 (each two 16 x 16 fragments at 64 x 32). K
+// runs in chunks of KCH <= 32 staged in LDS ((WY_TM + WY_TN) x (KCH + 4) doubles). The first chunk's
 // operand rows (-A and B, k-contiguous, L2 hits) are loaded before the tile's C entries go into the
 // accumulators, so their LDS writes wait for the operands alone while C's HBM reads are in flight; each
 // later chunk's operands are loaded during the previous chunk's MFMAs. Stores last.
 template <int KF>
 constexpr int wy_kch() { return KF <= 2 ? 16 * KF : (KF == 4 ? 32 : 16); }
 
+// Rows and columns of C per update workgroup. Measured per kb = 32 / 64 update (average over the n = 4096
+// sweeps): 64 x 128 28.2 / 33.2 us, 64 x 64 24.5 / 31.1, 64 x 32 24.1 / 29.8, 32 x 32 24.1 / 32.3.
+constexpr int WY_TM = 64, WY_TN = 32;
+
 template <int KF>
 __global__ __launch_bounds__(256) void wy_update(int m, int n, int kb, const double* __restrict__ A, int64_t lda,
                                                  const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
                                                  int64_t ldc, int tiles_n, int cshift) {
-    constexpr int KB = 16 * KF, KCH = wy_kch<KF>(), NCH = KB / KCH, SK = KCH + 4, PER = 64 * KCH / 256;
-    __shared__ double as[64 * SK];
-    __shared__ double bs[64 * SK];
+    constexpr int KB = 16 * KF, KCH = wy_kch<KF>(), NCH = KB / KCH, SK = KCH + 4;
+    constexpr int PA = WY_TM * KCH / 256, PB = WY_TN * KCH / 256, WM = WY_TM / 2, WN = WY_TN / 2;
+    constexpr int NI = WM / 16, NJ = WN / 16;
+    __shared__ double as[WY_TM * SK];
+    __shared__ double bs[WY_TN * SK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1, g = lane >> 4, c16 = lane & 15;
-    // tile columns start cshift before column 0, so that tile rows begin on 128-byte lines
-    const int r0 = (blockIdx.x / tiles_n) * 64, c0 = (blockIdx.x % tiles_n) * 64 - cshift;
-    const bool interior = r0 + 64 <= m && c0 >= 0 && c0 + 64 <= n;
-    double xa[PER], xb[PER];
+    const int r0 = (blockIdx.x / tiles_n) * WY_TM, c0 = (blockIdx.x % tiles_n) * WY_TN - cshift;
+    const bool interior = r0 + WY_TM <= m && c0 >= 0 && c0 + WY_TN <= n;
+    double xa[PA], xb[PB];
     auto load_ops = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < PER; u++) {
+        for (int u = 0; u < PA; u++) {
             const int e = t + 256 * u;
-            const int rr = e / KCH;
             const int64_t kc = min(k0 + e % KCH, kb - 1);
-            const int64_t ra = min(r0 + rr, m - 1), rb = max(0, min(c0 + rr, n - 1));
-            xa[u] = A[ra * lda + kc];
-            xb[u] = B[rb * ldb + kc];
+            xa[u] = A[(int64_t)min(r0 + e / KCH, m - 1) * lda + kc];
+        }
+#pragma unroll
+        for (int u = 0; u < PB; u++) {
+            const int e = t + 256 * u;
+            const int64_t kc = min(k0 + e % KCH, kb - 1);
+            xb[u] = B[(int64_t)max(0, min(c0 + e / KCH, n - 1)) * ldb + kc];
         }
     };
     auto store_ops = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < PER; u++) {
+        for (int u = 0; u < PA; u++) {
             const int e = t + 256 * u;
             const int rr = e / KCH, k = e % KCH;
             as[rr * SK + k] = k0 + k < kb ? -xa[u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PB; u++) {
+            const int e = t + 256 * u;
+            const int rr = e / KCH, k = e % KCH;
             bs[rr * SK + k] = k0 + k < kb ? xb[u] : 0.0;
         }
     };
     load_ops(0);
-    d4 acc[2][2];
+    d4 acc[NI][NJ];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < NI; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int64_t col = max(0, min(c0 + wn * 32 + 16 * j + c16, n - 1));
+        for (int j = 0; j < NJ; j++) {
+            const int64_t col = max(0, min(c0 + wn * WN + 16 * j + c16, n - 1));
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int64_t row = min(r0 + wm * 32 + 16 * i + g + 4 * q, m - 1);
+                const int64_t row = min(r0 + wm * WM + 16 * i + g + 4 * q, m - 1);
                 acc[i][j][q] = C[row * ldc + col];
             }
         }
 #pragma unroll
     for (int ch = 0; ch < NCH; ch++) {
-        if (ch) __syncthreads();  // the previous chunk's fragment reads are done
+        if (ch) __syncthreads();
         store_ops(ch * KCH);
         __syncthreads();
         if (ch + 1 < NCH) load_ops((ch + 1) * KCH);
 #pragma unroll
         for (int kk = 0; kk < KCH; kk += 4) {
-            double a[2], b[2];
+            double a[NI], b[NJ];
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                a[i] = as[(wm * 32 + 16 * i + c16) * SK + kk + g];
-                b[i] = bs[(wn * 32 + 16 * i + c16) * SK + kk + g];
-            }
+            for (int i = 0; i < NI; i++) a[i] = as[(wm * WM + 16 * i + c16) * SK + kk + g];
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int j = 0; j < NJ; j++) b[j] = bs[(wn * WN + 16 * j + c16) * SK + kk + g];
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+            for (int i = 0; i < NI; i++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
         }
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < NI; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int col = c0 + wn * 32 + 16 * j + c16;
+        for (int j = 0; j < NJ; j++) {
+            const int col = c0 + wn * WN + 16 * j + c16;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int row = r0 + wm * 32 + 16 * i + g + 4 * q;
+                const int row = r0 + wm * WM + 16 * i + g + 4 * q;
                 if (interior || (row < m && col >= 0 && col < n)) C[(int64_t)row * ldc + col] = acc[i][j][q];
             }
         }
@@ -503,7 +515,7 @@ hipError_t wy_launch(int side, int trans_t, int m, int n, int kb, const double* 
                      int64_t ldt, double* c, int64_t ldc, const WyPlan& p, double* part, double* w2,
                      hipStream_t stream) {
     constexpr int KB = 16 * KF;
-    const int tiles_m = (m + 63) / 64, tiles_n = (n + p.cshift + 63) / 64;
+    const int tiles_m = (m + WY_TM - 1) / WY_TM, tiles_n = (n + p.cshift + WY_TN - 1) / WY_TN;
     if (side == JLC_LEFT) {
         hipLaunchKernelGGL(wy_partial_left<KF>, dim3((unsigned)((n + p.cshift + 127) / 128), (unsigned)p.slices),
                            dim3(256), 0, stream, m, n, kb, v, ldv, c, ldc, part, p.span, p.cshift);
@@ -512,7 +524,7 @@ hipError_t wy_launch(int side, int trans_t, int m, int n, int kb, const double* 
         hipLaunchKernelGGL(wy_update<KF>, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, stream, m, n, kb, v,
                            ldv, w2, (int64_t)KB, c, ldc, tiles_n, p.cshift);
     } else {
-        hipLaunchKernelGGL(wy_partial_right<KF>, dim3((unsigned)tiles_m, (unsigned)p.slices), dim3(256), 0, stream,
+        hipLaunchKernelGGL(wy_partial_right<KF>, dim3((unsigned)((m + 63) / 64), (unsigned)p.slices), dim3(256), 0, stream,
                            m, n, kb, c, ldc, v, ldv, part, p.span, p.cshift);
         hipLaunchKernelGGL(wy_w2_right<KF>, dim3((unsigned)((m + WY_W2 - 1) / WY_W2)), dim3(256), 0, stream, m, kb,
                            p.slices, part, t, ldt, trans_t, w2);
