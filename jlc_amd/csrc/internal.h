@@ -50,7 +50,7 @@ struct TuneTable {
                                       // in registers; larger ones in LDS
     int qr_cqr_min_rows = 256;        // blocked QR: panels of at least this many rows (and 2 * width)
                                       // by CholeskyQR2 + reconstruction (0 = column kernels only)
-    int wy_left_wgs_per_cu = 1;       // compact-WY updates (kb <= 64, FP64): partial-W workgroups per CU,
+    int wy_left_wgs_per_cu = 2;       // compact-WY updates (kb <= 64, FP64): partial-W workgroups per CU,
     int wy_right_wgs_per_cu = 2;      // left and right side (0 = the three-GEMM sequence for that side)
 };
 TuneTable tune_snapshot();

@@ -76,7 +76,7 @@ __device__ __forceinline__ void wy_left_mfma(const double (&a)[WY_U][KF], const 
 }
 
 template <int KF>
-__global__ __launch_bounds__(256) void wy_partial_left(int m, int n, int kb, const double* __restrict__ V,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void wy_partial_left(int m, int n, int kb, const double* __restrict__ V,
                                                        int64_t ldv, const double* __restrict__ C, int64_t ldc,
                                                        double* __restrict__ P, int rows_per_slice, int cshift) {
     constexpr int KB = 16 * KF, WY_U = wy_u<KF>();
